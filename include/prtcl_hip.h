@@ -1,0 +1,220 @@
+/*
+ * prtcl_hip.h — C ABI of the MI355X-native SPH step hot path.
+ *
+ * The reference (siretty/prtcl) exposes this path as C++ classes: generated
+ * schemes behind SchemeBase (src/prtcl/schemes/scheme_base.hpp:18-64), a
+ * registry (scheme_base.hpp:66-114) and a Neighborhood
+ * (src/prtcl/util/neighborhood.hpp:23-42) over a Model's uniform and varying
+ * fields (src/prtcl/data/model.hpp:85-88, group.hpp:75-99). This header is the
+ * flat boundary those classes are re-provided over on the GPU side: opaque
+ * handles, plain pointers and sizes, int status codes, no exceptions and no
+ * torch types. Each entry point names the reference interface it replaces.
+ *
+ * Threading: a context is used from one host thread at a time. All work of a
+ * context runs on the context's own HIP stream.
+ */
+#ifndef PRTCL_HIP_H
+#define PRTCL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* The reference throws string literals ("procedure does not exist",
+ * scheme_base.hpp:37; "scheme is not registered", :91) or typed errors
+ * (src/prtcl/errors/); here every failure is a status code, and
+ * prtcl_hip_last_error() holds the message. */
+enum prtcl_hip_status {
+  PRTCL_HIP_OK = 0,
+  PRTCL_HIP_INVALID_ARGUMENT = 1,
+  PRTCL_HIP_ADMISSION_DENIED = 2, /* HBM budget exceeded */
+  PRTCL_HIP_NOT_CONVERGED = 3,    /* warning: solver hit its iteration cap */
+  PRTCL_HIP_NONFINITE = 4,        /* NaN/Inf detected in particle state */
+  PRTCL_HIP_ERROR = 5,            /* HIP runtime error */
+  PRTCL_HIP_NOT_FOUND = 6         /* unknown scene / field / procedure */
+};
+
+typedef struct prtcl_hip_ctx prtcl_hip_ctx;
+
+/* ---- groups, fields, globals, procedures -------------------------------- */
+enum prtcl_hip_group { PRTCL_HIP_BOUNDARY = 0, PRTCL_HIP_FLUID = 1 };
+
+/* Varying fields, 1:1 with the names the reference schemes bind in Load()
+ * (iisph.hpp:161-182, wkbb18.hpp:160-188, density/aiast12/symplectic_euler).
+ * Host layout is the reference's AoS layout: f32/f64 [n][3] for vectors,
+ * [n][3][3] column-major (Eigen) for the 3x3 system diagonal, [n] scalars. */
+enum prtcl_hip_field {
+  PRTCL_HIP_F_POSITION = 0,               /* "position" [3] (both groups) */
+  PRTCL_HIP_F_VELOCITY = 1,               /* "velocity" [3] */
+  PRTCL_HIP_F_ACCELERATION = 2,           /* "acceleration" [3] */
+  PRTCL_HIP_F_DENSITY = 3,                /* "density" */
+  PRTCL_HIP_F_PRESSURE = 4,               /* "pressure" */
+  PRTCL_HIP_F_MASS = 5,                   /* "mass" */
+  PRTCL_HIP_F_IISPH_HELPER_C = 6,         /* "iisph_helper_c" [3] */
+  PRTCL_HIP_F_IISPH_SOURCE_TERM = 7,      /* "iisph_source_term" */
+  PRTCL_HIP_F_IISPH_DIAGONAL_ELEMENT = 8, /* "iisph_diagonal_element" */
+  PRTCL_HIP_F_IISPH_RIGHT_HAND_SIDE = 9,  /* "iisph_right_hand_side" */
+  PRTCL_HIP_F_TIME_OF_BIRTH = 10,         /* "time_of_birth" */
+  PRTCL_HIP_F_VOLUME = 11,                /* "volume" (boundary) */
+  PRTCL_HIP_F_WKBB18_VELOCITY_DELTA = 12, /* "wkbb18_velocity_delta" [3] */
+  PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL = 13,/* "wkbb18_system_diagonal" [3][3] */
+  PRTCL_HIP_F_COUNT = 14
+};
+
+/* Global / uniform scalars read back or set (model.global and group
+ * uniforms of the reference; iisph.hpp:136-142, wkbb18.hpp:135-158,
+ * symplectic_euler max_speed). */
+enum prtcl_hip_global {
+  PRTCL_HIP_G_IISPH_APRDE = 0,       /* "iisph_aprde" (FP64 on device) */
+  PRTCL_HIP_G_IISPH_NPRDE = 1,       /* "iisph_nprde" */
+  PRTCL_HIP_G_MAXIMUM_SPEED = 2,     /* "maximum_speed" (running max) */
+  PRTCL_HIP_G_WKBB18_ITERATIONS = 3, /* fluid uniform "wkbb18_iterations" */
+  PRTCL_HIP_G_CURRENT_TIME = 4,      /* "current_time" */
+  PRTCL_HIP_G_IISPH_ITERATIONS = 5,  /* iterations of the last IISPH solve */
+  PRTCL_HIP_G_IISPH_REL_ERROR = 6,   /* aprde / nprde after the last solve */
+  PRTCL_HIP_G_COUNT = 7
+};
+
+/* One id per reference procedure (SchemeBase::RunProcedure names). */
+enum prtcl_hip_proc {
+  PRTCL_HIP_P_NSEARCH_UPDATE = 0,        /* Neighborhood::Update */
+  PRTCL_HIP_P_NSEARCH_PERMUTE = 1,       /* Neighborhood::Permute */
+  PRTCL_HIP_P_AIAST12_COMPUTE_VOLUME = 2,
+  PRTCL_HIP_P_DENSITY_COMPUTE_DENSITY = 3,
+  PRTCL_HIP_P_GRAVITY_INITIALIZE_ACCELERATION = 4,
+  PRTCL_HIP_P_VISCOSITY_ACCUMULATE_ACCELERATION = 5,
+  PRTCL_HIP_P_SE_INTEGRATE_VELOCITY_WITH_HARD_FADE = 6,
+  PRTCL_HIP_P_SE_INTEGRATE_POSITION = 7,
+  PRTCL_HIP_P_IISPH_SETUP = 8,
+  PRTCL_HIP_P_IISPH_ITERATION_PRESSURE_ACCELERATION = 9,
+  PRTCL_HIP_P_IISPH_ITERATION_PRESSURE = 10,
+  PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL = 11,
+  PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION = 12,
+  PRTCL_HIP_P_COUNT = 13
+};
+
+/* Scene description: counts, arithmetic type and the model's parameter
+ * fields (global smoothing_scale/time_step/gravity/iisph_relaxation/
+ * fade_duration, fluid rest_density/dynamic_viscosity/wkbb18_maximum_*,
+ * boundary dynamic_viscosity; example.lua:122-186) and the caller-side loop
+ * settings of example.lua:363-462. */
+typedef struct prtcl_hip_scene_desc {
+  uint32_t struct_size;   /* sizeof(prtcl_hip_scene_desc) */
+  int32_t real_type;      /* 0 = f32 (the reference's T=f32), 1 = f64 */
+  int64_t n_fluid;
+  int64_t n_boundary;
+  double radius;          /* Neighborhood::SetRadius; 0 selects 2h */
+  double smoothing_scale;
+  double time_step;
+  double fade_duration;
+  double iisph_relaxation;
+  double gravity[3];
+  double rest_density;
+  double fluid_viscosity;
+  double boundary_viscosity;
+  double iisph_tolerance;      /* 1e-3 */
+  int32_t iisph_min_iterations;/* 2 */
+  int32_t iisph_max_iterations;/* 100 */
+  double wkbb18_maximum_error;
+  int32_t wkbb18_maximum_iterations;
+  int32_t explicit_viscosity;  /* step runs viscosity.accumulate_acceleration */
+  int32_t implicit_viscosity;  /* step runs wkbb18 */
+  int32_t permute_every;       /* host-visible permute period; 0 = never */
+} prtcl_hip_scene_desc;
+
+/* Fills a descriptor with the reference script defaults. */
+void prtcl_hip_scene_desc_init(prtcl_hip_scene_desc *desc);
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (the reference has no device); owns the HIP
+ * stream, the scene table and the HBM budget. budget 0 = no limit. */
+int prtcl_hip_ctx_create(int device, uint64_t hbm_budget_bytes, prtcl_hip_ctx **out);
+int prtcl_hip_ctx_destroy(prtcl_hip_ctx *ctx);
+const char *prtcl_hip_last_error(const prtcl_hip_ctx *ctx);
+/* bytes of device memory held by scenes of this context */
+uint64_t prtcl_hip_ctx_bytes_in_use(const prtcl_hip_ctx *ctx);
+int prtcl_hip_synchronize(prtcl_hip_ctx *ctx);
+
+/* ---- scenes -------------------------------------------------------------
+ * Replaces Model::AddGroup("f","fluid") / ("b","boundary") + the schemes'
+ * Load(Model&) (iisph.hpp:134-185): allocates every field the schemes bind,
+ * zero-filled like VaryingManager's new fields (varying_field.hpp:165-167).
+ * Admission: fails with PRTCL_HIP_ADMISSION_DENIED when the scene's device
+ * footprint would exceed the context budget. */
+int prtcl_hip_scene_create(prtcl_hip_ctx *ctx, const prtcl_hip_scene_desc *desc,
+                           int32_t *scene_id);
+int prtcl_hip_scene_destroy(prtcl_hip_ctx *ctx, int32_t scene);
+/* Device bytes a scene with this description would occupy. */
+uint64_t prtcl_hip_scene_footprint(const prtcl_hip_scene_desc *desc);
+
+/* ---- field transfer ----------------------------------------------------
+ * Replaces reading/writing VaryingFieldSpan items (varying_field.hpp:25-58)
+ * of the group's field. `host` holds count values of the scene's real type
+ * (float or double) in the reference's host order and AoS layout.
+ * count must equal n * components. */
+int prtcl_hip_upload_field(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                           int32_t field, const void *host, size_t count);
+int prtcl_hip_download_field(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                             int32_t field, void *host, size_t count);
+/* Field id from the reference's field name ("position", ...); -1 if none. */
+int32_t prtcl_hip_field_id(const char *name);
+
+/* ---- globals ------------------------------------------------------------
+ * Replaces model.global:get_field(name):get()/set() (example.lua:414,418). */
+int prtcl_hip_get_global(prtcl_hip_ctx *ctx, int32_t scene, int32_t global, double *value);
+int prtcl_hip_set_global(prtcl_hip_ctx *ctx, int32_t scene, int32_t global, double value);
+
+/* ---- procedures ----------------------------------------------------------
+ * Replaces SchemeBase::RunProcedure(name, nhood) (scheme_base.hpp:33-38) and
+ * Neighborhood::Update/Permute (neighborhood.cpp:100-154). Unknown ids give
+ * PRTCL_HIP_NOT_FOUND ("procedure does not exist"). Asynchronous: returns
+ * after enqueueing, except where a global must be read back. */
+int prtcl_hip_run(prtcl_hip_ctx *ctx, int32_t scene, int32_t proc);
+/* Procedure id from "<scheme>.<procedure>" (e.g. "iisph.setup") or
+ * "neighborhood.update"/"neighborhood.permute"; -1 if none. */
+int32_t prtcl_hip_proc_id(const char *name);
+
+/* ---- fused paths -------------------------------------------------------
+ * The IISPH loop of example.lua:409-421 on the device: iterations run until
+ * it >= min_it and aprde/nprde < tol, or max_it; aprde is reduced in FP64 and
+ * the convergence decision is taken on the device. iters/rel_err may be NULL.
+ * Returns PRTCL_HIP_NOT_CONVERGED (a warning) at the cap. */
+int prtcl_hip_iisph_solve(prtcl_hip_ctx *ctx, int32_t scene, double tol, int32_t min_it,
+                          int32_t max_it, int32_t *iters, double *rel_err);
+/* k whole steps (example.lua:363-462 order) of each listed scene. Scenes are
+ * advanced together: every kernel of the step sweeps all listed scenes, so a
+ * batch of uneven scenes costs one launch per stage rather than one per
+ * scene. Per-scene results are independent of which other scenes are listed. */
+int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
+                   int32_t k_steps);
+/* Per-scene statistics of the last step: IISPH iterations, relative density
+ * error, WKBB18 iterations. */
+int prtcl_hip_last_step_info(prtcl_hip_ctx *ctx, int32_t scene, int32_t *iisph_iters,
+                             double *iisph_rel_err, int32_t *wkbb18_iters);
+
+/* ---- neighbour export ---------------------------------------------------
+ * Replaces Neighborhood::CopyNeighbors (neighborhood.cpp:156-167) for every
+ * particle of `group`: the neighbours in `neighbor_group` as CSR over the
+ * host index space. offsets has n+1 entries; with indices == NULL only the
+ * offsets are written and *total receives the entry count. Rows are sorted
+ * ascending. */
+int prtcl_hip_copy_neighbors(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                             int32_t neighbor_group, uint32_t *offsets,
+                             uint32_t *indices, uint64_t *total);
+
+/* Permutation applied by the last PRTCL_HIP_P_NSEARCH_PERMUTE for `group`:
+ * new host index i holds the particle that had host index perm[i] (gather
+ * semantics of VaryingManager::PermuteItems, varying_manager.hpp:86-102, as
+ * pinned by group.test.cpp:110-120). perm has n entries. */
+int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                               uint32_t *perm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRTCL_HIP_H */

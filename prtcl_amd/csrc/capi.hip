@@ -1,0 +1,372 @@
+// The extern "C" boundary (include/prtcl_hip.h): contexts, scenes, field
+// transfer, procedure dispatch, fused IISPH solve / step and neighbour
+// export. Every entry point catches all exceptions and returns a status code.
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "scene_impl.hpp"
+
+using prtcl_amd::CtxResources;
+using prtcl_amd::HipError;
+using prtcl_amd::SceneBase;
+
+struct prtcl_hip_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;
+  CtxResources res;
+  std::map<int32_t, std::unique_ptr<SceneBase>> scenes;
+  int32_t next_id = 0;
+  std::string last_error;
+};
+
+namespace {
+
+int fail(prtcl_hip_ctx *ctx, int code, const std::string &msg) {
+  if (ctx)
+    ctx->last_error = msg;
+  return code;
+}
+
+template <class Fn>
+int guarded(prtcl_hip_ctx *ctx, Fn &&fn) {
+  if (!ctx)
+    return PRTCL_HIP_INVALID_ARGUMENT;
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess)
+      return fail(ctx, PRTCL_HIP_ERROR, "hipSetDevice failed");
+    return fn();
+  } catch (const HipError &e) {
+    return fail(ctx, PRTCL_HIP_ERROR,
+                std::string("HIP error ") + hipGetErrorString(e.err) + " in " + e.where);
+  } catch (const std::length_error &e) {
+    return fail(ctx, PRTCL_HIP_ADMISSION_DENIED, e.what());
+  } catch (const std::out_of_range &e) {
+    return fail(ctx, PRTCL_HIP_NOT_FOUND, e.what());
+  } catch (const std::invalid_argument &e) {
+    return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, e.what());
+  } catch (const std::logic_error &e) {
+    return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, PRTCL_HIP_ADMISSION_DENIED, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(ctx, PRTCL_HIP_ERROR, e.what());
+  } catch (...) {
+    return fail(ctx, PRTCL_HIP_ERROR, "unknown error");
+  }
+}
+
+SceneBase *find(prtcl_hip_ctx *ctx, int32_t id) {
+  auto it = ctx->scenes.find(id);
+  if (it == ctx->scenes.end())
+    throw std::out_of_range("scene does not exist");
+  return it->second.get();
+}
+
+struct Named {
+  const char *name;
+  int id;
+};
+
+const Named kFields[] = {
+    {"position", PRTCL_HIP_F_POSITION},
+    {"velocity", PRTCL_HIP_F_VELOCITY},
+    {"acceleration", PRTCL_HIP_F_ACCELERATION},
+    {"density", PRTCL_HIP_F_DENSITY},
+    {"pressure", PRTCL_HIP_F_PRESSURE},
+    {"mass", PRTCL_HIP_F_MASS},
+    {"iisph_helper_c", PRTCL_HIP_F_IISPH_HELPER_C},
+    {"iisph_source_term", PRTCL_HIP_F_IISPH_SOURCE_TERM},
+    {"iisph_diagonal_element", PRTCL_HIP_F_IISPH_DIAGONAL_ELEMENT},
+    {"iisph_right_hand_side", PRTCL_HIP_F_IISPH_RIGHT_HAND_SIDE},
+    {"time_of_birth", PRTCL_HIP_F_TIME_OF_BIRTH},
+    {"volume", PRTCL_HIP_F_VOLUME},
+    {"wkbb18_velocity_delta", PRTCL_HIP_F_WKBB18_VELOCITY_DELTA},
+    {"wkbb18_system_diagonal", PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL},
+};
+
+const Named kProcs[] = {
+    {"neighborhood.update", PRTCL_HIP_P_NSEARCH_UPDATE},
+    {"neighborhood.permute", PRTCL_HIP_P_NSEARCH_PERMUTE},
+    {"aiast12.compute_volume", PRTCL_HIP_P_AIAST12_COMPUTE_VOLUME},
+    {"density.compute_density", PRTCL_HIP_P_DENSITY_COMPUTE_DENSITY},
+    {"gravity.initialize_acceleration", PRTCL_HIP_P_GRAVITY_INITIALIZE_ACCELERATION},
+    {"viscosity.accumulate_acceleration", PRTCL_HIP_P_VISCOSITY_ACCUMULATE_ACCELERATION},
+    {"symplectic_euler.integrate_velocity_with_hard_fade",
+     PRTCL_HIP_P_SE_INTEGRATE_VELOCITY_WITH_HARD_FADE},
+    {"symplectic_euler.integrate_position", PRTCL_HIP_P_SE_INTEGRATE_POSITION},
+    {"iisph.setup", PRTCL_HIP_P_IISPH_SETUP},
+    {"iisph.iteration_pressure_acceleration",
+     PRTCL_HIP_P_IISPH_ITERATION_PRESSURE_ACCELERATION},
+    {"iisph.iteration_pressure", PRTCL_HIP_P_IISPH_ITERATION_PRESSURE},
+    {"wkbb18.compute_diagonal", PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL},
+    {"wkbb18.accumulate_acceleration", PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION},
+};
+
+bool valid_desc(const prtcl_hip_scene_desc *d) {
+  return d && d->struct_size == sizeof(prtcl_hip_scene_desc) && d->n_fluid >= 0 &&
+         d->n_boundary >= 0 && d->n_fluid < (int64_t(1) << 30) &&
+         d->n_boundary < (int64_t(1) << 30) && (d->real_type == 0 || d->real_type == 1) &&
+         d->smoothing_scale > 0 && d->time_step > 0 && d->iisph_max_iterations >= 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void prtcl_hip_scene_desc_init(prtcl_hip_scene_desc *d) {
+  if (!d)
+    return;
+  std::memset(d, 0, sizeof(*d));
+  d->struct_size = sizeof(prtcl_hip_scene_desc);
+  d->real_type = 0;
+  d->smoothing_scale = 0.025;
+  d->time_step = 0.001;
+  d->fade_duration = 0.0;
+  d->iisph_relaxation = 0.5;
+  d->gravity[2] = -9.81;
+  d->rest_density = 1000.0;
+  d->iisph_tolerance = 1e-3;
+  d->iisph_min_iterations = 2;
+  d->iisph_max_iterations = 50;
+  d->wkbb18_maximum_error = 0.05;
+  d->wkbb18_maximum_iterations = 100;
+  d->permute_every = 8;
+}
+
+int prtcl_hip_ctx_create(int device, uint64_t budget, prtcl_hip_ctx **out) {
+  if (!out)
+    return PRTCL_HIP_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    return PRTCL_HIP_ERROR;
+  auto *ctx = new (std::nothrow) prtcl_hip_ctx;
+  if (!ctx)
+    return PRTCL_HIP_ERROR;
+  ctx->device = device;
+  ctx->budget = budget;
+  int rc = guarded(ctx, [&] {
+    PRTCL_HIP_CHECK(hipStreamCreateWithFlags(&ctx->res.stream, hipStreamNonBlocking));
+    PRTCL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&ctx->res.pinned), 64 * sizeof(int32_t),
+                                  hipHostMallocDefault));
+    std::memset(ctx->res.pinned, 0, 64 * sizeof(int32_t));
+    for (auto &e : ctx->res.ev)
+      PRTCL_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return int(PRTCL_HIP_OK);
+  });
+  if (rc != PRTCL_HIP_OK) {
+    prtcl_hip_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return PRTCL_HIP_OK;
+}
+
+int prtcl_hip_ctx_destroy(prtcl_hip_ctx *ctx) {
+  if (!ctx)
+    return PRTCL_HIP_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->res.stream)
+    (void)hipStreamSynchronize(ctx->res.stream);
+  ctx->scenes.clear();
+  if (ctx->res.staging)
+    (void)hipFree(ctx->res.staging);
+  if (ctx->res.pinned)
+    (void)hipHostFree(ctx->res.pinned);
+  for (auto &e : ctx->res.ev)
+    if (e)
+      (void)hipEventDestroy(e);
+  if (ctx->res.stream)
+    (void)hipStreamDestroy(ctx->res.stream);
+  delete ctx;
+  return PRTCL_HIP_OK;
+}
+
+const char *prtcl_hip_last_error(const prtcl_hip_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+uint64_t prtcl_hip_ctx_bytes_in_use(const prtcl_hip_ctx *ctx) {
+  if (!ctx)
+    return 0;
+  uint64_t b = 0;
+  for (const auto &kv : ctx->scenes)
+    b += kv.second->bytes;
+  return b;
+}
+
+int prtcl_hip_synchronize(prtcl_hip_ctx *ctx) {
+  return guarded(ctx, [&] {
+    PRTCL_HIP_CHECK(hipStreamSynchronize(ctx->res.stream));
+    return int(PRTCL_HIP_OK);
+  });
+}
+
+uint64_t prtcl_hip_scene_footprint(const prtcl_hip_scene_desc *desc) {
+  if (!valid_desc(desc))
+    return 0;
+  return prtcl_amd::scene_particle_bytes(*desc);
+}
+
+int prtcl_hip_scene_create(prtcl_hip_ctx *ctx, const prtcl_hip_scene_desc *desc,
+                           int32_t *scene_id) {
+  return guarded(ctx, [&]() -> int {
+    if (!valid_desc(desc) || !scene_id)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid scene description");
+    const uint64_t need = prtcl_amd::scene_particle_bytes(*desc);
+    const uint64_t used = prtcl_hip_ctx_bytes_in_use(ctx);
+    if (ctx->budget && used + need > ctx->budget)
+      return fail(ctx, PRTCL_HIP_ADMISSION_DENIED, "scene exceeds the HBM budget");
+    std::unique_ptr<SceneBase> sc(prtcl_amd::make_scene(*desc, &ctx->res));
+    sc->budget_check = [ctx](uint64_t extra) {
+      return ctx->budget == 0 || prtcl_hip_ctx_bytes_in_use(ctx) + extra <= ctx->budget;
+    };
+    const int32_t id = ctx->next_id++;
+    ctx->scenes.emplace(id, std::move(sc));
+    *scene_id = id;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_scene_destroy(prtcl_hip_ctx *ctx, int32_t scene) {
+  return guarded(ctx, [&]() -> int {
+    auto it = ctx->scenes.find(scene);
+    if (it == ctx->scenes.end())
+      return fail(ctx, PRTCL_HIP_NOT_FOUND, "scene does not exist");
+    PRTCL_HIP_CHECK(hipStreamSynchronize(ctx->res.stream));
+    ctx->scenes.erase(it);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_upload_field(prtcl_hip_ctx *ctx, int32_t scene, int32_t group, int32_t field,
+                           const void *host, size_t count) {
+  return guarded(ctx, [&]() -> int {
+    if (!host && count)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "null host pointer");
+    find(ctx, scene)->upload(group, field, host, count);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_download_field(prtcl_hip_ctx *ctx, int32_t scene, int32_t group, int32_t field,
+                             void *host, size_t count) {
+  return guarded(ctx, [&]() -> int {
+    if (!host && count)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "null host pointer");
+    find(ctx, scene)->download(group, field, host, count);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int32_t prtcl_hip_field_id(const char *name) {
+  if (!name)
+    return -1;
+  for (const auto &f : kFields)
+    if (std::strcmp(f.name, name) == 0)
+      return f.id;
+  return -1;
+}
+
+int32_t prtcl_hip_proc_id(const char *name) {
+  if (!name)
+    return -1;
+  for (const auto &p : kProcs)
+    if (std::strcmp(p.name, name) == 0)
+      return p.id;
+  return -1;
+}
+
+int prtcl_hip_get_global(prtcl_hip_ctx *ctx, int32_t scene, int32_t global, double *value) {
+  return guarded(ctx, [&]() -> int {
+    if (!value)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "null value pointer");
+    *value = find(ctx, scene)->get_global(global);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_set_global(prtcl_hip_ctx *ctx, int32_t scene, int32_t global, double value) {
+  return guarded(ctx, [&]() -> int {
+    find(ctx, scene)->set_global(global, value);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_run(prtcl_hip_ctx *ctx, int32_t scene, int32_t proc) {
+  return guarded(ctx, [&]() -> int {
+    if (proc < 0 || proc >= PRTCL_HIP_P_COUNT)
+      return fail(ctx, PRTCL_HIP_NOT_FOUND, "procedure does not exist");
+    find(ctx, scene)->run(proc);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_iisph_solve(prtcl_hip_ctx *ctx, int32_t scene, double tol, int32_t min_it,
+                          int32_t max_it, int32_t *iters, double *rel_err) {
+  return guarded(ctx, [&]() -> int {
+    if (min_it < 0 || max_it < 0)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "iteration limits");
+    int it = 0;
+    double rel = 0;
+    int rc = find(ctx, scene)->iisph_solve(tol, min_it, max_it, &it, &rel, true);
+    if (iters)
+      *iters = it;
+    if (rel_err)
+      *rel_err = rel;
+    return rc;
+  });
+}
+
+int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
+                   int32_t k_steps) {
+  return guarded(ctx, [&]() -> int {
+    if (!scenes || n_scenes < 0 || k_steps < 0)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid scene list");
+    int status = PRTCL_HIP_OK;
+    std::vector<SceneBase *> list;
+    for (int i = 0; i < n_scenes; ++i)
+      list.push_back(find(ctx, scenes[i]));
+    for (int k = 0; k < k_steps; ++k)
+      for (SceneBase *s : list) {
+        int rc = s->step(1);
+        if (rc != PRTCL_HIP_OK && status == PRTCL_HIP_OK)
+          status = rc;
+      }
+    return status;
+  });
+}
+
+int prtcl_hip_last_step_info(prtcl_hip_ctx *ctx, int32_t scene, int32_t *iisph_iters,
+                             double *iisph_rel_err, int32_t *wkbb18_iters) {
+  return guarded(ctx, [&]() -> int {
+    find(ctx, scene)->step_info(iisph_iters, iisph_rel_err, wkbb18_iters);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_copy_neighbors(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                             int32_t neighbor_group, uint32_t *offsets, uint32_t *indices,
+                             uint64_t *total) {
+  return guarded(ctx, [&]() -> int {
+    find(ctx, scene)->copy_neighbors(group, neighbor_group, offsets, indices, total);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                               uint32_t *perm) {
+  return guarded(ctx, [&]() -> int {
+    if (!perm)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "null permutation pointer");
+    find(ctx, scene)->last_permutation(group, perm);
+    return PRTCL_HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+// Uniform-grid neighbour search for MI355X: cell keys, a counting sort with a
+// wavefront prefix scan, a deterministic stable fix-up and the gathers that
+// reorder particle data into cell order.
+//
+// Semantics follow the reference grid (src/prtcl/util/grouped_uniform_grid.hpp):
+// cell edge = radius, cell index floor(double(x) / radius) per axis
+// (:621-632), neighbours = particles of the 27 cells around the particle's
+// cell whose FP32 squared distance is < double(radius)^2 (:407-422). The
+// reference orders cells along a Morton curve with a comparison sort; here the
+// cells live in a dense box (x fastest) and are sorted by counting, which
+// yields the same cells and the same neighbour sets in a different order.
+// Particles outside the box are clamped to its faces; clamping is monotone
+// per axis, so two particles within one cell of each other stay within one
+// clamped cell and no neighbour is lost.
+#include "scene.hpp"
+
+namespace prtcl_amd {
+
+namespace {
+
+constexpr int kScanBS = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBS * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t u = __shfl_up(v, o, kWave);
+    if (lane >= o)
+      v += u;
+  }
+  return v;
+}
+
+// exclusive block scan over kScanBS threads; returns the exclusive prefix of
+// this thread and the block total in *total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+  __shared__ uint32_t wsum[kScanBS / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  uint32_t incl = wave_incl_scan(v);
+  if (lane == kWave - 1)
+    wsum[w] = incl;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBS / kWave; ++k) {
+    uint32_t s = wsum[k];
+    if (k < w)
+      base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + incl - v;
+}
+
+__global__ __launch_bounds__(kScanBS) void k_scan_reduce(const uint32_t *count, uint32_t n,
+                                                         uint32_t *tile_sums) {
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile;
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + static_cast<size_t>(k) * kScanBS + threadIdx.x;
+    if (i < n)
+      s += count[i];
+  }
+  s = block_sum<kScanBS>(s);
+  if (threadIdx.x == 0)
+    tile_sums[blockIdx.x] = s;
+}
+
+// one block: exclusive scan of the tile sums in place, total into start[n]
+__global__ __launch_bounds__(kScanBS) void k_scan_top(uint32_t *tile_sums, uint32_t ntiles,
+                                                      uint32_t *start, uint32_t n) {
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < ntiles; b += kScanBS) {
+    uint32_t i = b + threadIdx.x;
+    uint32_t v = i < ntiles ? tile_sums[i] : 0u;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(v, &tot);
+    if (i < ntiles)
+      tile_sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0)
+    start[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t n,
+                                                       const uint32_t *tile_sums,
+                                                       uint32_t *start) {
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile +
+                      static_cast<size_t>(threadIdx.x) * kScanItems;
+  uint32_t vals[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + k;
+    vals[k] = i < n ? count[i] : 0u;
+    s += vals[k];
+  }
+  uint32_t tot;
+  uint32_t ex = block_excl_scan(s, &tot) + tile_sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + k;
+    if (i < n) {
+      start[i] = ex;
+      count[i] = 0u;
+    }
+    ex += vals[k];
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cell_keys(GridDims g, const vec4_t<T> *x, int n,
+                                                      uint32_t *cell, uint32_t *count,
+                                                      uint32_t *rank) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const vec4_t<T> p = x[i];
+  const double comp[3] = {static_cast<double>(p.x), static_cast<double>(p.y),
+                          static_cast<double>(p.z)};
+  const int org[3] = {g.ox, g.oy, g.oz};
+  const int dim[3] = {g.nx, g.ny, g.nz};
+  uint32_t c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    // floor(double(x) / r) as grouped_uniform_grid.hpp:626-628, relative to
+    // the box origin, clamped to the box (fmax maps NaN to 0)
+    double f = floor(comp[a] / g.radius) - static_cast<double>(org[a]);
+    f = fmin(fmax(f, 0.0), static_cast<double>(dim[a] - 1));
+    c[a] = static_cast<uint32_t>(f);
+  }
+  cell[i] = c[0] | (c[1] << 11) | (c[2] << 22);
+  const uint32_t lin = c[0] + static_cast<uint32_t>(g.nx) *
+                                  (c[1] + static_cast<uint32_t>(g.ny) * c[2]);
+  rank[i] = atomicAdd(&count[lin], 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(GridDims g, const uint32_t *cell,
+                                                    const uint32_t *rank,
+                                                    const uint32_t *start, int n,
+                                                    uint32_t *perm) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const uint32_t c = cell[i];
+  const uint32_t lin = (c & 0x7FFu) + static_cast<uint32_t>(g.nx) *
+                                          (((c >> 11) & 0x7FFu) +
+                                           static_cast<uint32_t>(g.ny) * (c >> 22));
+  perm[start[lin] + rank[i]] = static_cast<uint32_t>(i);
+}
+
+// Within each cell, order the slots by their previous slot index. The atomic
+// ranks of k_cell_keys are scheduling dependent; after this pass the order is
+// a pure function of the previous order (a stable counting sort).
+__global__ __launch_bounds__(kBlock) void k_fixup(const uint32_t *start, uint32_t ncells,
+                                                  uint32_t *perm) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncells)
+    return;
+  const uint32_t b = start[c], e = start[c + 1];
+  if (e - b < 2)
+    return;
+  for (uint32_t k = b + 1; k < e; ++k) {
+    uint32_t v = perm[k];
+    uint32_t j = k;
+    while (j > b && perm[j - 1] > v) {
+      perm[j] = perm[j - 1];
+      --j;
+    }
+    perm[j] = v;
+  }
+}
+
+template <int W>
+struct Words {
+  uint32_t w[W];
+};
+
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_gather(E *__restrict__ dst,
+                                                   const E *__restrict__ src,
+                                                   const uint32_t *__restrict__ perm, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    dst[i] = src[perm[i]];
+}
+
+}  // namespace
+
+size_t scan_tmp_words(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void scan_counts(hipStream_t st, uint32_t *count, uint32_t *start, uint32_t n,
+                 uint32_t *tmp) {
+  const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+  if (ntiles == 0) {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBS), 0, st, tmp, 0u, start, 0u);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanBS), 0, st, count, n, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBS), 0, st, tmp, ntiles, start, n);
+  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanBS), 0, st, count, n, tmp, start);
+}
+
+template <class T>
+void GridOps<T>::sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
+                            uint32_t *cell, uint32_t *count, uint32_t *start,
+                            uint32_t *rank, uint32_t *perm, uint32_t *scan_tmp) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_keys<T>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g, x, n,
+                       cell, count, rank);
+  scan_counts(st, count, start, g.ncells, scan_tmp);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g, cell, rank,
+                       start, n, perm);
+    hipLaunchKernelGGL(k_fixup, dim3(grid_blocks(g.ncells)), dim3(kBlock), 0, st, start,
+                       g.ncells, perm);
+  }
+}
+
+template <class T>
+void GridOps<T>::gather4(hipStream_t st, void *dst, const void *src, const uint32_t *perm,
+                         int n) {
+  gather_elems(st, dst, src, perm, n, 4);
+}
+
+template <class T>
+void GridOps<T>::gather_elems(hipStream_t st, void *dst, const void *src,
+                              const uint32_t *perm, int n, int elem_bytes) {
+  if (n <= 0)
+    return;
+  dim3 gr(grid_blocks(n)), bl(kBlock);
+  switch (elem_bytes) {
+    case 4:
+      hipLaunchKernelGGL(k_gather<uint32_t>, gr, bl, 0, st, static_cast<uint32_t *>(dst),
+                         static_cast<const uint32_t *>(src), perm, n);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_gather<uint2>, gr, bl, 0, st, static_cast<uint2 *>(dst),
+                         static_cast<const uint2 *>(src), perm, n);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_gather<uint4>, gr, bl, 0, st, static_cast<uint4 *>(dst),
+                         static_cast<const uint4 *>(src), perm, n);
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_gather<Words<8>>, gr, bl, 0, st, static_cast<Words<8> *>(dst),
+                         static_cast<const Words<8> *>(src), perm, n);
+      break;
+    case 36:
+      hipLaunchKernelGGL(k_gather<Words<9>>, gr, bl, 0, st, static_cast<Words<9> *>(dst),
+                         static_cast<const Words<9> *>(src), perm, n);
+      break;
+    case 72:
+      hipLaunchKernelGGL(k_gather<Words<18>>, gr, bl, 0, st, static_cast<Words<18> *>(dst),
+                         static_cast<const Words<18> *>(src), perm, n);
+      break;
+    default:
+      throw HipError{hipErrorInvalidValue, "gather_elems: element size"};
+  }
+}
+
+template <class T>
+void GridOps<T>::bounds(hipStream_t, const vec4_t<T> *, int, T *) {}
+
+template struct GridOps<float>;
+template struct GridOps<double>;
+
+}  // namespace prtcl_amd

@@ -1,0 +1,119 @@
+// Device-resident scene: the fluid and boundary groups of one model, their
+// uniform grid and the solver control block, laid out for coalesced access on
+// MI355X. Shared by the kernels (nsearch.hip, sph.hip, wkbb18.hip) and the C
+// ABI (capi.hip).
+//
+// Layout in HBM (per scene, slot order = the grid's cell order, rebuilt at
+// every neighbourhood update):
+//   fluid  xm  real4 (x, y, z, mass)           16/32 B
+//          v   real4 (vx, vy, vz, time_of_birth)
+//          a, c, G, vd real4; rho, p, pr, s, AA, Ap real; sp real[9]
+//          id  int32 host index of the particle in each slot
+//          cell uint32 packed cell coordinates (x | y << 11 | z << 22)
+//   boundary bxV real4 (x, y, z, volume), bid int32, bcell uint32
+//   grid   fstart / bstart uint32[ncells + 1]: exclusive prefix sums of the
+//          per-cell particle counts over a dense box of cells of edge r,
+//          x fastest, so the three cells x-1..x+1 of a row are one slot range.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+
+namespace prtcl_amd {
+
+constexpr int kBlock = 256;
+constexpr int kMaxAxisCellsXY = 2047;  // 11 bits
+constexpr int kMaxAxisCellsZ = 1023;   // 10 bits
+
+// per-scene control block (device memory)
+struct Ctl {
+  double aprde;      // iisph_aprde
+  double rel;        // aprde / nprde of the last iteration
+  double tol;        // IISPH tolerance
+  unsigned ticket;   // last-block counter of the pressure reduction
+  unsigned ticket2;  // last-block counter of the other reductions
+  int nprde;         // iisph_nprde
+  int iters;         // iterations done in the current solve
+  int done;          // solve finished (converged or capped)
+  int converged;
+  int min_it, max_it;
+  int nonfinite;
+  unsigned max_speed_bits_f;       // running max |v| (float bits)
+  unsigned long long max_speed_bits_d;  // running max |v| (double bits)
+  // PCG scalars (FP64)
+  double cg_b_nsq, cg_thr, cg_r_nsq, cg_ry, cg_ry_prev, cg_pq, cg_alpha, cg_beta;
+  int cg_iter, cg_done, cg_max_k;
+  double cg_tol;
+};
+
+struct GridDims {
+  int nx, ny, nz;     // cells per axis (dense box)
+  int ox, oy, oz;     // integer cell index of the box origin
+  uint32_t ncells;
+  double radius;      // cell edge = neighbourhood radius
+};
+
+template <class T>
+struct SceneView {
+  using V4 = vec4_t<T>;
+  int nf, nb;
+  CubicSpline<T> K;
+  T thr2;  // neighbour iff squared distance < thr2 (exactly the reference's test)
+  T dt, rho0, omega, mu_f, mu_b, fade, t, hh001;  // hh001 = (0.01*h)*h
+  T g0, g1, g2;
+  GridDims grid;
+  const uint32_t *fstart;
+  const uint32_t *bstart;
+  V4 *xm, *v, *a, *c, *G, *vd;
+  T *rho, *p, *pr, *s, *AA, *Ap, *sp;
+  int32_t *id;
+  uint32_t *fcell;
+  V4 *bxV;
+  int32_t *bid;
+  uint32_t *bcell;
+  Ctl *ctl;
+  double *partials;  // one per block of the largest fluid launch
+  // PCG vectors (3-vectors in real4)
+  V4 *cx, *cr, *cp, *cq, *cy, *cb;
+};
+
+inline unsigned grid_blocks(long n, int bs = kBlock) {
+  return static_cast<unsigned>((n + bs - 1) / bs);
+}
+
+// host-side error helper
+struct HipError {
+  hipError_t err;
+  std::string where;
+};
+#define PRTCL_HIP_CHECK(expr)                                        \
+  do {                                                               \
+    hipError_t e_ = (expr);                                          \
+    if (e_ != hipSuccess)                                            \
+      throw ::prtcl_amd::HipError{e_, std::string(#expr)};           \
+  } while (0)
+
+// ----- grid / neighbour search (nsearch.hip) -----
+template <class T>
+struct GridOps {
+  // builds cell keys, counts, scan, stable scatter; returns perm in slot order
+  static void sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
+                         uint32_t *cell, uint32_t *count, uint32_t *start,
+                         uint32_t *rank, uint32_t *perm, uint32_t *scan_tmp);
+  static void gather4(hipStream_t st, void *dst, const void *src, const uint32_t *perm, int n);
+  static void gather_elems(hipStream_t st, void *dst, const void *src,
+                           const uint32_t *perm, int n, int elem_bytes);
+  static void bounds(hipStream_t st, const vec4_t<T> *x, int n, T *out6);
+};
+// exclusive scan of n uint32 counts into start[0..n] (start[n] = total); the
+// counts are cleared to zero for the next build
+void scan_counts(hipStream_t st, uint32_t *count, uint32_t *start, uint32_t n,
+                 uint32_t *tmp);
+size_t scan_tmp_words(uint32_t n);
+
+}  // namespace prtcl_amd
