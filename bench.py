@@ -1,0 +1,179 @@
+"""Benchmark of the SPH step hot path on one MI355X.
+
+Workload (default): the 1M-particle IISPH dam break (100^3 fluid particles at
+0.01 m in a closed 4x2x2 m box of 400,002 boundary particles, dt 5e-4,
+omega 0.5, stop at 0.1% average density error, <= 100 iterations). A step is
+the reference's whole step (neighbour search, density, gravity, integrate
+velocity, IISPH setup + solve, integrate velocity, integrate position) on
+device-resident data. Prints ONE JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes per launch of each stage (compulsory traffic: each field
+# read or written once per pass; neighbour re-reads served on chip), per fluid
+# particle and per active boundary particle (boundary particle with a fluid
+# neighbour), FP32 fields.
+STAGE_BYTES = {
+    "iisph_pressure_acceleration": (36, 16),  # x, p, rho, m -> a
+    "iisph_pressure": (52, 16),               # x, a, m, rho, s, AA, p -> Ap, p
+    "density": (20, 16),                      # x, m -> rho
+    "iisph_setup": (44, 16),                  # two passes: 32 + 56 B, per launch mean
+    "integrate_position": (36, 0),            # x, v -> x
+    "nsearch_sort": (96, 0),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="dambreak_1m",
+                    choices=["dambreak_1m", "dambreak_8k"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target length of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from prtcl_amd import hip, scenes
+
+    sc = scenes.config1() if args.workload == "dambreak_1m" else scenes.config0()
+    sc.permute_every = 8
+    ctx = hip.Context(0)
+    g = hip.HipScene(ctx, sc)
+    g.prepare()
+    for _ in range(args.warmup):
+        g.step(1)
+    ctx.synchronize()
+
+    # active boundary particles (with at least one fluid neighbour)
+    off, _ = g.neighbors(hip.BOUNDARY, hip.FLUID)
+    nb_active = int(np.count_nonzero(np.diff(off)))
+
+    ctx.profile(True)
+    iters, lat = [], []
+    t_start = time.perf_counter()
+    prev = t_start
+    for _ in range(args.steps):
+        g.step(1)
+        iters.append(g.step_info()["iisph_iterations"])
+        now = time.perf_counter()
+        lat.append(now - prev)
+        prev = now
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    prof = ctx.profile_read()
+    ctx.profile(False)
+
+    nf = sc.n_fluid
+    value = nf * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    # dominant kernel by total time
+    stages = {k: v for k, v in prof.items() if v[1] > 0}
+    dom = max(stages, key=lambda k: stages[k][0])
+    dom_ms, dom_n = stages[dom]
+    per_f, per_b = STAGE_BYTES.get(dom, (0, 0))
+    bytes_launch = per_f * nf + per_b * nb_active
+    avg_s = dom_ms / dom_n / 1e3
+    achieved = bytes_launch / avg_s / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+        "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5),
+        "launches": dom_n,
+    }
+    # whole-step roofline: algorithmic bytes per step (332 + 88 k per fluid
+    # particle + 16 (4 + 2k) per active boundary particle)
+    k_mean = float(np.mean(iters))
+    step_bytes = (332 + 88 * k_mean) * nf + 16 * (4 + 2 * k_mean) * nb_active
+    step_frac = step_bytes / (elapsed / args.steps) / (HBM_PEAK_GBS * 1e9)
+
+    out = {
+        "metric": "IISPH particle-steps/s at 1M particles; batched-scene p50/p99 step latency",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": sc.name, "fluid_particles": nf, "boundary_particles": sc.n_boundary,
+            "active_boundary_particles": nb_active, "spacing_m": sc.spacing,
+            "time_step_s": sc.time_step, "iisph_tolerance": sc.iisph_tolerance,
+            "iisph_max_iterations": sc.iisph_max_iterations, "reductions": "f64",
+        },
+        "iisph_iterations_mean": round(k_mean, 3),
+        "p50_step_ms": round(1e3 * float(np.percentile(lat, 50)), 4),
+        "p99_step_ms": round(1e3 * float(np.percentile(lat, 99)), 4),
+        "step_roofline_frac": round(step_frac, 5),
+        "roofline": roofline,
+        "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items()},
+    }
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc, g, args.cpu_seconds)
+
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(sc, g, seconds):
+    """The oracle (CPU/OpenMP restatement of the reference scheme) timed on
+    this host for a bounded sample of the same workload: it starts from the
+    GPU's state at the end of the timed region and runs whole steps until
+    `seconds` of wall time have passed."""
+    from oracle import pyoracle
+    st = sc
+    o = pyoracle.OracleScene(st, fp64_reductions=False)
+    o.set("fluid", "position", g.download("fluid", "position"))
+    o.set("fluid", "velocity", g.download("fluid", "velocity"))
+    o.set("boundary", "position", g.download("boundary", "position"))
+    o.set_params(permute_every=8)
+    o.L.oracle_set_time(o.h, g.get_global("current_time"))
+    o.prepare()
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step()
+        steps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    return {
+        "value": round(sc.n_fluid * steps / el, 1), "unit": "particle-steps/s",
+        "cores": pyoracle.lib().oracle_num_threads(), "kind": "port",
+        "sample": f"{steps} whole steps of {sc.name} from the GPU state after the timed "
+                  f"region ({el:.1f} s wall)",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -124,6 +124,9 @@ typedef struct prtcl_hip_scene_desc {
   int32_t explicit_viscosity;  /* step runs viscosity.accumulate_acceleration */
   int32_t implicit_viscosity;  /* step runs wkbb18 */
   int32_t permute_every;       /* host-visible permute period; 0 = never */
+  int32_t step_mode;           /* prtcl_hip_step kernels: 0 = fused passes over a
+                                  per-step neighbour list (default), 1 = one
+                                  kernel per reference procedure */
 } prtcl_hip_scene_desc;
 
 /* Fills a descriptor with the reference script defaults. */
@@ -212,6 +215,16 @@ int prtcl_hip_copy_neighbors(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
  * pinned by group.test.cpp:110-120). perm has n entries. */
 int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
                                uint32_t *perm);
+
+/* ---- stage timing -------------------------------------------------------
+ * No reference counterpart (the reference logs host-side timestamps around
+ * each RunProcedure, module_schemes.cpp:30-37). When enabled, hipEvents are
+ * recorded on the context stream around each kernel stage of
+ * prtcl_hip_step(); launches of solver iterations that exited at the
+ * convergence flag are not counted. Enabling clears the accumulators. */
+int prtcl_hip_profile_enable(prtcl_hip_ctx *ctx, int on);
+int prtcl_hip_profile_read(prtcl_hip_ctx *ctx, int32_t stage, double *total_ms,
+                           uint64_t *launches, const char **name);
 
 #ifdef __cplusplus
 }

@@ -369,4 +369,30 @@ int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
   });
 }
 
+int prtcl_hip_profile_enable(prtcl_hip_ctx *ctx, int on) {
+  return guarded(ctx, [&]() -> int {
+    PRTCL_HIP_CHECK(hipStreamSynchronize(ctx->res.stream));
+    ctx->res.prof.reset();
+    ctx->res.prof.on = on != 0;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_profile_read(prtcl_hip_ctx *ctx, int32_t stage, double *total_ms,
+                           uint64_t *launches, const char **name) {
+  return guarded(ctx, [&]() -> int {
+    if (stage < 0 || stage >= prtcl_amd::kStageCount)
+      return fail(ctx, PRTCL_HIP_NOT_FOUND, "stage does not exist");
+    PRTCL_HIP_CHECK(hipStreamSynchronize(ctx->res.stream));
+    ctx->res.prof.resolve();
+    if (total_ms)
+      *total_ms = ctx->res.prof.ms[stage];
+    if (launches)
+      *launches = ctx->res.prof.launches[stage];
+    if (name)
+      *name = prtcl_amd::stage_name(stage);
+    return PRTCL_HIP_OK;
+  });
+}
+
 }  // extern "C"

@@ -29,6 +29,7 @@ namespace prtcl_amd {
 constexpr int kBlock = 256;
 constexpr int kMaxAxisCellsXY = 2047;  // 11 bits
 constexpr int kMaxAxisCellsZ = 1023;   // 10 bits
+constexpr uint32_t kNbrCap = 48;       // fluid neighbours kept per particle and step
 
 // per-scene control block (device memory)
 struct Ctl {

@@ -9,6 +9,7 @@
 
 #include "scene_impl.hpp"
 #include "sph.hpp"
+#include "fused.hpp"
 
 namespace prtcl_amd {
 
@@ -22,6 +23,63 @@ void *CtxResources::ensure_staging(size_t bytes) {
     staging_bytes = bytes;
   }
   return staging;
+}
+
+const char *stage_name(int s) {
+  static const char *names[kStageCount] = {
+      "nsearch_sort",      "density",       "gravity_integrate_v", "viscosity",
+      "iisph_setup",       "iisph_pressure_acceleration", "iisph_pressure",
+      "integrate_position", "wkbb18_diagonal", "pcg_spmv", "pcg_vector"};
+  return (s >= 0 && s < kStageCount) ? names[s] : "?";
+}
+
+hipEvent_t Profiler::get() {
+  if (pool.empty()) {
+    hipEvent_t e;
+    PRTCL_HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = pool.back();
+  pool.pop_back();
+  return e;
+}
+
+int Profiler::begin(hipStream_t st, int stage, int tag) {
+  if (!on)
+    return -1;
+  Rec r{stage, tag, get(), get()};
+  PRTCL_HIP_CHECK(hipEventRecord(r.a, st));
+  recs.push_back(r);
+  return static_cast<int>(recs.size()) - 1;
+}
+
+void Profiler::end(hipStream_t st, int rec) {
+  if (!on || rec < 0)
+    return;
+  PRTCL_HIP_CHECK(hipEventRecord(recs[rec].b, st));
+}
+
+void Profiler::resolve(int valid_iters) {
+  for (auto &r : recs) {
+    PRTCL_HIP_CHECK(hipEventSynchronize(r.b));
+    if (r.tag < 0 || r.tag < valid_iters) {
+      float t = 0;
+      PRTCL_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.stage] += t;
+      launches[r.stage] += 1;
+    }
+    pool.push_back(r.a);
+    pool.push_back(r.b);
+  }
+  recs.clear();
+}
+
+void Profiler::reset() {
+  resolve();
+  for (int k = 0; k < kStageCount; ++k) {
+    ms[k] = 0;
+    launches[k] = 0;
+  }
 }
 
 namespace {
@@ -486,6 +544,8 @@ class SceneImpl final : public SceneBase {
     return S;
   }
 
+  NbrList nbr_list() const { return NbrList{P_.nidx, P_.ncnt, kNbrCap}; }
+
   void require_grid() {
     if (!grid_built_)
       throw std::logic_error("neighborhood not updated: run the nsearch update first");
@@ -541,18 +601,35 @@ class SceneImpl final : public SceneBase {
   // their first instruction.
   int iisph_solve(double tol, int min_it, int max_it, int *iters, double *rel,
                   bool wait) override {
+    return solve_impl(tol, min_it, max_it, iters, rel, wait, false);
+  }
+
+  int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool wait,
+                 bool fused) {
     require_grid();
     hipStream_t st = res_->stream;
     ctl_op(kCtlBeginSolve, tol, min_it, max_it);
     const SceneView<T> S = view();
+    const NbrList L = nbr_list();
     int enq = 0, chunk = 0;
     int first = std::max(min_it, std::min(predicted_iters_, max_it));
     bool done = (nf_ == 0);
     while (!done && enq < max_it) {
       const int n = std::min(chunk == 0 ? first : 2, max_it - enq);
+      Profiler &pf = res_->prof;
       for (int k = 0; k < n; ++k) {
-        SphOps<T>::iisph_accel(st, S, true);
-        SphOps<T>::iisph_pressure(st, S, true);
+        int r = pf.begin(st, kStagePressureAccel, enq + k);
+        if (fused)
+          FusedOps<T>::accel(st, S, L);
+        else
+          SphOps<T>::iisph_accel(st, S, true);
+        pf.end(st, r);
+        r = pf.begin(st, kStagePressure, enq + k);
+        if (fused)
+          FusedOps<T>::pressure(st, S, L);
+        else
+          SphOps<T>::iisph_pressure(st, S, true);
+        pf.end(st, r);
       }
       enq += n;
       int32_t *slot = res_->pinned + 8 * (chunk & 1);
@@ -569,6 +646,8 @@ class SceneImpl final : public SceneBase {
     if (!wait && !iters && !rel)
       return PRTCL_HIP_OK;
     Ctl c = read_ctl();
+    if (res_->prof.on)
+      res_->prof.resolve(c.iters);
     predicted_iters_ = std::max(1, c.iters);
     last_iisph_iters_ = c.iters;
     last_rel_ = c.rel;
@@ -590,7 +669,10 @@ class SceneImpl final : public SceneBase {
         SphOps<T>::volume(st, view());  // boundary volumes follow the boundary grid
       }
       steps_since_permute_ += 1;
+      Profiler &pf = res_->prof;
+      int r = pf.begin(st, kStageSort);
       build_fluid_grid(false);
+      pf.end(st, r);
       if (desc.permute_every > 0 && steps_since_permute_ >= desc.permute_every) {
         if (nf_ > 0)
           hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nf_)), dim3(kBlock), 0, st, P_.id, nf_);
@@ -598,25 +680,57 @@ class SceneImpl final : public SceneBase {
           hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nb_)), dim3(kBlock), 0, st, P_.bid, nb_);
         steps_since_permute_ = 0;
       }
-      const SceneView<T> S = view();
-      SphOps<T>::density(st, S);
-      SphOps<T>::gravity(st, S);
-      if (desc.explicit_viscosity)
-        SphOps<T>::viscosity(st, S);
-      SphOps<T>::integrate_v(st, S);
-      SphOps<T>::iisph_setup(st, S);
+      const bool fused = desc.step_mode == 0;
+      if (fused) {
+        // one neighbour pass for density + list + IISPH setup (three with
+        // explicit viscosity); the integrated velocity lands in tmpV4
+        r = pf.begin(st, kStageSetup);
+        FusedOps<T>::setup(st, view(), nbr_list(), desc.explicit_viscosity != 0, P_.tmpV4);
+        pf.end(st, r);
+        std::swap(P_.v, P_.tmpV4);
+      } else {
+        const SceneView<T> S = view();
+        r = pf.begin(st, kStageDensity);
+        SphOps<T>::density(st, S);
+        pf.end(st, r);
+        r = pf.begin(st, kStageGravityIntegrateV);
+        SphOps<T>::gravity(st, S);
+        pf.end(st, r);
+        if (desc.explicit_viscosity) {
+          r = pf.begin(st, kStageViscosity);
+          SphOps<T>::viscosity(st, S);
+          pf.end(st, r);
+        }
+        r = pf.begin(st, kStageGravityIntegrateV);
+        SphOps<T>::integrate_v(st, S);
+        pf.end(st, r);
+        r = pf.begin(st, kStageSetup);
+        SphOps<T>::iisph_setup(st, S);
+        pf.end(st, r);
+      }
       ctl_op(kCtlSetNprde, 0, nf_);
       int it = 0;
       double rel = 0;
-      int rc = iisph_solve(desc.iisph_tolerance, desc.iisph_min_iterations,
-                           desc.iisph_max_iterations, &it, &rel, true);
+      int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
+                          desc.iisph_max_iterations, &it, &rel, true, fused);
       if (rc != PRTCL_HIP_OK)
         status = rc;
-      SphOps<T>::integrate_v(st, S);
-      if (desc.implicit_viscosity) {
-        run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL);
+      const SceneView<T> S = view();
+      if (fused && !desc.implicit_viscosity) {
+        r = pf.begin(st, kStageIntegrateX);
+        FusedOps<T>::integrate(st, S);
+        pf.end(st, r);
+      } else {
+        r = pf.begin(st, kStageGravityIntegrateV);
+        SphOps<T>::integrate_v(st, S);
+        pf.end(st, r);
+        if (desc.implicit_viscosity) {
+          run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL);
+        }
+        r = pf.begin(st, kStageIntegrateX);
+        SphOps<T>::integrate_x(st, S);
+        pf.end(st, r);
       }
-      SphOps<T>::integrate_x(st, S);
       clock_ += static_cast<double>(static_cast<T>(desc.time_step));
       PRTCL_HIP_CHECK(hipGetLastError());
     }
@@ -679,7 +793,7 @@ class SceneImpl final : public SceneBase {
     V4 *xm, *v, *a, *c, *G, *vd, *tmpV4;
     T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp;
     int32_t *id, *tmpI;
-    uint32_t *fcell, *frank, *fperm;
+    uint32_t *fcell, *frank, *fperm, *nidx, *ncnt;
     V4 *bxV, *btmpV4;
     int32_t *bid, *btmpI;
     uint32_t *bcell, *brank, *bperm;
@@ -709,6 +823,8 @@ class SceneImpl final : public SceneBase {
     A.take(P.fcell, f);
     A.take(P.frank, f);
     A.take(P.fperm, f);
+    A.take(P.nidx, static_cast<size_t>(kNbrCap) * f);
+    A.take(P.ncnt, f);
     A.take(P.bxV, b);
     A.take(P.btmpV4, b);
     A.take(P.bid, b);
@@ -736,7 +852,8 @@ class SceneImpl final : public SceneBase {
       Arena::rebase(*q, base);
     for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
       Arena::rebase(*q, base);
-    for (uint32_t **q : {&P_.fcell, &P_.frank, &P_.fperm, &P_.bcell, &P_.brank, &P_.bperm})
+    for (uint32_t **q : {&P_.fcell, &P_.frank, &P_.fperm, &P_.nidx, &P_.ncnt, &P_.bcell,
+                         &P_.brank, &P_.bperm})
       Arena::rebase(*q, base);
     Arena::rebase(P_.partials, base);
     Arena::rebase(P_.ctl, base);

@@ -16,10 +16,38 @@
 
 namespace prtcl_amd {
 
-struct HostPinned;
+// Stage timer: hipEvents recorded around kernel groups on the context
+// stream, resolved at the next synchronisation point.
+enum Stage {
+  kStageSort = 0, kStageDensity, kStageGravityIntegrateV, kStageViscosity, kStageSetup,
+  kStagePressureAccel, kStagePressure, kStageIntegrateX, kStageWkbb18Diag, kStagePcgSpmv,
+  kStagePcgVector, kStageCount
+};
+const char *stage_name(int s);
+
+struct Profiler {
+  bool on = false;
+  struct Rec {
+    int stage;
+    int tag;  // iteration index inside a solve, -1 otherwise
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> pool;
+  std::vector<Rec> recs;
+  double ms[kStageCount] = {};
+  uint64_t launches[kStageCount] = {};
+  hipEvent_t get();
+  int begin(hipStream_t st, int stage, int tag = -1);
+  void end(hipStream_t st, int rec);
+  // accumulate finished records; records of solve iterations with tag >=
+  // valid_iters are dropped (launches that exited at the convergence flag)
+  void resolve(int valid_iters = 1 << 30);
+  void reset();
+};
 
 // Shared per-context resources a scene borrows.
 struct CtxResources {
+  Profiler prof;
   hipStream_t stream = nullptr;
   // device staging for host transfers, grown on demand
   void *staging = nullptr;

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_x(SceneView<T> S) {
   }
   speed = block_max<kBlock>(speed);
   if (threadIdx.x == 0)
-    atomic_max_speed<T>(S.ctl, speed);
+    S.partials[blockIdx.x] = static_cast<double>(speed);
   if (bad)
     S.ctl->nonfinite = 1;
 }
@@ -332,23 +332,23 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pressure(SceneView<T> S) {
     term = static_cast<double>(((Ap - si) / S.rho0) * step_l(T(1e-9), pn));
   }
   const double bs = block_sum<kBlock>(term);
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&S.partials[blockIdx.x], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned t =
-        __hip_atomic_fetch_add(&S.ctl->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last)
+  if (threadIdx.x == 0)
+    S.partials[blockIdx.x] = bs;
+}
+
+// aprde = sum of the block partials in block order (FP64); kSolve also takes
+// the convergence decision, otherwise the sum is added to the global as the
+// reference's reduction clause does (iisph.hpp:440,505-507)
+template <bool kSolve>
+__global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const double *partials,
+                                                              unsigned nparts) {
+  if (kSolve && c->done)
     return;
   double acc = 0.0;
-  for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock)
-    acc += __hip_atomic_load(&S.partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
+    acc += partials[k];
   acc = block_sum<kBlock>(acc);
   if (threadIdx.x == 0) {
-    Ctl *c = S.ctl;
-    c->ticket = 0u;
     if (kSolve) {
       c->aprde = acc;
       const int it = c->iters + 1;
@@ -364,6 +364,17 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pressure(SceneView<T> S) {
       c->rel = c->aprde / static_cast<double>(c->nprde);
     }
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double *partials,
+                                                           unsigned nparts) {
+  double m = 0.0;
+  for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
+    m = partials[k] > m ? partials[k] : m;
+  m = block_max<kBlock>(m);
+  if (threadIdx.x == 0)
+    atomic_max_speed<T>(c, static_cast<T>(m));
 }
 
 // ---- neighbour export (Neighborhood::CopyNeighbors, neighborhood.cpp:156-167)
@@ -429,7 +440,11 @@ void SphOps<T>::integrate_v(hipStream_t st, const SceneView<T> &S) {
 }
 template <class T>
 void SphOps<T>::integrate_x(hipStream_t st, const SceneView<T> &S) {
-  PRTCL_FLUID_LAUNCH(k_integrate_x<T>);
+  if (S.nf <= 0)
+    return;
+  const unsigned nb = grid_blocks(S.nf);
+  hipLaunchKernelGGL(k_integrate_x<T>, dim3(nb), dim3(kBlock), 0, st, S);
+  hipLaunchKernelGGL(k_speed_finalize<T>, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials, nb);
 }
 template <class T>
 void SphOps<T>::iisph_setup(hipStream_t st, const SceneView<T> &S) {
@@ -448,12 +463,16 @@ template <class T>
 void SphOps<T>::iisph_pressure(hipStream_t st, const SceneView<T> &S, bool solve) {
   if (S.nf <= 0)
     return;
-  if (solve)
-    hipLaunchKernelGGL((k_iisph_pressure<T, true>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0,
-                       st, S);
-  else
-    hipLaunchKernelGGL((k_iisph_pressure<T, false>), dim3(grid_blocks(S.nf)), dim3(kBlock),
-                       0, st, S);
+  const unsigned nb = grid_blocks(S.nf);
+  if (solve) {
+    hipLaunchKernelGGL((k_iisph_pressure<T, true>), dim3(nb), dim3(kBlock), 0, st, S);
+    hipLaunchKernelGGL(k_pressure_finalize<true>, dim3(1), dim3(kBlock), 0, st, S.ctl,
+                       S.partials, nb);
+  } else {
+    hipLaunchKernelGGL((k_iisph_pressure<T, false>), dim3(nb), dim3(kBlock), 0, st, S);
+    hipLaunchKernelGGL(k_pressure_finalize<false>, dim3(1), dim3(kBlock), 0, st, S.ctl,
+                       S.partials, nb);
+  }
 }
 template <class T>
 void SphOps<T>::count_neighbors(hipStream_t st, const SceneView<T> &S, int qgroup, int ngroup,

@@ -48,7 +48,8 @@ EXPORTS = [
     "prtcl_hip_upload_field", "prtcl_hip_download_field", "prtcl_hip_field_id",
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
     "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
-    "prtcl_hip_copy_neighbors", "prtcl_hip_last_permutation",
+    "prtcl_hip_copy_neighbors", "prtcl_hip_last_permutation", "prtcl_hip_profile_enable",
+    "prtcl_hip_profile_read",
 ]
 
 
@@ -75,6 +76,7 @@ class SceneDesc(C.Structure):
         ("explicit_viscosity", C.c_int32),
         ("implicit_viscosity", C.c_int32),
         ("permute_every", C.c_int32),
+        ("step_mode", C.c_int32),
     ]
 
 
@@ -124,15 +126,19 @@ def lib():
         L.prtcl_hip_copy_neighbors.argtypes = [vp, i32, i32, i32, u32p, u32p,
                                                C.POINTER(C.c_uint64)]
         L.prtcl_hip_last_permutation.argtypes = [vp, i32, i32, u32p]
+        L.prtcl_hip_profile_enable.argtypes = [vp, C.c_int]
+        L.prtcl_hip_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_char_p)]
         _lib = L
     return _lib
 
 
-def desc_for(scene, real_type: int = 0) -> SceneDesc:
+def desc_for(scene, real_type: int = 0, step_mode: int = 0) -> SceneDesc:
     """Scene description from a prtcl_amd.scenes.Scene."""
     d = SceneDesc()
     lib().prtcl_hip_scene_desc_init(C.byref(d))
     d.real_type = real_type
+    d.step_mode = step_mode
     d.n_fluid = scene.n_fluid
     d.n_boundary = scene.n_boundary
     d.radius = scene.radius
@@ -187,6 +193,24 @@ class Context:
     def synchronize(self):
         self.check(self.L.prtcl_hip_synchronize(self.h), "synchronize")
 
+    def profile(self, on: bool):
+        self.check(self.L.prtcl_hip_profile_enable(self.h, int(on)), "profile_enable")
+
+    def profile_read(self) -> Dict[str, tuple]:
+        """{stage name: (total_ms, launches)} of the stages timed since
+        profile(True)."""
+        out = {}
+        k = 0
+        while True:
+            ms, n, name = C.c_double(), C.c_uint64(), C.c_char_p()
+            rc = self.L.prtcl_hip_profile_read(self.h, k, C.byref(ms), C.byref(n), C.byref(name))
+            if rc == NOT_FOUND:
+                break
+            self.check(rc, "profile_read")
+            out[name.value.decode()] = (ms.value, n.value)
+            k += 1
+        return out
+
     def step(self, scenes: Sequence["HipScene"], k: int = 1) -> int:
         ids = (C.c_int32 * len(scenes))(*[s.id for s in scenes])
         rc = self.L.prtcl_hip_step(self.h, ids, len(scenes), k)
@@ -198,12 +222,13 @@ class Context:
 class HipScene:
     """One device-resident scene created from a prtcl_amd.scenes.Scene."""
 
-    def __init__(self, ctx: Context, scene, real_type: int = 0, upload: bool = True):
+    def __init__(self, ctx: Context, scene, real_type: int = 0, upload: bool = True,
+                 step_mode: int = 0):
         self.ctx, self.L = ctx, ctx.L
         self.scene = scene
         self.real_type = real_type
         self.dtype = np.float64 if real_type == 1 else np.float32
-        self.desc = desc_for(scene, real_type)
+        self.desc = desc_for(scene, real_type, step_mode)
         sid = C.c_int32()
         ctx.check(self.L.prtcl_hip_scene_create(ctx.h, C.byref(self.desc), C.byref(sid)),
                   "scene_create")

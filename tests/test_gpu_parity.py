@@ -126,13 +126,14 @@ def _compare_state(o, g, tol, what):
     return errs
 
 
+@pytest.mark.parametrize("step_mode", [1, 0], ids=["procedures", "fused"])
 @pytest.mark.parametrize("nsteps", [10])
-def test_steps_config0(hip_ctx, oracle_lib, nsteps):
+def test_steps_config0(hip_ctx, oracle_lib, nsteps, step_mode):
     sc = scenes.config0()
     sc.permute_every = 0  # GPU keeps the original host order; the oracle permutes
     o = oracle_lib.OracleScene(sc, fp64_reductions=True, permute_every=8)
     from prtcl_amd import hip
-    g = hip.HipScene(hip_ctx, sc)
+    g = hip.HipScene(hip_ctx, sc, step_mode=step_mode)
     o.prepare()
     g.prepare()
     its_o, its_g = [], []
