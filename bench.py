@@ -73,8 +73,9 @@ def main():
     off, _ = g.neighbors(hip.BOUNDARY, hip.FLUID)
     nb_active = int(np.count_nonzero(np.diff(off)))
 
-    ctx.profile(True)
+    # timed region: K whole steps, no instrumentation
     iters, lat = [], []
+    ctx.synchronize()
     t_start = time.perf_counter()
     prev = t_start
     for _ in range(args.steps):
@@ -85,6 +86,15 @@ def main():
         prev = now
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
+
+    # kernel timing pass: the same steps continued with hipEvents recorded on
+    # the library's stream around every stage (the events add small gaps, so
+    # this pass is kept out of the timed region above)
+    ctx.profile(True)
+    prof_steps = min(args.steps, 20)
+    for _ in range(prof_steps):
+        g.step(1)
+    ctx.synchronize()
     prof = ctx.profile_read()
     ctx.profile(False)
 
@@ -136,7 +146,7 @@ def main():
         "p99_step_ms": round(1e3 * float(np.percentile(lat, 99)), 4),
         "step_roofline_frac": round(step_frac, 5),
         "roofline": roofline,
-        "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items()},
+        "stages_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in stages.items()},
     }
 
     if not args.no_cpu_baseline:

@@ -23,6 +23,10 @@ namespace prtcl_amd {
 
 namespace {
 
+// neighbours staged per thread in LDS by the setup pass (40 KB per block)
+constexpr uint32_t kSetupLdsCap = 40;
+static_assert(kSetupLdsCap <= kNbrCap, "the LDS table must fit the list");
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // works on one contiguous range of cell-ordered particles and its L2 holds
 // that range's neighbourhood.
@@ -73,9 +77,14 @@ __device__ __forceinline__ void runs(const GridDims &g, uint32_t cellc,
 // square root: q = r/h, W = aw ((2-q)^3 - 4 (1-q)^3_+),
 // dW/dr = ag (12 (1-q)^2_+ - 3 (2-q)^2), gradW = (dW/dr / r) x_ij, and
 // gradW = 0 below r = sqrt(eps) as the reference (kernel_facade.hpp:226).
+//
+// Neighbours satisfy r < 2h, so 2 - q >= 0 up to rounding; with
+// t = 2 - q and u = max(2 - 2q, 0):  12 (1-q)^2_+ - 3 (2-q)^2 = 3 (u^2 - t^2).
+// gradf_unscaled() returns (u^2 - t^2) / r; the caller multiplies the sum by
+// ag3 = 3 alpha / h^4 once.
 template <class T>
 struct FastSpline {
-  T inv_h, aw, ag, eps2;
+  T inv_h, aw, ag, ag3, eps2;
   __device__ __forceinline__ void eval(T r2, T &w, T &f) const {
     const T rinv = rsqrt_fast<T>(r2 > T(0) ? r2 : T(1));
     const T r = r2 * rinv;
@@ -89,14 +98,15 @@ struct FastSpline {
     if (r2 <= T(0))
       w = aw * T(4);  // W(0) = alpha/h^3 (8 - 4)
   }
-  __device__ __forceinline__ T gradf(T r2) const {
-    const T rinv = rsqrt_fast<T>(r2 > T(0) ? r2 : T(1));
+  __device__ __forceinline__ T gradf(T r2) const { return ag3 * gradf_unscaled(r2); }
+  __device__ __forceinline__ T gradf_unscaled(T r2) const {
+    const T rinv = rsqrt_fast<T>(r2);  // r2 = 0 gives inf/NaN, masked below
     const T q = r2 * rinv * inv_h;
-    const T b2 = q < T(2) ? T(2) - q : T(0);
-    const T b1 = T(1) - q;
-    const T b1p = b1 > T(0) ? b1 : T(0);
-    const T dw = ag * (T(12) * b1p * b1p - T(3) * b2 * b2);
-    return r2 < eps2 ? T(0) : dw * rinv;
+    const T t = T(2) - q;
+    T u = T(2) - T(2) * q;
+    u = u > T(0) ? u : T(0);
+    const T d = (u * u - t * t) * rinv;
+    return r2 < eps2 ? T(0) : d;
   }
 };
 
@@ -107,6 +117,7 @@ __device__ __forceinline__ FastSpline<T> fast_spline(const SceneView<T> &S) {
   k.inv_h = T(1) / h;
   k.aw = S.K.alpha / (h * h * h);
   k.ag = S.K.alpha / (h * h * h * h);
+  k.ag3 = T(3) * k.ag;
   k.eps2 = S.K.eps * S.K.eps;
   return k;
 }
@@ -170,53 +181,106 @@ __global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
   const T vs0 = vi.x + dtg0 * Hi, vs1 = vi.y + dtg1 * Hi, vs2 = vi.z + dtg2 * Hi;
   const uint32_t cell = S.fcell[i];
   const int n = S.nf;
-  T rho = 0, gf0 = 0, gf1 = 0, gf2 = 0, s2 = 0, dsum = 0;
+  // phase 1: exact predicate over the 27-cell candidates, accepted fluid
+  // neighbours appended to the list (the kernel arithmetic of the accepted
+  // pairs runs in phase 2 over the compacted list, so the SIMD lanes that
+  // reject a candidate do not wait for the ones that accept it)
+  // The accepted indices go to this thread's column of an LDS table first
+  // (conflict-free: entry k of thread t at lst[k][t]); phase 2 copies them
+  // to the list row by row, so every global store is a coalesced row write.
+  __shared__ uint32_t lst[kSetupLdsCap][kBlock];
+  const unsigned t = threadIdx.x;
   uint32_t cnt = 0;
   runs(S.grid, cell, S.fstart, [&](uint32_t b, uint32_t e) {
-    for (uint32_t j = b; j < e; ++j) {
-      const vec4_t<T> pj = S.xm[j];
-      const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-      const T r2 = sqdist_exact(d0, d1, d2);
-      if (!(r2 < S.thr2))
-        continue;
-      T w, f;
-      K.eval(r2, w, f);
-      rho += pj.w * w;
-      if (j == static_cast<uint32_t>(i))
-        continue;
-      if (cnt < L.cap)
-        L.idx[static_cast<size_t>(cnt) * n + i] = j;
-      ++cnt;
-      const T g0 = f * d0, g1 = f * d1, g2 = f * d2;
-      gf0 += pj.w * g0;
-      gf1 += pj.w * g1;
-      gf2 += pj.w * g2;
-      s2 += pj.w * (g0 * g0 + g1 * g1 + g2 * g2);
-      if (kWithSource) {
-        const vec4_t<T> vj = S.v[j];
-        const T Hj = fade_step(S, vj.w);
-        const T u0 = vs0 - (vj.x + dtg0 * Hj);
-        const T u1 = vs1 - (vj.y + dtg1 * Hj);
-        const T u2 = vs2 - (vj.z + dtg2 * Hj);
-        dsum += pj.w * (u0 * g0 + u1 * g1 + u2 * g2);
+    // four candidates per trip, loads issued together (indices clamped to the
+    // run; the clamped duplicates are rejected by the bound check)
+    for (uint32_t j = b; j < e; j += 4) {
+      vec4_t<T> pk[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        pk[u] = S.xm[min(j + u, e - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t ju = j + u;
+        if (ju < e && ju != static_cast<uint32_t>(i) &&
+            sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) < S.thr2) {
+          if (cnt < kSetupLdsCap)
+            lst[cnt][t] = ju;
+          ++cnt;
+        }
       }
     }
   });
   L.count[i] = cnt;
+  // phase 2: density (self term W(0) = 4 alpha / h^3), G_f, sum |gradW|^2,
+  // and the velocity-divergence sum of the source term
+  T rho = xi.w * K.aw * T(4), gf0 = 0, gf1 = 0, gf2 = 0, s2 = 0, dsum = 0;
+  auto pair = [&](uint32_t j) {
+    const vec4_t<T> pj = S.xm[j];
+    const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
+    T w, f;
+    K.eval(d0 * d0 + d1 * d1 + d2 * d2, w, f);
+    rho += pj.w * w;
+    const T g0 = f * d0, g1 = f * d1, g2 = f * d2;
+    gf0 += pj.w * g0;
+    gf1 += pj.w * g1;
+    gf2 += pj.w * g2;
+    s2 += pj.w * (g0 * g0 + g1 * g1 + g2 * g2);
+    if (kWithSource) {
+      const vec4_t<T> vj = S.v[j];
+      const T Hj = fade_step(S, vj.w);
+      const T u0 = vs0 - (vj.x + dtg0 * Hj);
+      const T u1 = vs1 - (vj.y + dtg1 * Hj);
+      const T u2 = vs2 - (vj.z + dtg2 * Hj);
+      dsum += pj.w * (u0 * g0 + u1 * g1 + u2 * g2);
+    }
+  };
+  const size_t nn = static_cast<size_t>(n);
+  if (cnt <= kSetupLdsCap) {
+#pragma unroll 4
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t j = lst[k][t];
+      L.idx[k * nn + i] = j;  // row k of the list, written by all lanes together
+      pair(j);
+    }
+  } else {
+    // more neighbours than the LDS table holds: walk the grid again in the
+    // same candidate order, writing the first L.cap entries
+    uint32_t k = 0;
+    runs(S.grid, cell, S.fstart, [&](uint32_t b, uint32_t e) {
+      for (uint32_t j = b; j < e; ++j) {
+        const vec4_t<T> pj = S.xm[j];
+        if (j != static_cast<uint32_t>(i) &&
+            sqdist_exact(xi.x - pj.x, xi.y - pj.y, xi.z - pj.z) < S.thr2) {
+          if (k < L.cap)
+            L.idx[k * nn + i] = j;
+          ++k;
+          pair(j);
+        }
+      }
+    });
+  }
   T rhob = 0, gb0 = 0, gb1 = 0, gb2 = 0;
   runs(S.grid, cell, S.bstart, [&](uint32_t b, uint32_t e) {
-    for (uint32_t j = b; j < e; ++j) {
-      const vec4_t<T> pj = S.bxV[j];
-      const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-      const T r2 = sqdist_exact(d0, d1, d2);
-      if (!(r2 < S.thr2))
-        continue;
-      T w, f;
-      K.eval(r2, w, f);
-      rhob += pj.w * w;
-      gb0 += pj.w * (f * d0);
-      gb1 += pj.w * (f * d1);
-      gb2 += pj.w * (f * d2);
+    for (uint32_t j = b; j < e; j += 4) {
+      vec4_t<T> pk[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        pk[u] = S.bxV[min(j + u, e - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const vec4_t<T> pj = pk[u];
+        const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
+        const T r2 = sqdist_exact(d0, d1, d2);
+        if (j + u >= e || !(r2 < S.thr2))
+          continue;
+        T w, f;
+        K.eval(r2, w, f);
+        rhob += pj.w * w;
+        gb0 += pj.w * (f * d0);
+        gb1 += pj.w * (f * d1);
+        gb2 += pj.w * (f * d2);
+      }
     }
   });
   const T rho0 = S.rho0;
@@ -236,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
   S.AA[i] = AA;
   S.p[i] = T(0);
   S.pr[i] = T(0);
+  S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, T(0));
   if (kWithSource) {
     const T src = (rho0 - rho) -
                   S.dt * rho0 * irho * (dsum + rho0 * (vs0 * gb0 + vs1 * gb1 + vs2 * gb2));
@@ -331,17 +396,18 @@ __global__ __launch_bounds__(kBlock) void k_accel_list(SceneView<T> S, NbrList L
   const vec4_t<T> xi = S.xm[i];
   const vec4_t<T> ci = S.c[i];
   const T pi = S.p[i];
-  T a0 = pi * ci.x, a1 = pi * ci.y, a2 = pi * ci.z;
+  T a0 = 0, a1 = 0, a2 = 0;
   auto pair = [&](uint32_t j) {
-    const vec4_t<T> pj = S.xm[j];
-    const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-    const T s = pj.w * S.pr[j] * K.gradf(d0 * d0 + d1 * d1 + d2 * d2);
-    a0 -= s * d0;
-    a1 -= s * d1;
-    a2 -= s * d2;
+    const vec4_t<T> qj = S.xq[j];  // (x_j, m_j p_j / rho_j^2)
+    const T d0 = xi.x - qj.x, d1 = xi.y - qj.y, d2 = xi.z - qj.z;
+    const T s = qj.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2);
+    a0 += s * d0;
+    a1 += s * d1;
+    a2 += s * d2;
   };
   walk(S, L, i, xi, pair);
-  S.a[i] = make4<vec4_t<T>>(a0, a1, a2, T(0));
+  S.a[i] = make4<vec4_t<T>>(pi * ci.x - K.ag3 * a0, pi * ci.y - K.ag3 * a1,
+                            pi * ci.z - K.ag3 * a2, T(0));
 }
 
 // IISPH relaxed Jacobi update over the list, FP64 aprde reduction and the
@@ -362,10 +428,12 @@ __global__ __launch_bounds__(kBlock) void k_pressure_list(SceneView<T> S, NbrLis
       const vec4_t<T> pj = S.xm[j];
       const vec4_t<T> aj = S.a[j];
       const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-      sum += pj.w * K.gradf(d0 * d0 + d1 * d1 + d2 * d2) * (aj.x * d0 + aj.y * d1 + aj.z * d2);
+      sum += pj.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2) *
+             (aj.x * d0 + aj.y * d1 + aj.z * d2);
     };
     walk(S, L, i, xi, pair);
-    const T Ap = S.dt * S.dt * S.rho0 / rhoi * ((ai.x * gs.x + ai.y * gs.y + ai.z * gs.z) - sum);
+    const T Ap = S.dt * S.dt * S.rho0 / rhoi *
+                 ((ai.x * gs.x + ai.y * gs.y + ai.z * gs.z) - K.ag3 * sum);
     const T AA = S.AA[i], si = S.s[i];
     const T rcp = (fabs(AA) > T(1e-9)) ? T(1) / AA : T(0);
     const T upd = S.p[i] + S.omega * (si - Ap) * rcp;
@@ -373,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_list(SceneView<T> S, NbrLis
     S.Ap[i] = Ap;
     S.p[i] = pn;
     S.pr[i] = pn / (rhoi * rhoi);
+    S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w * pn / (rhoi * rhoi));
     term = static_cast<double>((Ap - si) / S.rho0 * ((T(1e-9) < pn) ? T(1) : T(0)));
   }
   const double bs = block_sum<kBlock>(term);
@@ -401,6 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const doub
       c->done = 1;
       c->converged = (it >= c->min_it && rel < c->tol) ? 1 : 0;
     }
+    publish_progress(c);
   }
 }
 

@@ -93,22 +93,53 @@ __global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t
                       static_cast<size_t>(threadIdx.x) * kScanItems;
   uint32_t vals[kScanItems];
   uint32_t s = 0;
+  const bool full = base + kScanItems <= n;  // 16 items in range: 4 x 16-byte accesses
+  if (full) {
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    size_t i = base + k;
-    vals[k] = i < n ? count[i] : 0u;
-    s += vals[k];
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      const uint4 q = reinterpret_cast<const uint4 *>(count + base)[k];
+      vals[4 * k] = q.x;
+      vals[4 * k + 1] = q.y;
+      vals[4 * k + 2] = q.z;
+      vals[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      size_t i = base + k;
+      vals[k] = i < n ? count[i] : 0u;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    s += vals[k];
   uint32_t tot;
   uint32_t ex = block_excl_scan(s, &tot) + tile_sums[blockIdx.x];
+  if (full) {
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    size_t i = base + k;
-    if (i < n) {
-      start[i] = ex;
-      count[i] = 0u;
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      uint4 q;
+      q.x = ex;
+      ex += vals[4 * k];
+      q.y = ex;
+      ex += vals[4 * k + 1];
+      q.z = ex;
+      ex += vals[4 * k + 2];
+      q.w = ex;
+      ex += vals[4 * k + 3];
+      reinterpret_cast<uint4 *>(start + base)[k] = q;
+      reinterpret_cast<uint4 *>(count + base)[k] = make_uint4(0u, 0u, 0u, 0u);
     }
-    ex += vals[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      size_t i = base + k;
+      if (i < n) {
+        start[i] = ex;
+        count[i] = 0u;
+      }
+      ex += vals[k];
+    }
   }
 }
 
@@ -117,26 +148,45 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(GridDims g, const vec4_t<T
                                                       uint32_t *cell, uint32_t *count,
                                                       uint32_t *rank) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n)
-    return;
-  const vec4_t<T> p = x[i];
-  const double comp[3] = {static_cast<double>(p.x), static_cast<double>(p.y),
-                          static_cast<double>(p.z)};
-  const int org[3] = {g.ox, g.oy, g.oz};
-  const int dim[3] = {g.nx, g.ny, g.nz};
-  uint32_t c[3];
+  uint32_t lin = 0xFFFFFFFFu;
+  if (i < n) {
+    const vec4_t<T> p = x[i];
+    const double comp[3] = {static_cast<double>(p.x), static_cast<double>(p.y),
+                            static_cast<double>(p.z)};
+    const int org[3] = {g.ox, g.oy, g.oz};
+    const int dim[3] = {g.nx, g.ny, g.nz};
+    uint32_t c[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    // floor(double(x) / r) as grouped_uniform_grid.hpp:626-628, relative to
-    // the box origin, clamped to the box (fmax maps NaN to 0)
-    double f = floor(comp[a] / g.radius) - static_cast<double>(org[a]);
-    f = fmin(fmax(f, 0.0), static_cast<double>(dim[a] - 1));
-    c[a] = static_cast<uint32_t>(f);
+    for (int a = 0; a < 3; ++a) {
+      // floor(double(x) / r) as grouped_uniform_grid.hpp:626-628, relative to
+      // the box origin, clamped to the box (fmax maps NaN to 0)
+      double f = floor(comp[a] / g.radius) - static_cast<double>(org[a]);
+      f = fmin(fmax(f, 0.0), static_cast<double>(dim[a] - 1));
+      c[a] = static_cast<uint32_t>(f);
+    }
+    cell[i] = c[0] | (c[1] << 11) | (c[2] << 22);
+    lin = c[0] + static_cast<uint32_t>(g.nx) * (c[1] + static_cast<uint32_t>(g.ny) * c[2]);
   }
-  cell[i] = c[0] | (c[1] << 11) | (c[2] << 22);
-  const uint32_t lin = c[0] + static_cast<uint32_t>(g.nx) *
-                                  (c[1] + static_cast<uint32_t>(g.ny) * c[2]);
-  rank[i] = atomicAdd(&count[lin], 1u);
+  // One atomic per distinct cell of the wavefront (particles arrive in the
+  // previous step's cell order, so a wavefront covers only a few cells); the
+  // lanes of a cell take consecutive ranks after the returned base.
+  const int lane = static_cast<int>(__lane_id());
+  unsigned long long pending = __ballot(i < n);
+  uint32_t r = 0;
+  while (pending) {
+    const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+    const uint32_t key = __shfl(lin, leader, kWave);
+    const unsigned long long same = __ballot(lin == key) & pending;
+    uint32_t base = 0;
+    if (lane == leader)
+      base = atomicAdd(&count[key], static_cast<uint32_t>(__popcll(same)));
+    base = __shfl(base, leader, kWave);
+    if ((same >> lane) & 1ull)
+      r = base + static_cast<uint32_t>(__popcll(same & ((1ull << lane) - 1ull)));
+    pending &= ~same;
+  }
+  if (i < n)
+    rank[i] = r;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(GridDims g, const uint32_t *cell,
@@ -153,26 +203,27 @@ __global__ __launch_bounds__(kBlock) void k_scatter(GridDims g, const uint32_t *
   perm[start[lin] + rank[i]] = static_cast<uint32_t>(i);
 }
 
-// Within each cell, order the slots by their previous slot index. The atomic
-// ranks of k_cell_keys are scheduling dependent; after this pass the order is
-// a pure function of the previous order (a stable counting sort).
-__global__ __launch_bounds__(kBlock) void k_fixup(const uint32_t *start, uint32_t ncells,
-                                                  uint32_t *perm) {
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= ncells)
+// Stable placement: the atomic ranks of k_cell_keys are scheduling
+// dependent, so every particle recomputes its rank inside its cell as the
+// number of cell members with a smaller previous slot. The result is a pure
+// function of the previous order (a stable counting sort).
+__global__ __launch_bounds__(kBlock) void k_stable_rank(GridDims g, const uint32_t *cell,
+                                                        const uint32_t *start,
+                                                        const uint32_t *perm, int n,
+                                                        uint32_t *perm_out) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n)
     return;
-  const uint32_t b = start[c], e = start[c + 1];
-  if (e - b < 2)
-    return;
-  for (uint32_t k = b + 1; k < e; ++k) {
-    uint32_t v = perm[k];
-    uint32_t j = k;
-    while (j > b && perm[j - 1] > v) {
-      perm[j] = perm[j - 1];
-      --j;
-    }
-    perm[j] = v;
-  }
+  const uint32_t o = perm[s];
+  const uint32_t c = cell[o];
+  const uint32_t lin = (c & 0x7FFu) + static_cast<uint32_t>(g.nx) *
+                                          (((c >> 11) & 0x7FFu) +
+                                           static_cast<uint32_t>(g.ny) * (c >> 22));
+  const uint32_t b = start[lin], e = start[lin + 1];
+  uint32_t r = 0;
+  for (uint32_t k = b; k < e; ++k)
+    r += perm[k] < o ? 1u : 0u;
+  perm_out[b + r] = o;
 }
 
 template <int W>
@@ -187,6 +238,25 @@ __global__ __launch_bounds__(kBlock) void k_gather(E *__restrict__ dst,
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n)
     dst[i] = src[perm[i]];
+}
+
+// the state carried between steps, reordered in one pass
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_gather_state(V *__restrict__ xd, const V *__restrict__ xs,
+                                                         V *__restrict__ vd, const V *__restrict__ vs,
+                                                         uint32_t *__restrict__ id_d,
+                                                         const uint32_t *__restrict__ id_s,
+                                                         uint32_t *__restrict__ c_d,
+                                                         const uint32_t *__restrict__ c_s,
+                                                         const uint32_t *__restrict__ perm, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const uint32_t o = perm[i];
+  xd[i] = xs[o];
+  vd[i] = vs[o];
+  id_d[i] = id_s[o];
+  c_d[i] = c_s[o];
 }
 
 }  // namespace
@@ -206,7 +276,7 @@ void scan_counts(hipStream_t st, uint32_t *count, uint32_t *start, uint32_t n,
 }
 
 template <class T>
-void GridOps<T>::sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
+const uint32_t *GridOps<T>::sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
                             uint32_t *cell, uint32_t *count, uint32_t *start,
                             uint32_t *rank, uint32_t *perm, uint32_t *scan_tmp) {
   if (n > 0)
@@ -214,11 +284,14 @@ void GridOps<T>::sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *
                        cell, count, rank);
   scan_counts(st, count, start, g.ncells, scan_tmp);
   if (n > 0) {
+    // scatter by the atomic ranks into `perm`, then the stable placement into
+    // `rank` (the ranks are consumed by the scatter): the result is in `rank`
     hipLaunchKernelGGL(k_scatter, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g, cell, rank,
                        start, n, perm);
-    hipLaunchKernelGGL(k_fixup, dim3(grid_blocks(g.ncells)), dim3(kBlock), 0, st, start,
-                       g.ncells, perm);
+    hipLaunchKernelGGL(k_stable_rank, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g, cell,
+                       start, perm, n, rank);
   }
+  return rank;
 }
 
 template <class T>
@@ -264,7 +337,14 @@ void GridOps<T>::gather_elems(hipStream_t st, void *dst, const void *src,
 }
 
 template <class T>
-void GridOps<T>::bounds(hipStream_t, const vec4_t<T> *, int, T *) {}
+void GridOps<T>::gather_state(hipStream_t st, vec4_t<T> *xd, const vec4_t<T> *xs,
+                              vec4_t<T> *vd, const vec4_t<T> *vs, uint32_t *id_d,
+                              const uint32_t *id_s, uint32_t *c_d, const uint32_t *c_s,
+                              const uint32_t *perm, int n) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_gather_state<vec4_t<T>>, dim3(grid_blocks(n)), dim3(kBlock), 0, st,
+                       xd, xs, vd, vs, id_d, id_s, c_d, c_s, perm, n);
+}
 
 template struct GridOps<float>;
 template struct GridOps<double>;

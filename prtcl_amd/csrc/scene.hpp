@@ -46,11 +46,28 @@ struct Ctl {
   int nonfinite;
   unsigned max_speed_bits_f;       // running max |v| (float bits)
   unsigned long long max_speed_bits_d;  // running max |v| (double bits)
+  // host-visible progress word (pinned, mapped): (seq << 32) | (done << 31) | iters,
+  // written by the solver's finalize kernel so the host can poll it
+  unsigned long long *host_mirror;
+  unsigned seq;
   // PCG scalars (FP64)
   double cg_b_nsq, cg_thr, cg_r_nsq, cg_ry, cg_ry_prev, cg_pq, cg_alpha, cg_beta;
   int cg_iter, cg_done, cg_max_k;
   double cg_tol;
 };
+
+// device -> host progress of a solve (one 64-bit store at system scope)
+__device__ __forceinline__ void publish_progress(Ctl *c) {
+  if (c->host_mirror) {
+    __hip_atomic_store(c->host_mirror + 1,
+                       static_cast<unsigned long long>(__double_as_longlong(c->rel)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const unsigned long long w = (static_cast<unsigned long long>(c->seq) << 32) |
+                                 (static_cast<unsigned long long>(c->done ? 1u : 0u) << 31) |
+                                 static_cast<unsigned long long>(static_cast<unsigned>(c->iters));
+    __hip_atomic_store(c->host_mirror, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
 
 struct GridDims {
   int nx, ny, nz;     // cells per axis (dense box)
@@ -71,6 +88,7 @@ struct SceneView {
   const uint32_t *fstart;
   const uint32_t *bstart;
   V4 *xm, *v, *a, *c, *G, *vd;
+  V4 *xq;  // (x, y, z, m p / rho^2): the one gather of the pressure-acceleration pass
   T *rho, *p, *pr, *s, *AA, *Ap, *sp;
   int32_t *id;
   uint32_t *fcell;
@@ -102,14 +120,19 @@ struct HipError {
 // ----- grid / neighbour search (nsearch.hip) -----
 template <class T>
 struct GridOps {
-  // builds cell keys, counts, scan, stable scatter; returns perm in slot order
-  static void sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
-                         uint32_t *cell, uint32_t *count, uint32_t *start,
-                         uint32_t *rank, uint32_t *perm, uint32_t *scan_tmp);
+  // Cell keys, counts, scan and a stable counting sort of one group. Returns
+  // the permutation in slot order (new slot s holds old slot perm[s]); it is
+  // written to `rank` (perm is scratch).
+  static const uint32_t *sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x,
+                                    int n, uint32_t *cell, uint32_t *count, uint32_t *start,
+                                    uint32_t *rank, uint32_t *perm, uint32_t *scan_tmp);
   static void gather4(hipStream_t st, void *dst, const void *src, const uint32_t *perm, int n);
   static void gather_elems(hipStream_t st, void *dst, const void *src,
                            const uint32_t *perm, int n, int elem_bytes);
-  static void bounds(hipStream_t st, const vec4_t<T> *x, int n, T *out6);
+  // x, v (real4), id and cell of the carried state, reordered in one pass
+  static void gather_state(hipStream_t st, vec4_t<T> *xd, const vec4_t<T> *xs, vec4_t<T> *vd,
+                           const vec4_t<T> *vs, uint32_t *id_d, const uint32_t *id_s,
+                           uint32_t *c_d, const uint32_t *c_s, const uint32_t *perm, int n);
 };
 // exclusive scan of n uint32 counts into start[0..n] (start[n] = total); the
 // counts are cleared to zero for the next build

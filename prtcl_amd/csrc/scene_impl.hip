@@ -2,6 +2,7 @@
 // host-index <-> slot transfers, procedure dispatch and the IISPH loop with a
 // device-side convergence decision.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -119,9 +120,11 @@ __global__ void k_iota(int32_t *id, int n) {
 enum CtlOp { kCtlInit = 0, kCtlBeginSolve = 1, kCtlSetNprde = 2, kCtlSetAprde = 3,
              kCtlSetMaxSpeed = 4, kCtlSetNprdeValue = 5 };
 
-__global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2) {
+__global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2,
+                      unsigned long long *mirror) {
   switch (op) {
     case kCtlInit:  // the arena is zero-filled at allocation
+      c->host_mirror = mirror;
       break;
     case kCtlBeginSolve:
       c->aprde = 0.0;
@@ -133,6 +136,8 @@ __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2) {
       c->min_it = i0;
       c->max_it = i1;
       c->ticket = 0u;
+      c->seq = static_cast<unsigned>(i2);
+      publish_progress(c);
       break;
     case kCtlSetNprde:
     case kCtlSetNprdeValue:
@@ -146,7 +151,6 @@ __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2) {
       c->max_speed_bits_d = static_cast<unsigned long long>(__double_as_longlong(dv));
       break;
   }
-  (void)i2;
 }
 
 struct Arena {
@@ -198,6 +202,8 @@ class SceneImpl final : public SceneBase {
   }
 
   ~SceneImpl() override {
+    if (mirror_host_)
+      (void)hipHostFree(mirror_host_);
     if (arena_)
       (void)hipFree(arena_);
     if (grid_mem_)
@@ -318,8 +324,9 @@ class SceneImpl final : public SceneBase {
     PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
     return c;
   }
-  void ctl_op(int op, double dv = 0, int i0 = 0, int i1 = 0) {
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, res_->stream, P_.ctl, op, dv, i0, i1, 0);
+  void ctl_op(int op, double dv = 0, int i0 = 0, int i1 = 0, int i2 = 0) {
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, res_->stream, P_.ctl, op, dv, i0, i1, i2,
+                       mirror_dev_);
   }
 
   double get_global(int g) override {
@@ -422,14 +429,12 @@ class SceneImpl final : public SceneBase {
 
   void build_boundary_grid() {
     hipStream_t st = res_->stream;
-    GridOps<T>::sort_group(st, gd_, P_.bxV, nb_, P_.bcell, G_.count, G_.bstart, P_.brank,
-                           P_.bperm, G_.scan_tmp);
+    const uint32_t *pm = GridOps<T>::sort_group(st, gd_, P_.bxV, nb_, P_.bcell, G_.count,
+                                                G_.bstart, P_.brank, P_.bperm, G_.scan_tmp);
     if (nb_ > 0) {
-      gather_swap(P_.bxV, P_.btmpV4, P_.bperm, nb_, sizeof(V4));
-      gather_swap(P_.bid, P_.btmpI, P_.bperm, nb_, 4);
-      uint32_t *tmpu = reinterpret_cast<uint32_t *>(P_.btmpI);
-      gather_swap(P_.bcell, tmpu, P_.bperm, nb_, 4);
-      P_.btmpI = reinterpret_cast<int32_t *>(tmpu);
+      gather_swap(P_.bxV, P_.btmpV4, pm, nb_, sizeof(V4));
+      gather_swap(P_.bid, P_.btmpI, pm, nb_, 4);
+      gather_swap(P_.bcell, P_.bperm, pm, nb_, 4);  // bperm was scratch of the sort
     }
     bgrid_dirty_ = false;
   }
@@ -439,16 +444,19 @@ class SceneImpl final : public SceneBase {
   // carries over between steps (x, m, v, t_birth, id)
   void build_fluid_grid(bool all) {
     hipStream_t st = res_->stream;
-    GridOps<T>::sort_group(st, gd_, P_.xm, nf_, P_.fcell, G_.count, G_.fstart, P_.frank,
-                           P_.fperm, G_.scan_tmp);
+    const uint32_t *pm = GridOps<T>::sort_group(st, gd_, P_.xm, nf_, P_.fcell, G_.count,
+                                                G_.fstart, P_.frank, P_.fperm, G_.scan_tmp);
     if (nf_ > 0) {
-      const uint32_t *pm = P_.fperm;
-      gather_swap(P_.xm, P_.tmpV4, pm, nf_, sizeof(V4));
-      gather_swap(P_.v, P_.tmpV4, pm, nf_, sizeof(V4));
-      gather_swap(P_.id, P_.tmpI, pm, nf_, 4);
-      uint32_t *tmpu = reinterpret_cast<uint32_t *>(P_.tmpI);
-      gather_swap(P_.fcell, tmpu, pm, nf_, 4);
-      P_.tmpI = reinterpret_cast<int32_t *>(tmpu);
+      // x -> xq (rebuilt from x by the step before it is read), v -> tmpV4,
+      // id -> tmpI, cell -> fperm (scratch of the sort), then swap
+      GridOps<T>::gather_state(st, P_.xq, P_.xm, P_.tmpV4, P_.v,
+                               reinterpret_cast<uint32_t *>(P_.tmpI),
+                               reinterpret_cast<const uint32_t *>(P_.id), P_.fperm, P_.fcell, pm,
+                               nf_);
+      std::swap(P_.xm, P_.xq);
+      std::swap(P_.v, P_.tmpV4);
+      std::swap(P_.id, P_.tmpI);
+      std::swap(P_.fcell, P_.fperm);
       if (all) {
         for (V4 **f : {&P_.a, &P_.c, &P_.G, &P_.vd})
           gather_swap(*f, P_.tmpV4, pm, nf_, sizeof(V4));
@@ -521,6 +529,7 @@ class SceneImpl final : public SceneBase {
     S.c = P_.c;
     S.G = P_.G;
     S.vd = P_.vd;
+    S.xq = P_.xq;
     S.rho = P_.rho;
     S.p = P_.p;
     S.pr = P_.pr;
@@ -594,29 +603,71 @@ class SceneImpl final : public SceneBase {
   void run_wkbb18(int) { throw std::logic_error("wkbb18 is not available in this build"); }
 
   // --------------------------------------------------------------- IISPH loop
-  // Enqueues iterations in chunks; each chunk ends with an asynchronous copy
-  // of the device's done flag to pinned memory. While chunk c runs, the host
-  // waits for the flag of chunk c-1, so the device queue never drains while
-  // the decision is pending; iterations enqueued past convergence exit at
-  // their first instruction.
+  // The convergence decision is taken on the device (finalize kernel), which
+  // also publishes (seq, done, iterations, aprde/nprde) to pinned host memory.
+  // The host enqueues the predicted number of iterations, then keeps exactly
+  // one iteration queued ahead while it polls for the previous ones, so the
+  // device never idles on the decision; an iteration enqueued past
+  // convergence exits at its first instruction.
   int iisph_solve(double tol, int min_it, int max_it, int *iters, double *rel,
-                  bool wait) override {
-    return solve_impl(tol, min_it, max_it, iters, rel, wait, false);
+                  bool) override {
+    int rc = solve_impl(tol, min_it, max_it, iters, rel, false);
+    PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
+    return rc;
   }
 
-  int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool wait,
-                 bool fused) {
+  struct Progress {
+    int iters = 0;
+    bool done = false;
+    double rel = 0;
+  };
+
+  Progress decode(uint64_t w) const {
+    Progress p;
+    p.iters = static_cast<int>(w & 0x7FFFFFFFull);
+    p.done = ((w >> 31) & 1ull) != 0;
+    const uint64_t rb = __atomic_load_n(&mirror_host_[1], __ATOMIC_ACQUIRE);
+    std::memcpy(&p.rel, &rb, sizeof(double));
+    return p;
+  }
+
+  // waits until solve `seq` reports done or at least `target` iterations
+  Progress wait_progress(unsigned seq, int target) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; ++spin) {
+      const uint64_t w = __atomic_load_n(&mirror_host_[0], __ATOMIC_ACQUIRE);
+      if ((w >> 32) == seq) {
+        Progress p = decode(w);
+        if (p.done || p.iters >= target)
+          return p;
+      }
+      if ((spin & 1023u) == 1023u) {
+        // the stream drained without the expected word: read the control
+        // block itself (the word is only a shortcut)
+        if (hipStreamQuery(res_->stream) == hipSuccess) {
+          Ctl c = read_ctl();
+          Progress p;
+          p.iters = c.iters;
+          p.done = c.done != 0 || c.iters < target;  // < target: nothing left to wait for
+          p.rel = c.rel;
+          return p;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300))
+          throw HipError{hipErrorLaunchTimeOut, "IISPH solve made no progress for 300 s"};
+      }
+    }
+  }
+
+  int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused) {
     require_grid();
     hipStream_t st = res_->stream;
-    ctl_op(kCtlBeginSolve, tol, min_it, max_it);
+    const unsigned seq = ++seq_;
+    ctl_op(kCtlBeginSolve, tol, min_it, max_it, static_cast<int>(seq));
     const SceneView<T> S = view();
     const NbrList L = nbr_list();
-    int enq = 0, chunk = 0;
-    int first = std::max(min_it, std::min(predicted_iters_, max_it));
-    bool done = (nf_ == 0);
-    while (!done && enq < max_it) {
-      const int n = std::min(chunk == 0 ? first : 2, max_it - enq);
-      Profiler &pf = res_->prof;
+    Profiler &pf = res_->prof;
+    int enq = 0;
+    auto enqueue = [&](int n) {
       for (int k = 0; k < n; ++k) {
         int r = pf.begin(st, kStagePressureAccel, enq + k);
         if (fused)
@@ -632,30 +683,39 @@ class SceneImpl final : public SceneBase {
         pf.end(st, r);
       }
       enq += n;
-      int32_t *slot = res_->pinned + 8 * (chunk & 1);
-      PRTCL_HIP_CHECK(hipMemcpyAsync(slot, &P_.ctl->done, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-      PRTCL_HIP_CHECK(hipEventRecord(res_->ev[chunk & 1], st));
-      if (chunk >= 1) {
-        PRTCL_HIP_CHECK(hipEventSynchronize(res_->ev[(chunk - 1) & 1]));
-        if (res_->pinned[8 * ((chunk - 1) & 1)])
-          done = true;
+    };
+    Progress p;
+    if (nf_ > 0 && max_it > 0) {
+      enqueue(std::min(std::max(min_it, std::min(predicted_iters_, 8)), max_it));
+      for (;;) {
+        const int ahead = enq < max_it ? 1 : 0;
+        if (ahead)
+          enqueue(1);
+        p = wait_progress(seq, enq - ahead);
+        if (p.done)
+          break;
+        if (!ahead) {
+          p = wait_progress(seq, enq);
+          break;
+        }
       }
-      ++chunk;
+    } else {
+      p.done = true;
     }
     PRTCL_HIP_CHECK(hipGetLastError());
-    if (!wait && !iters && !rel)
-      return PRTCL_HIP_OK;
-    Ctl c = read_ctl();
-    if (res_->prof.on)
-      res_->prof.resolve(c.iters);
-    predicted_iters_ = std::max(1, c.iters);
-    last_iisph_iters_ = c.iters;
-    last_rel_ = c.rel;
+    if (pf.on) {
+      PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+      pf.resolve(p.iters);
+    }
+    predicted_iters_ = std::max(1, p.iters);
+    last_iisph_iters_ = p.iters;
+    last_rel_ = p.rel;
     if (iters)
-      *iters = c.iters;
+      *iters = p.iters;
     if (rel)
-      *rel = c.rel;
-    return (c.converged || c.nprde < 1) ? PRTCL_HIP_OK : PRTCL_HIP_NOT_CONVERGED;
+      *rel = p.rel;
+    const bool converged = p.iters >= min_it && p.rel < tol;
+    return (converged || nf_ == 0 || max_it == 0) ? PRTCL_HIP_OK : PRTCL_HIP_NOT_CONVERGED;
   }
 
   // ----------------------------------------------------------------- a step
@@ -712,7 +772,7 @@ class SceneImpl final : public SceneBase {
       int it = 0;
       double rel = 0;
       int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
-                          desc.iisph_max_iterations, &it, &rel, true, fused);
+                          desc.iisph_max_iterations, &it, &rel, fused);
       if (rc != PRTCL_HIP_OK)
         status = rc;
       const SceneView<T> S = view();
@@ -790,7 +850,7 @@ class SceneImpl final : public SceneBase {
 
  private:
   struct Ptrs {
-    V4 *xm, *v, *a, *c, *G, *vd, *tmpV4;
+    V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq;
     T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp;
     int32_t *id, *tmpI;
     uint32_t *fcell, *frank, *fperm, *nidx, *ncnt;
@@ -812,7 +872,7 @@ class SceneImpl final : public SceneBase {
   static void layout(Arena &A, Ptrs &P, int nf, int nb, bool pcg) {
     const size_t f = static_cast<size_t>(std::max(nf, 1));
     const size_t b = static_cast<size_t>(std::max(nb, 1));
-    for (V4 **q : {&P.xm, &P.v, &P.a, &P.c, &P.G, &P.vd, &P.tmpV4})
+    for (V4 **q : {&P.xm, &P.v, &P.a, &P.c, &P.G, &P.vd, &P.tmpV4, &P.xq})
       A.take(*q, f);
     for (T **q : {&P.rho, &P.p, &P.pr, &P.s, &P.AA, &P.Ap, &P.tmpT})
       A.take(*q, f);
@@ -845,7 +905,8 @@ class SceneImpl final : public SceneBase {
     PRTCL_HIP_CHECK(hipMalloc(&arena_, A.off));
     bytes = A.off;
     char *base = static_cast<char *>(arena_);
-    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.bxV, &P_.btmpV4,
+    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
+                   &P_.btmpV4,
                    &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb})
       Arena::rebase(*q, base);
     for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp})
@@ -864,6 +925,12 @@ class SceneImpl final : public SceneBase {
       hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nf_)), dim3(kBlock), 0, st, P_.id, nf_);
     if (nb_ > 0)
       hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nb_)), dim3(kBlock), 0, st, P_.bid, nb_);
+    // pinned, mapped, coherent progress words of the IISPH solve
+    PRTCL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&mirror_host_), 64,
+                                  hipHostMallocMapped | hipHostMallocCoherent));
+    mirror_host_[0] = mirror_host_[1] = 0;
+    PRTCL_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&mirror_dev_),
+                                            mirror_host_, 0));
     ctl_op(kCtlInit);
     PRTCL_HIP_CHECK(hipGetLastError());
     PRTCL_HIP_CHECK(hipStreamSynchronize(st));
@@ -883,6 +950,9 @@ class SceneImpl final : public SceneBase {
   double clock_ = 0;
   int steps_since_permute_ = 0;
   int predicted_iters_ = 2;
+  unsigned seq_ = 0;
+  uint64_t *mirror_host_ = nullptr;
+  unsigned long long *mirror_dev_ = nullptr;
   int last_iisph_iters_ = 0;
   double last_rel_ = 0;
   int last_wkbb_ = 0;

@@ -359,6 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const doub
         c->done = 1;
         c->converged = (it >= c->min_it && rel < c->tol) ? 1 : 0;
       }
+      publish_progress(c);
     } else {
       c->aprde = c->aprde + acc;
       c->rel = c->aprde / static_cast<double>(c->nprde);
