@@ -11,6 +11,7 @@
 #include "scene_impl.hpp"
 #include "sph.hpp"
 #include "fused.hpp"
+#include "wkbb18.hpp"
 
 namespace prtcl_amd {
 
@@ -600,7 +601,65 @@ class SceneImpl final : public SceneBase {
     PRTCL_HIP_CHECK(hipGetLastError());
   }
 
-  void run_wkbb18(int) { throw std::logic_error("wkbb18 is not available in this build"); }
+  // wkbb18.compute_diagonal / wkbb18.accumulate_acceleration; with `fused`
+  // the products walk the step's neighbour list
+  void run_wkbb18(int proc, bool fused = false) {
+    if (!desc.implicit_viscosity)
+      throw std::logic_error("the scene was created without implicit viscosity");
+    hipStream_t st = res_->stream;
+    const SceneView<T> S = view();
+    const NbrList L = nbr_list();
+    const NbrList *pl = fused ? &L : nullptr;
+    Profiler &pf = res_->prof;
+    if (proc == PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL) {
+      int r = pf.begin(st, kStageWkbb18Diag);
+      Wkbb18Ops<T>::diagonal(st, S, pl);
+      pf.end(st, r);
+      return;
+    }
+    // the PCG of cg_openmp.hpp:32-215, driven like the IISPH loop: the device
+    // takes every decision, the host keeps one iteration queued ahead
+    const unsigned seq = ++seq_;
+    const int max_k = std::max(0, desc.wkbb18_maximum_iterations);
+    int r = pf.begin(st, kStagePcgVector);
+    Wkbb18Ops<T>::setup(st, S, pl, P_.cg_part, desc.wkbb18_maximum_error, max_k, seq);
+    pf.end(st, r);
+    int enq = 0;
+    auto enqueue = [&](int n) {
+      for (int k = 0; k < n; ++k) {
+        int rr = pf.begin(st, kStagePcgSpmv, enq + k);
+        Wkbb18Ops<T>::iteration(st, S, pl, P_.cg_part);
+        pf.end(st, rr);
+      }
+      enq += n;
+    };
+    Progress p;
+    if (nf_ > 0) {
+      enqueue(std::min(std::max(1, std::min(predicted_cg_, 16)), max_k + 1));
+      for (;;) {
+        const int ahead = enq < max_k + 1 ? 1 : 0;
+        if (ahead)
+          enqueue(1);
+        p = wait_progress(seq, enq - ahead, 2);
+        if (p.done)
+          break;
+        if (!ahead) {
+          p = wait_progress(seq, enq, 2);
+          break;
+        }
+      }
+    }
+    r = pf.begin(st, kStagePcgVector);
+    Wkbb18Ops<T>::apply(st, S);
+    pf.end(st, r);
+    PRTCL_HIP_CHECK(hipGetLastError());
+    if (pf.on) {
+      PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+      pf.resolve(p.iters);
+    }
+    predicted_cg_ = std::max(1, p.iters);
+    last_wkbb_ = p.iters;
+  }
 
   // --------------------------------------------------------------- IISPH loop
   // The convergence decision is taken on the device (finalize kernel), which
@@ -632,10 +691,10 @@ class SceneImpl final : public SceneBase {
   }
 
   // waits until solve `seq` reports done or at least `target` iterations
-  Progress wait_progress(unsigned seq, int target) {
+  Progress wait_progress(unsigned seq, int target, int word = 0) {
     auto t0 = std::chrono::steady_clock::now();
     for (unsigned spin = 0;; ++spin) {
-      const uint64_t w = __atomic_load_n(&mirror_host_[0], __ATOMIC_ACQUIRE);
+      const uint64_t w = __atomic_load_n(&mirror_host_[word], __ATOMIC_ACQUIRE);
       if ((w >> 32) == seq) {
         Progress p = decode(w);
         if (p.done || p.iters >= target)
@@ -647,13 +706,15 @@ class SceneImpl final : public SceneBase {
         if (hipStreamQuery(res_->stream) == hipSuccess) {
           Ctl c = read_ctl();
           Progress p;
-          p.iters = c.iters;
-          p.done = c.done != 0 || c.iters < target;  // < target: nothing left to wait for
+          const int it = word == 0 ? c.iters : c.cg_iter;
+          const bool dn = word == 0 ? c.done != 0 : c.cg_done != 0;
+          p.iters = it;
+          p.done = dn || it < target;  // < target: nothing left to wait for
           p.rel = c.rel;
           return p;
         }
         if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300))
-          throw HipError{hipErrorLaunchTimeOut, "IISPH solve made no progress for 300 s"};
+          throw HipError{hipErrorLaunchTimeOut, "solver made no progress for 300 s"};
       }
     }
   }
@@ -785,7 +846,12 @@ class SceneImpl final : public SceneBase {
         SphOps<T>::integrate_v(st, S);
         pf.end(st, r);
         if (desc.implicit_viscosity) {
-          run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL);
+          // example.lua:427-433: diagonal, PCG, integrate velocity again
+          run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL, fused);
+          run_wkbb18(PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION, fused);
+          r = pf.begin(st, kStageGravityIntegrateV);
+          SphOps<T>::integrate_v(st, S);
+          pf.end(st, r);
         }
         r = pf.begin(st, kStageIntegrateX);
         SphOps<T>::integrate_x(st, S);
@@ -857,7 +923,7 @@ class SceneImpl final : public SceneBase {
     V4 *bxV, *btmpV4;
     int32_t *bid, *btmpI;
     uint32_t *bcell, *brank, *bperm;
-    double *partials;
+    double *partials, *cg_part;
     Ctl *ctl;
     V4 *cgx, *cgr, *cgp, *cgq, *cgy, *cgb;
   };
@@ -893,6 +959,7 @@ class SceneImpl final : public SceneBase {
     A.take(P.brank, b);
     A.take(P.bperm, b);
     A.take(P.partials, grid_blocks(static_cast<long>(std::max(f, b))) + 1);
+    A.take(P.cg_part, 3 * static_cast<size_t>(grid_blocks(static_cast<long>(f))) + 3);
     A.take(P.ctl, 1);
     const size_t nv = pcg ? f : 1;
     for (V4 **q : {&P.cgx, &P.cgr, &P.cgp, &P.cgq, &P.cgy, &P.cgb})
@@ -917,6 +984,7 @@ class SceneImpl final : public SceneBase {
                          &P_.brank, &P_.bperm})
       Arena::rebase(*q, base);
     Arena::rebase(P_.partials, base);
+    Arena::rebase(P_.cg_part, base);
     Arena::rebase(P_.ctl, base);
     hipStream_t st = res_->stream;
     // fields start zero-filled (varying_field.hpp:165-167); ids are identity
@@ -950,6 +1018,7 @@ class SceneImpl final : public SceneBase {
   double clock_ = 0;
   int steps_since_permute_ = 0;
   int predicted_iters_ = 2;
+  int predicted_cg_ = 4;
   unsigned seq_ = 0;
   uint64_t *mirror_host_ = nullptr;
   unsigned long long *mirror_dev_ = nullptr;

@@ -1,0 +1,566 @@
+// WKBB18 implicit viscosity (Weiler et al. 2018) on the device: the 3x3
+// block-diagonal system matrix and its inverse (wkbb18.hpp:194-270) and the
+// preconditioned CG of cg_openmp.hpp:32-215 with the matrix-free product of
+// wkbb18.hpp:342-390.
+//
+// Two product kernels share the CG engine:
+//   * exact: walks the 27-cell grid with the reference predicate and
+//     evaluates every pair term in the reference's FP32 operation order
+//     (used by the wkbb18.* procedures);
+//   * list: walks the step's fluid neighbour list with fused arithmetic and
+//     applies the boundary part as a precomputed symmetric 3x3 matrix per
+//     particle, B_i = sum_b k_ib gradW_ib x_ib^T (used by prtcl_hip_step).
+// The CG scalars (dot products, alpha, beta, thresholds) are FP64 on the
+// device; vectors are the scene's real type. The iteration logic follows the
+// reference exactly: r = A x - b, p0 = -y0, break on |p.q| < tol and on
+// |r.y|_prev < tol (absolute), stop when |r|^2 < max(tol^2 |b|^2, FLT_MIN),
+// at most max_k + 1 iterations, the returned count is the loop counter.
+#include "wkbb18.hpp"
+
+namespace prtcl_amd {
+
+namespace {
+
+template <class Fn>
+__device__ __forceinline__ void runs3(const GridDims &g, uint32_t cellc,
+                                      const uint32_t *__restrict__ start, Fn &&fn) {
+  const int cx = static_cast<int>(cellc & 0x7FFu);
+  const int cy = static_cast<int>((cellc >> 11) & 0x7FFu);
+  const int cz = static_cast<int>(cellc >> 22);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx + 1 < g.nx ? cx + 1 : g.nx - 1;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.nz)
+      continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.ny)
+        continue;
+      const uint32_t row = static_cast<uint32_t>(g.nx) *
+                           (static_cast<uint32_t>(y) +
+                            static_cast<uint32_t>(g.ny) * static_cast<uint32_t>(z));
+      fn(start[row + static_cast<uint32_t>(x0)], start[row + static_cast<uint32_t>(x1) + 1u]);
+    }
+  }
+}
+
+// exact neighbour visit (reference predicate), fn(j, d0, d1, d2, r2, pj)
+template <class T, class Fn>
+__device__ __forceinline__ void nbrs_exact(const SceneView<T> &S, const vec4_t<T> &xi,
+                                           uint32_t cellc, const uint32_t *start,
+                                           const vec4_t<T> *__restrict__ pts, Fn &&fn) {
+  runs3(S.grid, cellc, start, [&](uint32_t b, uint32_t e) {
+    for (uint32_t j = b; j < e; ++j) {
+      const vec4_t<T> pj = pts[j];
+      const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
+      const T r2 = sqdist_exact(d0, d1, d2);
+      if (r2 < S.thr2)
+        fn(j, d0, d1, d2, r2, pj);
+    }
+  });
+}
+
+template <class T>
+__device__ __forceinline__ T cof3(const T *m, int i, int j) {
+#pragma clang fp contract(off)
+  const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+  return m[i1 + 3 * j1] * m[i2 + 3 * j2] - m[i1 + 3 * j2] * m[i2 + 3 * j1];
+}
+
+// inverse by adjugate / determinant, column-major (element (r,c) at r + 3c)
+template <class T>
+__device__ __forceinline__ void invert3(const T *m, T *out) {
+#pragma clang fp contract(off)
+  const T c00 = cof3(m, 0, 0), c10 = cof3(m, 1, 0), c20 = cof3(m, 2, 0);
+  const T det = c00 * m[0] + (c10 * m[1] + c20 * m[2]);
+  const T invdet = T(1) / det;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      out[r + 3 * c] = cof3(m, c, r) * invdet;
+}
+
+template <class T>
+__device__ __forceinline__ T step_l(T eps, T x) {
+  return (eps < x) ? T(1) : T(0);
+}
+
+// ---- wkbb18.compute_diagonal, exact (wkbb18.hpp:194-270)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xi = S.xm[i];
+  const T rhoi = S.rho[i];
+  const uint32_t cell = S.fcell[i];
+  T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  nbrs_exact(S, xi, cell, S.fstart, S.xm, [&](uint32_t j, T d0, T d1, T d2, T r2,
+                                                const vec4_t<T> &pj) {
+    const T f = S.K.gradf(r2);
+    const T g[3] = {f * d0, f * d1, f * d2};
+    const T d[3] = {d0, d1, d2};
+    const T k = ((T(10) * (S.mu_f / rhoi)) * (pj.w / S.rho[j])) / (r2 + S.hh001);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        D[r + 3 * c] += k * (g[r] * d[c]);
+  });
+  nbrs_exact(S, xi, cell, S.bstart, S.bxV, [&](uint32_t, T d0, T d1, T d2, T r2,
+                                                 const vec4_t<T> &pj) {
+    const T f = S.K.gradf(r2);
+    const T g[3] = {f * d0, f * d1, f * d2};
+    const T d[3] = {d0, d1, d2};
+    const T k = ((T(10) * (S.mu_b / rhoi)) * ((S.rho0 * pj.w) / rhoi)) / (r2 + S.hh001);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        D[r + 3 * c] += k * (g[r] * d[c]);
+  });
+  T M[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      M[r + 3 * c] = ((r == c) ? T(1) : T(0)) - S.dt * D[r + 3 * c];
+  invert3(M, S.sp + 9 * static_cast<size_t>(i));
+}
+
+// ---- product_s, exact (wkbb18.hpp:342-390): out = w - dt * l(w)
+template <class T>
+__device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
+                                              const vec4_t<T> *__restrict__ w, T out[3]) {
+#pragma clang fp contract(off)
+  const vec4_t<T> xi = S.xm[i];
+  const vec4_t<T> wi = w[i];
+  const T rhoi = S.rho[i];
+  const uint32_t cell = S.fcell[i];
+  T l0 = 0, l1 = 0, l2 = 0;
+  nbrs_exact(S, xi, cell, S.fstart, S.xm, [&](uint32_t j, T d0, T d1, T d2, T r2,
+                                                const vec4_t<T> &pj) {
+    const vec4_t<T> wj = w[j];
+    const T f = S.K.gradf(r2);
+    const T s = (((T(10) * (S.mu_f / rhoi)) * (pj.w / S.rho[j])) *
+                 dot3_exact(wi.x - wj.x, wi.y - wj.y, wi.z - wj.z, d0, d1, d2)) /
+                (r2 + S.hh001);
+    l0 += s * (f * d0);
+    l1 += s * (f * d1);
+    l2 += s * (f * d2);
+  });
+  nbrs_exact(S, xi, cell, S.bstart, S.bxV, [&](uint32_t, T d0, T d1, T d2, T r2,
+                                                 const vec4_t<T> &pj) {
+    const T f = S.K.gradf(r2);
+    const T s = (((T(10) * (S.mu_b / rhoi)) * ((S.rho0 * pj.w) / rhoi)) *
+                 dot3_exact(wi.x, wi.y, wi.z, d0, d1, d2)) /
+                (r2 + S.hh001);
+    l0 += s * (f * d0);
+    l1 += s * (f * d1);
+    l2 += s * (f * d2);
+  });
+  out[0] = wi.x - S.dt * l0;
+  out[1] = wi.y - S.dt * l1;
+  out[2] = wi.z - S.dt * l2;
+}
+
+// ---- fast path: diagonal over the list, B_i, and (x, m/rho) packing
+template <class T>
+__device__ __forceinline__ T gradf_fast(const SceneView<T> &S, T r2) {
+  // same closed form as the fused IISPH kernels: 3 alpha/h^4 (u^2 - t^2)/r
+  const T h = S.K.h;
+  const T inv_h = T(1) / h;
+  const T rinv = T(1) / sqrt(r2);
+  const T q = r2 * rinv * inv_h;
+  const T t = T(2) - q;
+  T u = T(2) - T(2) * q;
+  u = u > T(0) ? u : T(0);
+  const T ag3 = T(3) * S.K.alpha / (h * h * h * h);
+  return r2 < S.K.eps * S.K.eps ? T(0) : ag3 * (u * u - t * t) * rinv;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xi = S.xm[i];
+  const T rhoi = S.rho[i];
+  const size_t n = static_cast<size_t>(S.nf);
+  T D[6] = {0, 0, 0, 0, 0, 0};  // symmetric: xx yy zz xy xz yz
+  auto add = [&](T k, T d0, T d1, T d2) {
+    D[0] += k * d0 * d0;
+    D[1] += k * d1 * d1;
+    D[2] += k * d2 * d2;
+    D[3] += k * d0 * d1;
+    D[4] += k * d0 * d2;
+    D[5] += k * d1 * d2;
+  };
+  const T cf = T(10) * S.mu_f / rhoi;
+  auto fluid = [&](uint32_t j) {
+    const vec4_t<T> pj = S.xm[j];
+    const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
+    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    add(cf * pj.w / S.rho[j] * gradf_fast(S, r2) / (r2 + S.hh001), d0, d1, d2);
+  };
+  const uint32_t cnt = L.count[i];
+  if (cnt <= L.cap) {
+    for (uint32_t k = 0; k < cnt; ++k)
+      fluid(L.idx[k * n + i]);
+  } else {
+    nbrs_exact(S, xi, S.fcell[i], S.fstart, S.xm,
+               [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
+                 if (j != static_cast<uint32_t>(i))
+                   fluid(j);
+               });
+  }
+  T B[6] = {0, 0, 0, 0, 0, 0};
+  const T cb = T(10) * S.mu_b / rhoi * S.rho0 / rhoi;
+  nbrs_exact(S, xi, S.fcell[i], S.bstart, S.bxV, [&](uint32_t, T d0, T d1, T d2, T r2e,
+                                                       const vec4_t<T> &pj) {
+    (void)r2e;
+    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const T k = cb * pj.w * gradf_fast(S, r2) / (r2 + S.hh001);
+    B[0] += k * d0 * d0;
+    B[1] += k * d1 * d1;
+    B[2] += k * d2 * d2;
+    B[3] += k * d0 * d1;
+    B[4] += k * d0 * d2;
+    B[5] += k * d1 * d2;
+  });
+  T M[9];
+  const T dt = S.dt;
+  const T Dt[6] = {D[0] + B[0], D[1] + B[1], D[2] + B[2], D[3] + B[3], D[4] + B[4], D[5] + B[5]};
+  M[0] = T(1) - dt * Dt[0];
+  M[4] = T(1) - dt * Dt[1];
+  M[8] = T(1) - dt * Dt[2];
+  M[1] = M[3] = -dt * Dt[3];
+  M[2] = M[6] = -dt * Dt[4];
+  M[5] = M[7] = -dt * Dt[5];
+  invert3(M, S.sp + 9 * static_cast<size_t>(i));
+  // B_i for the list product, (x_i, m_i / rho_i) for its gathers
+  S.cb[i] = make4<vec4_t<T>>(B[0], B[1], B[2], T(0));
+  S.G[i] = make4<vec4_t<T>>(B[3], B[4], B[5], T(0));
+  S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w / rhoi);
+}
+
+template <class T>
+__device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrList &L, int i,
+                                             const vec4_t<T> *__restrict__ w, T out[3]) {
+  const vec4_t<T> xq = S.xq[i];  // (x_i, m_i / rho_i)
+  const vec4_t<T> wi = w[i];
+  const T rhoi = S.rho[i];
+  const T cf = T(10) * S.mu_f / rhoi;
+  const size_t n = static_cast<size_t>(S.nf);
+  T l0 = 0, l1 = 0, l2 = 0;
+  auto fluid = [&](uint32_t j) {
+    const vec4_t<T> pj = S.xq[j];
+    const vec4_t<T> wj = w[j];
+    const T d0 = xq.x - pj.x, d1 = xq.y - pj.y, d2 = xq.z - pj.z;
+    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const T s = pj.w * ((wi.x - wj.x) * d0 + (wi.y - wj.y) * d1 + (wi.z - wj.z) * d2) *
+                gradf_fast(S, r2) / (r2 + S.hh001);
+    l0 += s * d0;
+    l1 += s * d1;
+    l2 += s * d2;
+  };
+  const uint32_t cnt = L.count[i];
+  if (cnt <= L.cap) {
+    uint32_t k = 0;
+    for (; k + 2 <= cnt; k += 2) {
+      const uint32_t j0 = L.idx[k * n + i], j1 = L.idx[(k + 1) * n + i];
+      fluid(j0);
+      fluid(j1);
+    }
+    if (k < cnt)
+      fluid(L.idx[k * n + i]);
+  } else {
+    nbrs_exact(S, xq, S.fcell[i], S.fstart, S.xq,
+               [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
+                 if (j != static_cast<uint32_t>(i))
+                   fluid(j);
+               });
+  }
+  const vec4_t<T> bd = S.cb[i], bo = S.G[i];
+  const T b0 = bd.x * wi.x + bo.x * wi.y + bo.y * wi.z;
+  const T b1 = bo.x * wi.x + bd.y * wi.y + bo.z * wi.z;
+  const T b2 = bo.y * wi.x + bo.z * wi.y + bd.z * wi.z;
+  out[0] = wi.x - S.dt * (cf * l0 + b0);
+  out[1] = wi.y - S.dt * (cf * l1 + b1);
+  out[2] = wi.z - S.dt * (cf * l2 + b2);
+}
+
+template <class T>
+__device__ __forceinline__ T dot3r(const vec4_t<T> &a, const vec4_t<T> &b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + (a.y * b.y + a.z * b.z);  // sum(cmul(l, r)) per item
+}
+
+template <class T>
+__device__ __forceinline__ vec4_t<T> precond(const T *M, const vec4_t<T> &w) {
+#pragma clang fp contract(off)
+  vec4_t<T> y;
+  y.x = M[0] * w.x + (M[3] * w.y + M[6] * w.z);
+  y.y = M[1] * w.x + (M[4] * w.y + M[7] * w.z);
+  y.z = M[2] * w.x + (M[5] * w.y + M[8] * w.z);
+  y.w = T(0);
+  return y;
+}
+
+// per-block partial sums of up to three FP64 dot products
+template <int K>
+__device__ __forceinline__ void store_partials(double *partials, unsigned nblocks,
+                                               const double (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double s = block_sum<kBlock>(v[k]);
+    if (threadIdx.x == 0)
+      partials[k * nblocks + blockIdx.x] = s;
+  }
+}
+
+// setup: x = v (guess), r = A x - b with b = v (the rhs of wkbb18.hpp:296-334
+// is v - dt a_b with a_b = 0 because the boundary velocity is zero),
+// y = P r, p = -y; partials of b.b, r.r, r.y
+template <class T, bool kExact>
+__global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, double *part) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < S.nf) {
+    const vec4_t<T> v = S.v[i];
+    T ax[3];
+    if (kExact)
+      product_exact(S, i, S.v, ax);
+    else
+      product_list(S, L, i, S.v, ax);
+    vec4_t<T> b = v;
+    b.w = T(0);
+    vec4_t<T> r;
+    r.x = ax[0] - b.x;
+    r.y = ax[1] - b.y;
+    r.z = ax[2] - b.z;
+    r.w = T(0);
+    const vec4_t<T> y = precond(S.sp + 9 * static_cast<size_t>(i), r);
+    S.cx[i] = b;
+    S.cr[i] = r;
+    S.cy[i] = y;
+    S.cp[i] = make4<vec4_t<T>>(-y.x, -y.y, -y.z, T(0));
+    acc[0] = static_cast<double>(dot3r<T>(b, b));
+    acc[1] = static_cast<double>(dot3r<T>(r, r));
+    acc[2] = static_cast<double>(dot3r<T>(r, y));
+  }
+  store_partials<3>(part, gridDim.x, acc);
+}
+
+__device__ __forceinline__ double sum_partials(const double *p, unsigned n) {
+  double acc = 0.0;
+  for (unsigned k = threadIdx.x; k < n; k += kBlock)
+    acc += p[k];
+  return block_sum<kBlock>(acc);
+}
+
+__device__ __forceinline__ void publish_cg(Ctl *c) {
+  if (c->host_mirror) {
+    const unsigned long long w = (static_cast<unsigned long long>(c->seq) << 32) |
+                                 (static_cast<unsigned long long>(c->cg_done ? 1u : 0u) << 31) |
+                                 static_cast<unsigned long long>(static_cast<unsigned>(c->cg_iter));
+    __hip_atomic_store(c->host_mirror + 2, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *part,
+                                                         unsigned nb, double tol, int max_k,
+                                                         unsigned seq) {
+  const double bb = sum_partials(part, nb);
+  const double rr = sum_partials(part + nb, nb);
+  const double ry = sum_partials(part + 2 * nb, nb);
+  if (threadIdx.x == 0) {
+    c->cg_tol = tol;
+    c->cg_max_k = max_k;
+    c->cg_b_nsq = bb;
+    // thr = max(tol^2 |b|^2, FLT_MIN)   (cg_openmp.hpp:74)
+    const double thr = tol * tol * bb;
+    c->cg_thr = thr > 1.1754943508222875e-38 ? thr : 1.1754943508222875e-38;
+    c->cg_r_nsq = rr;
+    c->cg_ry = ry;
+    c->cg_iter = 0;
+    c->cg_done = 0;  // ok(0) holds: k < min_k = 1
+    c->seq = seq;
+    publish_cg(c);
+  }
+}
+
+// q = A p, partial p.q
+template <class T, bool kExact>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, double *part) {
+  if (S.ctl->cg_done)
+    return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < S.nf) {
+    T q[3];
+    if (kExact)
+      product_exact(S, i, S.cp, q);
+    else
+      product_list(S, L, i, S.cp, q);
+    const vec4_t<T> qv = make4<vec4_t<T>>(q[0], q[1], q[2], T(0));
+    S.cq[i] = qv;
+    acc[0] = static_cast<double>(dot3r<T>(S.cp[i], qv));
+  }
+  store_partials<1>(part, gridDim.x, acc);
+}
+
+// |p.q| < tol breaks (cg_openmp.hpp:147); alpha = (r.y) / (p.q)
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl *c, const double *part, unsigned nb) {
+  if (c->cg_done)
+    return;
+  const double pq = sum_partials(part, nb);
+  if (threadIdx.x == 0) {
+    c->cg_ry_prev = c->cg_ry;
+    c->cg_pq = pq;
+    if (fabs(pq) < c->cg_tol) {
+      c->cg_done = 1;
+      publish_cg(c);
+    } else {
+      c->cg_alpha = c->cg_ry_prev / pq;
+    }
+  }
+}
+
+// x += alpha p, r += alpha q, y = P r; partials r.r, r.y
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(SceneView<T> S, double *part) {
+  if (S.ctl->cg_done)
+    return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < S.nf) {
+    const T alpha = static_cast<T>(S.ctl->cg_alpha);
+    const vec4_t<T> p = S.cp[i], q = S.cq[i];
+    vec4_t<T> x = S.cx[i], r = S.cr[i];
+    x.x += alpha * p.x;
+    x.y += alpha * p.y;
+    x.z += alpha * p.z;
+    r.x += alpha * q.x;
+    r.y += alpha * q.y;
+    r.z += alpha * q.z;
+    const vec4_t<T> y = precond(S.sp + 9 * static_cast<size_t>(i), r);
+    S.cx[i] = x;
+    S.cr[i] = r;
+    S.cy[i] = y;
+    acc[0] = static_cast<double>(dot3r<T>(r, r));
+    acc[1] = static_cast<double>(dot3r<T>(r, y));
+  }
+  store_partials<2>(part, gridDim.x, acc);
+}
+
+// |r.y|_prev < tol breaks (:187); beta; ++k; ok(k, |r|^2) (:100-111)
+__global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, unsigned nb) {
+  if (c->cg_done)
+    return;
+  const double rr = sum_partials(part, nb);
+  const double ry = sum_partials(part + nb, nb);
+  if (threadIdx.x == 0) {
+    c->cg_r_nsq = rr;
+    c->cg_ry = ry;
+    if (fabs(c->cg_ry_prev) < c->cg_tol) {
+      c->cg_done = 1;
+    } else {
+      c->cg_beta = ry / c->cg_ry_prev;
+      const int k = c->cg_iter + 1;
+      c->cg_iter = k;
+      if (k > c->cg_max_k || rr < c->cg_thr)
+        c->cg_done = 1;
+    }
+    publish_cg(c);
+  }
+}
+
+// p = -y + beta p
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_direction(SceneView<T> S) {
+  if (S.ctl->cg_done)
+    return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const T beta = static_cast<T>(S.ctl->cg_beta);
+  const vec4_t<T> y = S.cy[i];
+  vec4_t<T> p = S.cp[i];
+  p.x = -y.x + beta * p.x;
+  p.y = -y.y + beta * p.y;
+  p.z = -y.z + beta * p.z;
+  S.cp[i] = p;
+}
+
+// apply (wkbb18.hpp:401-413): vd = (x - v) H(t - t_birth - dt_fade), a = vd / dt
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_apply(SceneView<T> S) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> x = S.cx[i], v = S.v[i];
+  const T H = step_l(T(0), (S.t - v.w) - S.fade);
+  const vec4_t<T> vd = make4<vec4_t<T>>((x.x - v.x) * H, (x.y - v.y) * H, (x.z - v.z) * H, T(0));
+  S.vd[i] = vd;
+  S.a[i] = make4<vec4_t<T>>(vd.x / S.dt, vd.y / S.dt, vd.z / S.dt, T(0));
+}
+
+}  // namespace
+
+template <class T>
+void Wkbb18Ops<T>::diagonal(hipStream_t st, const SceneView<T> &S, const NbrList *L) {
+  if (S.nf <= 0)
+    return;
+  if (L)
+    hipLaunchKernelGGL(k_diag_list<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, *L);
+  else
+    hipLaunchKernelGGL(k_diag_exact<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+}
+
+template <class T>
+void Wkbb18Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+                         double *part, double tol, int max_k, unsigned seq) {
+  const unsigned nb = grid_blocks(S.nf);
+  if (S.nf > 0) {
+    if (L)
+      hipLaunchKernelGGL((k_cg_setup<T, false>), dim3(nb), dim3(kBlock), 0, st, S, *L, part);
+    else
+      hipLaunchKernelGGL((k_cg_setup<T, true>), dim3(nb), dim3(kBlock), 0, st, S, NbrList{},
+                         part);
+  }
+  hipLaunchKernelGGL(k_cg_setup_fin, dim3(1), dim3(kBlock), 0, st, S.ctl, part, nb, tol, max_k,
+                     seq);
+}
+
+template <class T>
+void Wkbb18Ops<T>::iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+                             double *part) {
+  if (S.nf <= 0)
+    return;
+  const unsigned nb = grid_blocks(S.nf);
+  const dim3 gr(nb), bl(kBlock);
+  if (L)
+    hipLaunchKernelGGL((k_cg_spmv<T, false>), gr, bl, 0, st, S, *L, part);
+  else
+    hipLaunchKernelGGL((k_cg_spmv<T, true>), gr, bl, 0, st, S, NbrList{}, part);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(1), bl, 0, st, S.ctl, part, nb);
+  hipLaunchKernelGGL(k_cg_update<T>, gr, bl, 0, st, S, part);
+  hipLaunchKernelGGL(k_cg_beta, dim3(1), bl, 0, st, S.ctl, part, nb);
+  hipLaunchKernelGGL(k_cg_direction<T>, gr, bl, 0, st, S);
+}
+
+template <class T>
+void Wkbb18Ops<T>::apply(hipStream_t st, const SceneView<T> &S) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_cg_apply<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+}
+
+template struct Wkbb18Ops<float>;
+template struct Wkbb18Ops<double>;
+
+}  // namespace prtcl_amd

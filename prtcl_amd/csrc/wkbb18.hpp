@@ -1,0 +1,24 @@
+// Launchers of the WKBB18 implicit-viscosity kernels (wkbb18.hip). A null
+// list selects the exact (reference operation order, grid walk) product.
+#pragma once
+
+#include "fused.hpp"
+
+namespace prtcl_amd {
+
+template <class T>
+struct Wkbb18Ops {
+  // sp_i = (I - dt D_i)^-1 (and, with a list, the boundary matrix B_i and the
+  // (x, m / rho) gather array of the list product)
+  static void diagonal(hipStream_t st, const SceneView<T> &S, const NbrList *L);
+  // x = v, r = A x - b, y = P r, p = -y and the FP64 set-up scalars
+  static void setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, double *part,
+                    double tol, int max_k, unsigned seq);
+  // one CG iteration (five kernels; all exit at once after convergence)
+  static void iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+                        double *part);
+  // vd = (x - v) H, a = vd / dt
+  static void apply(hipStream_t st, const SceneView<T> &S);
+};
+
+}  // namespace prtcl_amd
