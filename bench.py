@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU baseline sample")
+    ap.add_argument("--batch-scenes", type=int, default=64,
+                    help="scenes of the batched workload (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=20)
     return ap.parse_args()
 
 
@@ -152,7 +155,76 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, g, args.cpu_seconds)
 
+    if args.batch_scenes > 0:
+        g.destroy()
+        out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
+
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(ctx, count, steps, warmup):
+    """The batched workload: `count` independent dam-break scenes (N_f
+    log-uniform in [2k, 200k], uneven aspect ratios, random initial
+    velocities, each with its own IISPH convergence at 0.1%). Two identical
+    copies are stepped from the same initial state: one with one
+    prtcl_hip_step call per scene (the plain per-scene path), one with a
+    single call over all scenes (one launch per stage for the whole batch).
+    Both do the same work (their results are bit-identical)."""
+    from prtcl_amd import hip, scenes
+    scs = scenes.config3(count=count)
+    nf_total = sum(s.n_fluid for s in scs)
+
+    def make():
+        hs = [hip.HipScene(ctx, s) for s in scs]
+        for h in hs:
+            h.prepare()
+        return hs
+
+    def timed(hs, together):
+        for _ in range(warmup):
+            if together:
+                ctx.step(hs, 1)
+            else:
+                for h in hs:
+                    h.step(1)
+        ctx.synchronize()
+        lat, its = [], []
+        t0 = prev = time.perf_counter()
+        for _ in range(steps):
+            if together:
+                ctx.step(hs, 1)
+            else:
+                for h in hs:
+                    h.step(1)
+            now = time.perf_counter()
+            lat.append(now - prev)
+            prev = now
+            its.append(max(h.step_info()["iisph_iterations"] for h in hs))
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        return el, lat, its
+
+    per = make()
+    el_p, lat_p, _ = timed(per, False)
+    for h in per:
+        h.destroy()
+    bat = make()
+    el_b, lat_b, its_b = timed(bat, True)
+    for h in bat:
+        h.destroy()
+    v_b = nf_total * steps / el_b
+    v_p = nf_total * steps / el_p
+    return {
+        "workload": f"batch{count}", "scenes": count, "fluid_particles_total": nf_total,
+        "steps": steps, "warmup": warmup,
+        "value": round(v_b, 1), "unit": "particle-steps/s",
+        "per_scene_value": round(v_p, 1), "speedup_vs_per_scene": round(v_b / v_p, 3),
+        "p50_step_ms": round(1e3 * float(np.percentile(lat_b, 50)), 4),
+        "p99_step_ms": round(1e3 * float(np.percentile(lat_b, 99)), 4),
+        "per_scene_p50_step_ms": round(1e3 * float(np.percentile(lat_p, 50)), 4),
+        "per_scene_p99_step_ms": round(1e3 * float(np.percentile(lat_p, 99)), 4),
+        "max_iisph_iterations_mean": round(float(np.mean(its_b)), 2),
+    }
 
 
 def cpu_baseline(sc, g, seconds):

@@ -174,6 +174,7 @@ int prtcl_hip_ctx_destroy(prtcl_hip_ctx *ctx) {
   if (ctx->res.stream)
     (void)hipStreamSynchronize(ctx->res.stream);
   ctx->scenes.clear();
+  ctx->res.release();
   if (ctx->res.staging)
     (void)hipFree(ctx->res.staging);
   if (ctx->res.pinned)
@@ -328,17 +329,10 @@ int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
   return guarded(ctx, [&]() -> int {
     if (!scenes || n_scenes < 0 || k_steps < 0)
       return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid scene list");
-    int status = PRTCL_HIP_OK;
     std::vector<SceneBase *> list;
     for (int i = 0; i < n_scenes; ++i)
       list.push_back(find(ctx, scenes[i]));
-    for (int k = 0; k < k_steps; ++k)
-      for (SceneBase *s : list) {
-        int rc = s->step(1);
-        if (rc != PRTCL_HIP_OK && status == PRTCL_HIP_OK)
-          status = rc;
-      }
-    return status;
+    return step_scenes(list, k_steps, &ctx->res);
   });
 }
 

@@ -167,9 +167,9 @@ __device__ __forceinline__ void walk(const SceneView<T> &S, const NbrList &L, in
 // viscosity: v*_j = v_j + dt g H_j)
 // ---------------------------------------------------------------------------
 template <class T, bool kWithSource>
-__global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
-                                                  vec4_t<T> *vout) {
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void k_setup_body(SceneView<T> S, NbrList L,
+                                                  vec4_t<T> *vout, unsigned lb) {
+  
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -311,6 +311,12 @@ __global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
   }
 }
 
+template <class T, bool kWithSource>
+__global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
+                                                  vec4_t<T> *vout) {
+  k_setup_body<T, kWithSource>(S, L, vout, xcd_block(blockIdx.x, gridDim.x));
+}
+
 // explicit viscosity (viscosity.hpp:171-240) over the list, then
 // v* = v + dt (g + a_visc) H   (gravity.hpp:133, symplectic_euler.hpp:172-200)
 template <class T>
@@ -385,10 +391,10 @@ __global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList 
 
 // IISPH pressure acceleration over the list: a_i = p_i c_i - sum m_j pr_j gradW
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_accel_list(SceneView<T> S, NbrList L) {
+__device__ __forceinline__ void k_accel_list_body(SceneView<T> S, NbrList L, unsigned lb) {
   if (S.ctl->done)
     return;
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -410,13 +416,18 @@ __global__ __launch_bounds__(kBlock) void k_accel_list(SceneView<T> S, NbrList L
                             pi * ci.z - K.ag3 * a2, T(0));
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_accel_list(SceneView<T> S, NbrList L) {
+  k_accel_list_body<T>(S, L, xcd_block(blockIdx.x, gridDim.x));
+}
+
 // IISPH relaxed Jacobi update over the list, FP64 aprde reduction and the
 // device-side convergence decision (example.lua:409-421)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pressure_list(SceneView<T> S, NbrList L) {
+__device__ __forceinline__ void k_pressure_list_body(SceneView<T> S, NbrList L, unsigned lb) {
   if (S.ctl->done)
     return;
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double term = 0.0;
   if (i < S.nf) {
@@ -446,14 +457,19 @@ __global__ __launch_bounds__(kBlock) void k_pressure_list(SceneView<T> S, NbrLis
   }
   const double bs = block_sum<kBlock>(term);
   if (threadIdx.x == 0)
-    S.partials[blockIdx.x] = bs;
+    S.partials[lb] = bs;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pressure_list(SceneView<T> S, NbrList L) {
+  k_pressure_list_body<T>(S, L, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Sum of the block partials in block order (FP64, fixed tree) and the
 // convergence decision of example.lua:418-419: stop when it >= min_it and
 // aprde / nprde < tol, or at max_it.
-__global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const double *partials,
-                                                              unsigned nparts) {
+__device__ __forceinline__ void k_pressure_finalize_body(Ctl *c, const double *partials,
+                                                        unsigned nparts) {
   if (c->done)
     return;
   double acc = 0.0;
@@ -474,6 +490,11 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const doub
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const double *partials,
+                                                              unsigned nparts) {
+  k_pressure_finalize_body(c, partials, nparts);
+}
+
 template <class T>
 __device__ __forceinline__ void max_speed_atomic(Ctl *ctl, T s);
 template <>
@@ -487,8 +508,8 @@ __device__ __forceinline__ void max_speed_atomic<double>(Ctl *ctl, double s) {
 
 // v += dt a H (pressure acceleration), x += dt v, running max |v|
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_integrate(SceneView<T> S) {
-  const int i = static_cast<int>(blockIdx.x * kBlock + threadIdx.x);
+__device__ __forceinline__ void k_integrate_body(SceneView<T> S, unsigned lb) {
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   T speed = T(0);
   bool bad = false;
   if (i < S.nf) {
@@ -511,21 +532,78 @@ __global__ __launch_bounds__(kBlock) void k_integrate(SceneView<T> S) {
   }
   speed = block_max<kBlock>(speed);
   if (threadIdx.x == 0)
-    S.partials[blockIdx.x] = static_cast<double>(speed);
+    S.partials[lb] = static_cast<double>(speed);
   if (bad)
     S.ctl->nonfinite = 1;
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_integrate(SceneView<T> S) {
+  k_integrate_body<T>(S, blockIdx.x);
+}
+
 // running maximum speed (symplectic_euler.hpp:250-268) from the block maxima
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double *partials,
-                                                           unsigned nparts) {
+__device__ __forceinline__ void k_speed_finalize_body(Ctl *c, const double *partials,
+                                                      unsigned nparts) {
   double m = 0.0;
   for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
     m = partials[k] > m ? partials[k] : m;
   m = block_max<kBlock>(m);
   if (threadIdx.x == 0)
     max_speed_atomic<T>(c, static_cast<T>(m));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double *partials,
+                                                           unsigned nparts) {
+  k_speed_finalize_body<T>(c, partials, nparts);
+}
+
+// ---- batched wrappers (prtcl_hip_step over several scenes)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_setup_b(const BatchEntry<T> *tab,
+                                                    const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_setup_body<T, true>(E.S, NbrList{E.nidx, E.ncnt, kNbrCap}, E.vtmp, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_accel_b(const BatchEntry<T> *tab,
+                                                    const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_accel_list_body<T>(E.S, NbrList{E.nidx, E.ncnt, kNbrCap}, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pressure_b(const BatchEntry<T> *tab,
+                                                       const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_pressure_list_body<T>(E.S, NbrList{E.nidx, E.ncnt, kNbrCap}, u & kBatchLocalMask);
+}
+
+// one block per scene
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pressure_finalize_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  k_pressure_finalize_body(E.S.ctl, E.S.partials, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_integrate_b(const BatchEntry<T> *tab,
+                                                        const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_integrate_body<T>(E.S, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_speed_finalize_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  k_speed_finalize_body<T>(E.S.ctl, E.S.partials, E.nblocks);
 }
 
 }  // namespace
@@ -570,6 +648,38 @@ void FusedOps<T>::integrate(hipStream_t st, const SceneView<T> &S) {
   const unsigned nb = grid_blocks(S.nf);
   hipLaunchKernelGGL(k_integrate<T>, dim3(nb), dim3(kBlock), 0, st, S);
   hipLaunchKernelGGL(k_speed_finalize<T>, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials, nb);
+}
+
+template <class T>
+void FusedOps<T>::setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                              unsigned npblocks) {
+  if (npblocks > 0)
+    hipLaunchKernelGGL(k_setup_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+}
+
+template <class T>
+void FusedOps<T>::accel_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                              unsigned npblocks) {
+  if (npblocks > 0)
+    hipLaunchKernelGGL(k_accel_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+}
+
+template <class T>
+void FusedOps<T>::pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                                 unsigned npblocks, unsigned nscenes) {
+  if (npblocks == 0)
+    return;
+  hipLaunchKernelGGL(k_pressure_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  hipLaunchKernelGGL(k_pressure_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
+}
+
+template <class T>
+void FusedOps<T>::integrate_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                                  unsigned npblocks, unsigned nscenes) {
+  if (npblocks == 0)
+    return;
+  hipLaunchKernelGGL(k_integrate_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  hipLaunchKernelGGL(k_speed_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
 }
 
 template struct FusedOps<float>;

@@ -22,6 +22,15 @@ struct FusedOps {
   static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L);
   static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L);
   static void integrate(hipStream_t st, const SceneView<T> &S);
+  // the same passes over every scene of a batch (pmap: block map)
+  static void setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                          unsigned npblocks);
+  static void accel_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                          unsigned npblocks);
+  static void pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                             unsigned npblocks, unsigned nscenes);
+  static void integrate_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                              unsigned npblocks, unsigned nscenes);
 };
 
 }  // namespace prtcl_amd

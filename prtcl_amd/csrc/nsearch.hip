@@ -55,9 +55,9 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
   return base + incl - v;
 }
 
-__global__ __launch_bounds__(kScanBS) void k_scan_reduce(const uint32_t *count, uint32_t n,
-                                                         uint32_t *tile_sums) {
-  const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile;
+__device__ __forceinline__ void k_scan_reduce_body(const uint32_t *count, uint32_t n,
+                                                         uint32_t *tile_sums, unsigned bid) {
+  const size_t base = static_cast<size_t>(bid) * kScanTile;
   uint32_t s = 0;
   for (int k = 0; k < kScanItems; ++k) {
     size_t i = base + static_cast<size_t>(k) * kScanBS + threadIdx.x;
@@ -66,7 +66,12 @@ __global__ __launch_bounds__(kScanBS) void k_scan_reduce(const uint32_t *count, 
   }
   s = block_sum<kScanBS>(s);
   if (threadIdx.x == 0)
-    tile_sums[blockIdx.x] = s;
+    tile_sums[bid] = s;
+}
+
+__global__ __launch_bounds__(kScanBS) void k_scan_reduce(const uint32_t *count, uint32_t n,
+                                                         uint32_t *tile_sums) {
+  k_scan_reduce_body(count, n, tile_sums, blockIdx.x);
 }
 
 // one block: exclusive scan of the tile sums in place, total into start[n]
@@ -86,10 +91,10 @@ __global__ __launch_bounds__(kScanBS) void k_scan_top(uint32_t *tile_sums, uint3
     start[n] = carry;
 }
 
-__global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t n,
+__device__ __forceinline__ void k_scan_down_body(uint32_t *count, uint32_t n,
                                                        const uint32_t *tile_sums,
-                                                       uint32_t *start) {
-  const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile +
+                                                       uint32_t *start, unsigned bid) {
+  const size_t base = static_cast<size_t>(bid) * kScanTile +
                       static_cast<size_t>(threadIdx.x) * kScanItems;
   uint32_t vals[kScanItems];
   uint32_t s = 0;
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t
   for (int k = 0; k < kScanItems; ++k)
     s += vals[k];
   uint32_t tot;
-  uint32_t ex = block_excl_scan(s, &tot) + tile_sums[blockIdx.x];
+  uint32_t ex = block_excl_scan(s, &tot) + tile_sums[bid];
   if (full) {
 #pragma unroll
     for (int k = 0; k < kScanItems / 4; ++k) {
@@ -143,11 +148,17 @@ __global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t
   }
 }
 
+__global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t n,
+                                                       const uint32_t *tile_sums,
+                                                       uint32_t *start) {
+  k_scan_down_body(count, n, tile_sums, start, blockIdx.x);
+}
+
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_cell_keys(GridDims g, const vec4_t<T> *x, int n,
+__device__ __forceinline__ void k_cell_keys_body(GridDims g, const vec4_t<T> *x, int n,
                                                       uint32_t *cell, uint32_t *count,
-                                                      uint32_t *rank) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+                                                      uint32_t *rank, unsigned bid) {
+  const int i = bid * kBlock + threadIdx.x;
   uint32_t lin = 0xFFFFFFFFu;
   if (i < n) {
     const vec4_t<T> p = x[i];
@@ -189,11 +200,18 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(GridDims g, const vec4_t<T
     rank[i] = r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(GridDims g, const uint32_t *cell,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cell_keys(GridDims g, const vec4_t<T> *x, int n,
+                                                      uint32_t *cell, uint32_t *count,
+                                                      uint32_t *rank) {
+  k_cell_keys_body<T>(g, x, n, cell, count, rank, blockIdx.x);
+}
+
+__device__ __forceinline__ void k_scatter_body(GridDims g, const uint32_t *cell,
                                                     const uint32_t *rank,
                                                     const uint32_t *start, int n,
-                                                    uint32_t *perm) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+                                                    uint32_t *perm, unsigned bid) {
+  const int i = bid * kBlock + threadIdx.x;
   if (i >= n)
     return;
   const uint32_t c = cell[i];
@@ -203,15 +221,22 @@ __global__ __launch_bounds__(kBlock) void k_scatter(GridDims g, const uint32_t *
   perm[start[lin] + rank[i]] = static_cast<uint32_t>(i);
 }
 
+__global__ __launch_bounds__(kBlock) void k_scatter(GridDims g, const uint32_t *cell,
+                                                    const uint32_t *rank,
+                                                    const uint32_t *start, int n,
+                                                    uint32_t *perm) {
+  k_scatter_body(g, cell, rank, start, n, perm, blockIdx.x);
+}
+
 // Stable placement: the atomic ranks of k_cell_keys are scheduling
 // dependent, so every particle recomputes its rank inside its cell as the
 // number of cell members with a smaller previous slot. The result is a pure
 // function of the previous order (a stable counting sort).
-__global__ __launch_bounds__(kBlock) void k_stable_rank(GridDims g, const uint32_t *cell,
+__device__ __forceinline__ void k_stable_rank_body(GridDims g, const uint32_t *cell,
                                                         const uint32_t *start,
                                                         const uint32_t *perm, int n,
-                                                        uint32_t *perm_out) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
+                                                        uint32_t *perm_out, unsigned bid) {
+  const int s = bid * kBlock + threadIdx.x;
   if (s >= n)
     return;
   const uint32_t o = perm[s];
@@ -224,6 +249,13 @@ __global__ __launch_bounds__(kBlock) void k_stable_rank(GridDims g, const uint32
   for (uint32_t k = b; k < e; ++k)
     r += perm[k] < o ? 1u : 0u;
   perm_out[b + r] = o;
+}
+
+__global__ __launch_bounds__(kBlock) void k_stable_rank(GridDims g, const uint32_t *cell,
+                                                        const uint32_t *start,
+                                                        const uint32_t *perm, int n,
+                                                        uint32_t *perm_out) {
+  k_stable_rank_body(g, cell, start, perm, n, perm_out, blockIdx.x);
 }
 
 template <int W>
@@ -242,14 +274,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(E *__restrict__ dst,
 
 // the state carried between steps, reordered in one pass
 template <class V>
-__global__ __launch_bounds__(kBlock) void k_gather_state(V *__restrict__ xd, const V *__restrict__ xs,
+__device__ __forceinline__ void k_gather_state_body(V *__restrict__ xd, const V *__restrict__ xs,
                                                          V *__restrict__ vd, const V *__restrict__ vs,
                                                          uint32_t *__restrict__ id_d,
                                                          const uint32_t *__restrict__ id_s,
                                                          uint32_t *__restrict__ c_d,
                                                          const uint32_t *__restrict__ c_s,
-                                                         const uint32_t *__restrict__ perm, int n) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+                                                         const uint32_t *__restrict__ perm, int n, unsigned bid) {
+  const int i = bid * kBlock + threadIdx.x;
   if (i >= n)
     return;
   const uint32_t o = perm[i];
@@ -257,6 +289,87 @@ __global__ __launch_bounds__(kBlock) void k_gather_state(V *__restrict__ xd, con
   vd[i] = vs[o];
   id_d[i] = id_s[o];
   c_d[i] = c_s[o];
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_gather_state(V *__restrict__ xd, const V *__restrict__ xs,
+                                                         V *__restrict__ vd, const V *__restrict__ vs,
+                                                         uint32_t *__restrict__ id_d,
+                                                         const uint32_t *__restrict__ id_s,
+                                                         uint32_t *__restrict__ c_d,
+                                                         const uint32_t *__restrict__ c_s,
+                                                         const uint32_t *__restrict__ perm, int n) {
+  k_gather_state_body<V>(xd, xs, vd, vs, id_d, id_s, c_d, c_s, perm, n, blockIdx.x);
+}
+
+// ---- batched wrappers: one launch over the blocks of every scene of a batch
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cell_keys_b(const BatchEntry<T> *tab,
+                                                        const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_cell_keys_body<T>(E.S.grid, E.S.xm, E.S.nf, E.S.fcell, E.count, E.rank,
+                      u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanBS) void k_scan_reduce_b(const BatchEntry<T> *tab,
+                                                           const uint32_t *tmap) {
+  const uint32_t u = tmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_scan_reduce_body(E.count, E.S.grid.ncells, E.scan_tmp, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanBS) void k_scan_top_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < E.ntiles; b += kScanBS) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t v = i < E.ntiles ? E.scan_tmp[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, &tot);
+    if (i < E.ntiles)
+      E.scan_tmp[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0)
+    E.start[E.S.grid.ncells] = carry;
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanBS) void k_scan_down_b(const BatchEntry<T> *tab,
+                                                         const uint32_t *tmap) {
+  const uint32_t u = tmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_scan_down_body(E.count, E.S.grid.ncells, E.scan_tmp, E.start, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scatter_b(const BatchEntry<T> *tab,
+                                                      const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_scatter_body(E.S.grid, E.S.fcell, E.rank, E.start, E.S.nf, E.perm, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_stable_rank_b(const BatchEntry<T> *tab,
+                                                          const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_stable_rank_body(E.S.grid, E.S.fcell, E.start, E.perm, E.S.nf, E.rank,
+                     u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather_state_b(const BatchEntry<T> *tab,
+                                                           const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_gather_state_body<vec4_t<T>>(E.x_dst, E.S.xm, E.v_dst, E.S.v, E.id_dst,
+                                 reinterpret_cast<const uint32_t *>(E.S.id), E.cell_dst,
+                                 E.S.fcell, E.rank, E.S.nf, u & kBatchLocalMask);
 }
 
 }  // namespace
@@ -345,6 +458,27 @@ void GridOps<T>::gather_state(hipStream_t st, vec4_t<T> *xd, const vec4_t<T> *xs
     hipLaunchKernelGGL(k_gather_state<vec4_t<T>>, dim3(grid_blocks(n)), dim3(kBlock), 0, st,
                        xd, xs, vd, vs, id_d, id_s, c_d, c_s, perm, n);
 }
+
+template <class T>
+void GridOps<T>::sort_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                            unsigned npblocks, const uint32_t *tmap, unsigned ntiles,
+                            unsigned nscenes) {
+  if (npblocks > 0)
+    hipLaunchKernelGGL(k_cell_keys_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_scan_reduce_b<T>, dim3(ntiles), dim3(kScanBS), 0, st, tab, tmap);
+  if (nscenes > 0)
+    hipLaunchKernelGGL(k_scan_top_b<T>, dim3(nscenes), dim3(kScanBS), 0, st, tab);
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_scan_down_b<T>, dim3(ntiles), dim3(kScanBS), 0, st, tab, tmap);
+  if (npblocks > 0) {
+    hipLaunchKernelGGL(k_scatter_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+    hipLaunchKernelGGL(k_stable_rank_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+    hipLaunchKernelGGL(k_gather_state_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  }
+}
+
+unsigned scan_tiles(uint32_t ncells) { return (ncells + kScanTile - 1) / kScanTile; }
 
 template struct GridOps<float>;
 template struct GridOps<double>;

@@ -101,6 +101,29 @@ struct SceneView {
   V4 *cx, *cr, *cp, *cq, *cy, *cb;
 };
 
+// One scene of a batch (prtcl_hip_step over several scenes): its view and
+// the buffers the batched sort and setup use. A batched kernel maps its block
+// to (scene, local block) through a block map of words (scene << 20 | block).
+template <class T>
+struct BatchEntry {
+  SceneView<T> S;
+  uint32_t *nidx, *ncnt;      // neighbour list (NbrList of the scene)
+  vec4_t<T> *vtmp;            // integrated velocity written by the setup pass
+  uint32_t *count, *start;    // fluid grid counts / starts
+  uint32_t *rank, *perm;      // sort buffers (result in rank)
+  uint32_t *scan_tmp;         // tile sums
+  vec4_t<T> *x_dst, *v_dst;   // gather destinations
+  uint32_t *id_dst, *cell_dst;
+  unsigned nblocks;           // fluid blocks of kBlock particles
+  unsigned ntiles;            // scan tiles of the grid
+  // parameters of the scene's IISPH solve in this step
+  double tol;
+  int min_it, max_it, nprde;
+  unsigned seq;
+};
+constexpr uint32_t kBatchLocalBits = 20;
+constexpr uint32_t kBatchLocalMask = (1u << kBatchLocalBits) - 1u;
+
 inline unsigned grid_blocks(long n, int bs = kBlock) {
   return static_cast<unsigned>((n + bs - 1) / bs);
 }
@@ -129,6 +152,11 @@ struct GridOps {
   static void gather4(hipStream_t st, void *dst, const void *src, const uint32_t *perm, int n);
   static void gather_elems(hipStream_t st, void *dst, const void *src,
                            const uint32_t *perm, int n, int elem_bytes);
+  // the whole fluid sort (keys .. gather) of every scene of a batch; the
+  // permutation of each scene ends in its `rank` buffer
+  static void sort_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
+                         unsigned npblocks, const uint32_t *tmap, unsigned ntiles,
+                         unsigned nscenes);
   // x, v (real4), id and cell of the carried state, reordered in one pass
   static void gather_state(hipStream_t st, vec4_t<T> *xd, const vec4_t<T> *xs, vec4_t<T> *vd,
                            const vec4_t<T> *vs, uint32_t *id_d, const uint32_t *id_s,
@@ -139,5 +167,6 @@ struct GridOps {
 void scan_counts(hipStream_t st, uint32_t *count, uint32_t *start, uint32_t n,
                  uint32_t *tmp);
 size_t scan_tmp_words(uint32_t n);
+unsigned scan_tiles(uint32_t ncells);
 
 }  // namespace prtcl_amd

@@ -27,6 +27,37 @@ void *CtxResources::ensure_staging(size_t bytes) {
   return staging;
 }
 
+void CtxResources::ensure_batch(size_t bytes) {
+  if (!batch_ev)
+    PRTCL_HIP_CHECK(hipEventCreateWithFlags(&batch_ev, hipEventDisableTiming));
+  if (bytes <= batch_bytes)
+    return;
+  // kernels of earlier batches may still read the old tables
+  PRTCL_HIP_CHECK(hipStreamSynchronize(stream));
+  if (batch_host)
+    PRTCL_HIP_CHECK(hipHostFree(batch_host));
+  if (batch_dev)
+    PRTCL_HIP_CHECK(hipFree(batch_dev));
+  batch_host = batch_dev = nullptr;
+  batch_bytes = 0;
+  const size_t b = std::max<size_t>(bytes, 64 << 10);
+  PRTCL_HIP_CHECK(hipHostMalloc(&batch_host, b, hipHostMallocDefault));
+  PRTCL_HIP_CHECK(hipMalloc(&batch_dev, b));
+  batch_bytes = b;
+}
+
+void CtxResources::release() {
+  if (batch_host)
+    (void)hipHostFree(batch_host);
+  if (batch_dev)
+    (void)hipFree(batch_dev);
+  if (batch_ev)
+    (void)hipEventDestroy(batch_ev);
+  batch_host = batch_dev = nullptr;
+  batch_ev = nullptr;
+  batch_bytes = 0;
+}
+
 const char *stage_name(int s) {
   static const char *names[kStageCount] = {
       "nsearch_sort",      "density",       "gravity_integrate_v", "viscosity",
@@ -121,6 +152,20 @@ __global__ void k_iota(int32_t *id, int n) {
 enum CtlOp { kCtlInit = 0, kCtlBeginSolve = 1, kCtlSetNprde = 2, kCtlSetAprde = 3,
              kCtlSetMaxSpeed = 4, kCtlSetNprdeValue = 5 };
 
+__device__ void begin_solve(Ctl *c, double tol, int min_it, int max_it, unsigned seq) {
+  c->aprde = 0.0;
+  c->rel = 0.0;
+  c->tol = tol;
+  c->iters = 0;
+  c->done = (c->nprde >= 1) ? 0 : 1;
+  c->converged = 0;
+  c->min_it = min_it;
+  c->max_it = max_it;
+  c->ticket = 0u;
+  c->seq = seq;
+  publish_progress(c);
+}
+
 __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2,
                       unsigned long long *mirror) {
   switch (op) {
@@ -128,17 +173,7 @@ __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2,
       c->host_mirror = mirror;
       break;
     case kCtlBeginSolve:
-      c->aprde = 0.0;
-      c->rel = 0.0;
-      c->tol = dv;
-      c->iters = 0;
-      c->done = (c->nprde >= 1) ? 0 : 1;
-      c->converged = 0;
-      c->min_it = i0;
-      c->max_it = i1;
-      c->ticket = 0u;
-      c->seq = static_cast<unsigned>(i2);
-      publish_progress(c);
+      begin_solve(c, dv, i0, i1, static_cast<unsigned>(i2));
       break;
     case kCtlSetNprde:
     case kCtlSetNprdeValue:
@@ -152,6 +187,18 @@ __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2,
       c->max_speed_bits_d = static_cast<unsigned long long>(__double_as_longlong(dv));
       break;
   }
+}
+
+// nprde and the start of the IISPH solve of every scene of a batch (one
+// thread per scene)
+template <class T>
+__global__ void k_begin_solve_b(const BatchEntry<T> *tab, unsigned nscenes) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nscenes)
+    return;
+  const BatchEntry<T> &E = tab[k];
+  E.S.ctl->nprde = E.nprde;
+  begin_solve(E.S.ctl, E.tol, E.min_it, E.max_it, E.seq);
 }
 
 struct Arena {
@@ -863,6 +910,214 @@ class SceneImpl final : public SceneBase {
     return status;
   }
 
+  // ------------------------------------------------------- batched stepping
+  // a scene the batched step can take: fused passes, no viscosity stage
+  bool batchable() const {
+    return desc.step_mode == 0 && !desc.explicit_viscosity && !desc.implicit_viscosity &&
+           nf_ > 0 && desc.iisph_max_iterations > 0 &&
+           grid_blocks(nf_) <= kBatchLocalMask;
+  }
+
+  // One step of every scene in `v` (all batchable, at most 1 << 12), with one
+  // launch per stage over the blocks of all scenes. Per scene the kernels,
+  // their inputs and the order of every reduction are those of step(), so
+  // each scene's results do not depend on the rest of the batch.
+  static int step_batch(const std::vector<SceneImpl *> &v, CtxResources *res) {
+    const unsigned ns = static_cast<unsigned>(v.size());
+    if (ns == 0)
+      return PRTCL_HIP_OK;
+    hipStream_t st = res->stream;
+    Profiler &pf = res->prof;
+    // boundary grids and volumes (first step after an upload only)
+    for (SceneImpl *sc : v) {
+      sc->ensure_grid();
+      if (sc->bgrid_dirty_) {
+        sc->build_boundary_grid();
+        SphOps<T>::volume(st, sc->view());
+      }
+    }
+    // block maps: logical block b of the batch (scene order, then local
+    // block) is dispatched as hardware block xcd(b), as in the single-scene
+    // kernels, so each XCD takes a contiguous run of cell-ordered particles
+    std::vector<uint32_t> plist, tlist;
+    for (unsigned k = 0; k < ns; ++k) {
+      const unsigned nb = grid_blocks(v[k]->nf_);
+      for (unsigned b = 0; b < nb; ++b)
+        plist.push_back((k << kBatchLocalBits) | b);
+      const unsigned nt = scan_tiles(v[k]->gd_.ncells);
+      for (unsigned b = 0; b < nt; ++b)
+        tlist.push_back((k << kBatchLocalBits) | b);
+    }
+    const unsigned np = static_cast<unsigned>(plist.size());
+    const unsigned nt = static_cast<unsigned>(tlist.size());
+    // pinned / device table layout: [sort | setup | solve] entries, pmap, tmap
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t tab_bytes = align(sizeof(BatchEntry<T>) * ns);
+    const size_t off_pmap = 3 * tab_bytes;
+    const size_t off_tmap = off_pmap + align(sizeof(uint32_t) * np);
+    const size_t total = off_tmap + align(sizeof(uint32_t) * nt);
+    res->ensure_batch(total);
+    // the previous batch's copy out of the pinned tables must have finished
+    PRTCL_HIP_CHECK(hipEventSynchronize(res->batch_ev));
+    char *hb = static_cast<char *>(res->batch_host);
+    char *db = static_cast<char *>(res->batch_dev);
+    auto *t_sort = reinterpret_cast<BatchEntry<T> *>(hb);
+    auto *t_setup = reinterpret_cast<BatchEntry<T> *>(hb + tab_bytes);
+    auto *t_solve = reinterpret_cast<BatchEntry<T> *>(hb + 2 * tab_bytes);
+    uint32_t *pmap = reinterpret_cast<uint32_t *>(hb + off_pmap);
+    uint32_t *tmap = reinterpret_cast<uint32_t *>(hb + off_tmap);
+    for (unsigned b = 0; b < np; ++b)
+      pmap[b] = plist[xcd_of(b, np)];
+    for (unsigned b = 0; b < nt; ++b)
+      tmap[b] = tlist[b];
+    // the pointer swaps of build_fluid_grid() and of the setup pass are done
+    // on the host while the three tables are filled
+    std::vector<char> permute(ns, 0);
+    std::vector<unsigned> seqs(ns);
+    for (unsigned k = 0; k < ns; ++k) {
+      SceneImpl &s = *v[k];
+      BatchEntry<T> E{};
+      E.S = s.view();
+      E.nidx = s.P_.nidx;
+      E.ncnt = s.P_.ncnt;
+      E.count = s.G_.count;
+      E.start = s.G_.fstart;
+      E.rank = s.P_.frank;
+      E.perm = s.P_.fperm;
+      E.scan_tmp = s.G_.scan_tmp;
+      E.x_dst = s.P_.xq;
+      E.v_dst = s.P_.tmpV4;
+      E.id_dst = reinterpret_cast<uint32_t *>(s.P_.tmpI);
+      E.cell_dst = s.P_.fperm;
+      E.nblocks = grid_blocks(s.nf_);
+      E.ntiles = scan_tiles(s.gd_.ncells);
+      t_sort[k] = E;
+      std::swap(s.P_.xm, s.P_.xq);
+      std::swap(s.P_.v, s.P_.tmpV4);
+      std::swap(s.P_.id, s.P_.tmpI);
+      std::swap(s.P_.fcell, s.P_.fperm);
+      s.fgrid_dirty_ = false;
+      s.grid_built_ = true;
+      s.steps_since_permute_ += 1;
+      if (s.desc.permute_every > 0 && s.steps_since_permute_ >= s.desc.permute_every) {
+        permute[k] = 1;
+        s.steps_since_permute_ = 0;
+      }
+      E.S = s.view();
+      E.vtmp = s.P_.tmpV4;
+      t_setup[k] = E;
+      std::swap(s.P_.v, s.P_.tmpV4);
+      E.S = s.view();
+      E.vtmp = nullptr;
+      seqs[k] = ++s.seq_;
+      E.tol = s.desc.iisph_tolerance;
+      E.min_it = s.desc.iisph_min_iterations;
+      E.max_it = s.desc.iisph_max_iterations;
+      E.nprde = s.nf_;
+      E.seq = seqs[k];
+      t_solve[k] = E;
+    }
+    PRTCL_HIP_CHECK(hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, st));
+    PRTCL_HIP_CHECK(hipEventRecord(res->batch_ev, st));
+    const auto *d_sort = reinterpret_cast<const BatchEntry<T> *>(db);
+    const auto *d_setup = reinterpret_cast<const BatchEntry<T> *>(db + tab_bytes);
+    const auto *d_solve = reinterpret_cast<const BatchEntry<T> *>(db + 2 * tab_bytes);
+    const auto *d_pmap = reinterpret_cast<const uint32_t *>(db + off_pmap);
+    const auto *d_tmap = reinterpret_cast<const uint32_t *>(db + off_tmap);
+
+    int r = pf.begin(st, kStageSort);
+    GridOps<T>::sort_batch(st, d_sort, d_pmap, np, d_tmap, nt, ns);
+    pf.end(st, r);
+    for (unsigned k = 0; k < ns; ++k)
+      if (permute[k]) {
+        SceneImpl &s = *v[k];
+        hipLaunchKernelGGL(k_iota, dim3(grid_blocks(s.nf_)), dim3(kBlock), 0, st, s.P_.id, s.nf_);
+        if (s.nb_ > 0)
+          hipLaunchKernelGGL(k_iota, dim3(grid_blocks(s.nb_)), dim3(kBlock), 0, st, s.P_.bid,
+                             s.nb_);
+      }
+    r = pf.begin(st, kStageSetup);
+    FusedOps<T>::setup_batch(st, d_setup, d_pmap, np);
+    pf.end(st, r);
+    hipLaunchKernelGGL(k_begin_solve_b<T>, dim3(grid_blocks(ns, 64)), dim3(64), 0, st, d_solve,
+                       ns);
+
+    // IISPH iterations of all scenes; a scene whose solve is done exits its
+    // blocks at its flag while the others go on
+    int pred = 1, max_it = 0;
+    for (SceneImpl *sc : v) {
+      pred = std::max(pred, std::max(sc->desc.iisph_min_iterations,
+                                     std::min(sc->predicted_iters_, 8)));
+      max_it = std::max(max_it, static_cast<int>(sc->desc.iisph_max_iterations));
+    }
+    int enq = 0;
+    auto enqueue = [&](int n) {
+      for (int j = 0; j < n; ++j) {
+        int q = pf.begin(st, kStagePressureAccel, enq + j);
+        FusedOps<T>::accel_batch(st, d_solve, d_pmap, np);
+        pf.end(st, q);
+        q = pf.begin(st, kStagePressure, enq + j);
+        FusedOps<T>::pressure_batch(st, d_solve, d_pmap, np, ns);
+        pf.end(st, q);
+      }
+      enq += n;
+    };
+    std::vector<Progress> prog(ns);
+    std::vector<char> finished(ns, 0);
+    enqueue(std::min(pred, max_it));
+    for (;;) {
+      const int ahead = enq < max_it ? 1 : 0;
+      if (ahead)
+        enqueue(1);
+      bool all = true;
+      for (unsigned k = 0; k < ns; ++k) {
+        if (finished[k])
+          continue;
+        prog[k] = v[k]->wait_progress(seqs[k], ahead ? enq - ahead : enq);
+        if (prog[k].done || !ahead)
+          finished[k] = 1;
+        else
+          all = false;
+      }
+      if (all)
+        break;
+    }
+    PRTCL_HIP_CHECK(hipGetLastError());
+    r = pf.begin(st, kStageIntegrateX);
+    FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
+    pf.end(st, r);
+    if (pf.on) {
+      PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+      int it = 0;
+      for (const Progress &p : prog)
+        it = std::max(it, p.iters);
+      pf.resolve(it);
+    }
+    int status = PRTCL_HIP_OK;
+    for (unsigned k = 0; k < ns; ++k) {
+      SceneImpl &s = *v[k];
+      const Progress &p = prog[k];
+      s.predicted_iters_ = std::max(1, p.iters);
+      s.last_iisph_iters_ = p.iters;
+      s.last_rel_ = p.rel;
+      const bool converged = p.iters >= s.desc.iisph_min_iterations &&
+                             p.rel < s.desc.iisph_tolerance;
+      if (!converged && status == PRTCL_HIP_OK)
+        status = PRTCL_HIP_NOT_CONVERGED;
+      s.clock_ += static_cast<double>(static_cast<T>(s.desc.time_step));
+    }
+    PRTCL_HIP_CHECK(hipGetLastError());
+    return status;
+  }
+
+  // hardware block b of n -> logical block (the XCD remap of the fused kernels)
+  static unsigned xcd_of(unsigned b, unsigned n) {
+    constexpr unsigned kXcd = 8;
+    const unsigned per = n / kXcd, rem = n % kXcd;
+    const unsigned x = b % kXcd, k = b / kXcd;
+    return x * per + (x < rem ? x : rem) + k;
+  }
+
   void step_info(int *it, double *rel, int *wk) override {
     if (it)
       *it = last_iisph_iters_;
@@ -1034,6 +1289,53 @@ SceneBase *make_scene(const prtcl_hip_scene_desc &desc, CtxResources *res) {
   if (desc.real_type == 1)
     return new SceneImpl<double>(desc, res);
   return new SceneImpl<float>(desc, res);
+}
+
+int step_scenes(const std::vector<SceneBase *> &scenes, int k, CtxResources *res) {
+  constexpr size_t kMaxBatch = size_t(1) << (32 - kBatchLocalBits);
+  std::vector<SceneImpl<float> *> bf;
+  std::vector<SceneImpl<double> *> bd;
+  std::vector<SceneBase *> rest;
+  for (size_t i = 0; i < scenes.size(); ++i) {
+    for (size_t j = 0; j < i; ++j)
+      if (scenes[j] == scenes[i])
+        throw std::invalid_argument("a scene is listed twice");
+    if (auto *f = dynamic_cast<SceneImpl<float> *>(scenes[i]); f && f->batchable())
+      bf.push_back(f);
+    else if (auto *d = dynamic_cast<SceneImpl<double> *>(scenes[i]); d && d->batchable())
+      bd.push_back(d);
+    else
+      rest.push_back(scenes[i]);
+  }
+  // a batch of one is the single-scene step
+  if (bf.size() == 1) {
+    rest.push_back(bf[0]);
+    bf.clear();
+  }
+  if (bd.size() == 1) {
+    rest.push_back(bd[0]);
+    bd.clear();
+  }
+  int status = PRTCL_HIP_OK;
+  auto keep = [&](int rc) {
+    if (rc != PRTCL_HIP_OK && status == PRTCL_HIP_OK)
+      status = rc;
+  };
+  for (int s = 0; s < k; ++s) {
+    for (size_t b = 0; b < bf.size(); b += kMaxBatch) {
+      std::vector<SceneImpl<float> *> part(bf.begin() + b,
+                                           bf.begin() + std::min(bf.size(), b + kMaxBatch));
+      keep(SceneImpl<float>::step_batch(part, res));
+    }
+    for (size_t b = 0; b < bd.size(); b += kMaxBatch) {
+      std::vector<SceneImpl<double> *> part(bd.begin() + b,
+                                            bd.begin() + std::min(bd.size(), b + kMaxBatch));
+      keep(SceneImpl<double>::step_batch(part, res));
+    }
+    for (SceneBase *sc : rest)
+      keep(sc->step(1));
+  }
+  return status;
 }
 
 uint64_t scene_particle_bytes(const prtcl_hip_scene_desc &desc) {

@@ -56,6 +56,14 @@ struct CtxResources {
   int32_t *pinned = nullptr;  // 64 words
   hipEvent_t ev[2] = {nullptr, nullptr};
   void *ensure_staging(size_t bytes);
+  // tables of batched steps: a pinned host copy written per step, its device
+  // copy, and an event marking the end of the last host -> device copy
+  void *batch_host = nullptr;
+  void *batch_dev = nullptr;
+  size_t batch_bytes = 0;
+  hipEvent_t batch_ev = nullptr;
+  void ensure_batch(size_t bytes);
+  void release();
 };
 
 class SceneBase {
@@ -83,6 +91,11 @@ class SceneBase {
 };
 
 SceneBase *make_scene(const prtcl_hip_scene_desc &desc, CtxResources *res);
+// k whole steps of every listed scene; scenes of the fused path without
+// explicit or implicit viscosity are advanced together (one launch per stage
+// for all of them), the others one after the other. Returns the first
+// non-OK status of a scene.
+int step_scenes(const std::vector<SceneBase *> &scenes, int k, CtxResources *res);
 uint64_t scene_particle_bytes(const prtcl_hip_scene_desc &desc);
 
 }  // namespace prtcl_amd
