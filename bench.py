@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the batched workload (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -61,6 +64,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     from prtcl_amd import hip, scenes
 
     sc = scenes.config1() if args.workload == "dambreak_1m" else scenes.config0()
@@ -100,6 +106,24 @@ def main():
     ctx.synchronize()
     prof = ctx.profile_read()
     ctx.profile(False)
+    tuned = g.schedule()
+
+    # per-launch stage times of both schedules of the fused passes (global
+    # gathers vs LDS-staged slabs), each forced for a few further steps
+    schedules = {}
+    for name, mask in (("global", 0), ("lds", 7)):
+        g.set_schedule(mask)
+        g.step(1)
+        ctx.profile(True)
+        for _ in range(min(prof_steps, 10)):
+            g.step(1)
+        ctx.synchronize()
+        pr = ctx.profile_read()
+        ctx.profile(False)
+        schedules[name] = {k: round(v[0] / v[1], 5) for k, v in pr.items()
+                           if v[1] > 0 and k in ("iisph_setup", "iisph_pressure_acceleration",
+                                                 "iisph_pressure")}
+    g.set_schedule(tuned)
 
     nf = sc.n_fluid
     value = nf * args.steps / elapsed
@@ -150,6 +174,7 @@ def main():
         "step_roofline_frac": round(step_frac, 5),
         "roofline": roofline,
         "stages_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in stages.items()},
+        "schedule": {"tuned_mask": tuned, "ms_per_launch": schedules},
     }
 
     if not args.no_cpu_baseline:
@@ -160,6 +185,24 @@ def main():
         out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
 
     print(json.dumps(out), flush=True)
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch give the counters' scale (KiB per
+    byte actually moved) for the traffic figure of the roofline."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps,
+                      "read_plus_write_GBs": round(2 * nbytes * reps / el / 1e9, 1)}),
+          flush=True)
 
 
 def bench_batch(ctx, count, steps, warmup):

@@ -216,6 +216,17 @@ int prtcl_hip_copy_neighbors(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
 int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
                                uint32_t *perm);
 
+/* ---- schedule of the fused passes ---------------------------------------
+ * No reference counterpart (tuning of this implementation; results do not
+ * depend on it). The setup, pressure-acceleration and pressure passes of
+ * prtcl_hip_step either gather the neighbours' fields from global memory or
+ * stage them slab by slab in LDS; an autotuner times both on the first fused
+ * steps of each scene (and of batched steps) and keeps the faster per pass.
+ * mask: bit 0 setup, bit 1 pressure acceleration, bit 2 pressure in LDS;
+ * -1 restarts the tuner. scene < 0 addresses the schedule of batched steps. */
+int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask);
+int prtcl_hip_get_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t *mask);
+
 /* ---- stage timing -------------------------------------------------------
  * No reference counterpart (the reference logs host-side timestamps around
  * each RunProcedure, module_schemes.cpp:30-37). When enabled, hipEvents are

@@ -363,6 +363,29 @@ int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
   });
 }
 
+int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask) {
+  return guarded(ctx, [&]() -> int {
+    if (mask < -1 || mask > 7)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "schedule mask must be -1 or 0..7");
+    if (scene < 0) {
+      for (auto &t : ctx->res.batch_tuner)
+        t.force(mask);
+      return PRTCL_HIP_OK;
+    }
+    find(ctx, scene)->set_schedule(mask);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_get_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t *mask) {
+  return guarded(ctx, [&]() -> int {
+    if (!mask)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "mask is NULL");
+    *mask = scene < 0 ? ctx->res.batch_tuner[0].current().mask() : find(ctx, scene)->schedule();
+    return PRTCL_HIP_OK;
+  });
+}
+
 int prtcl_hip_profile_enable(prtcl_hip_ctx *ctx, int on) {
   return guarded(ctx, [&]() -> int {
     PRTCL_HIP_CHECK(hipStreamSynchronize(ctx->res.stream));

@@ -13,19 +13,18 @@
 // so boundary particles are visited once per step (in the setup pass) and
 // each IISPH iteration is two passes over fluid neighbours only. The setup
 // pass tests the 27-cell candidates with the exact predicate and records the
-// accepted fluid neighbours (self excluded) in a per-step list
-// (slot-interleaved: entry k of particle i at k * n + i, so a wavefront reads
-// 64 consecutive words); later passes walk the list. A particle with more
-// neighbours than the list holds is walked over the grid instead.
+// accepted fluid neighbours (self excluded) in a per-step list of 16-bit
+// slab indices (nbrlist.hpp); the IISPH passes stage each slab of their
+// block's neighbourhood in LDS and gather the neighbours' fields from there.
+// A particle with more neighbours than the list holds is walked over the
+// grid instead.
+#include <type_traits>
+
 #include "fused.hpp"
 
 namespace prtcl_amd {
 
 namespace {
-
-// neighbours staged per thread in LDS by the setup pass (40 KB per block)
-constexpr uint32_t kSetupLdsCap = 40;
-static_assert(kSetupLdsCap <= kNbrCap, "the LDS table must fit the list");
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // works on one contiguous range of cell-ordered particles and its L2 holds
@@ -127,96 +126,160 @@ __device__ __forceinline__ T fade_step(const SceneView<T> &S, T t_birth) {
   return (T(0) < (S.t - t_birth) - S.fade) ? T(1) : T(0);
 }
 
-// Calls fn(j) for every fluid neighbour j != i of particle i: from the list
-// (four indices loaded ahead so the dependent gathers of four neighbours are
-// in flight together) or, for a particle whose list overflowed, over the grid
-// with the exact predicate.
+// Exact-predicate walk of particle i's 27-cell candidates in global memory:
+// fn(j) for every fluid j != i with |x_i - x_j|^2 < r^2 (particles whose list
+// overflowed).
+template <class T, class Fn>
+__device__ __forceinline__ void grid_walk(const SceneView<T> &S, int i, const vec4_t<T> &xi,
+                                          Fn &&fn) {
+  runs(S.grid, S.fcell[i], S.fstart, [&](uint32_t b, uint32_t e) {
+    for (uint32_t j = b; j < e; ++j) {
+      const vec4_t<T> pj = S.xm[j];
+      if (j != static_cast<uint32_t>(i) &&
+          sqdist_exact(xi.x - pj.x, xi.y - pj.y, xi.z - pj.z) < S.thr2)
+        fn(j);
+    }
+  });
+}
+
+// Calls fn(j) for every fluid neighbour j != i of particle i, from the list
+// (global slots) or over the grid for a particle whose list overflowed.
 template <class T, class Fn>
 __device__ __forceinline__ void walk(const SceneView<T> &S, const NbrList &L, int i,
                                      const vec4_t<T> &xi, Fn &&fn) {
-  const uint32_t cnt = L.count[i];
-  const size_t n = static_cast<size_t>(S.nf);
-  const uint32_t *__restrict__ idx = L.idx + i;
-  if (cnt <= L.cap) {
-    uint32_t k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-      const uint32_t j0 = idx[(k + 0) * n], j1 = idx[(k + 1) * n];
-      const uint32_t j2 = idx[(k + 2) * n], j3 = idx[(k + 3) * n];
-      fn(j0);
-      fn(j1);
-      fn(j2);
-      fn(j3);
-    }
-    for (; k < cnt; ++k)
-      fn(idx[k * n]);
-  } else {
-    runs(S.grid, S.fcell[i], S.fstart, [&](uint32_t b, uint32_t e) {
-      for (uint32_t j = b; j < e; ++j) {
-        const vec4_t<T> pj = S.xm[j];
-        if (j != static_cast<uint32_t>(i) &&
-            sqdist_exact(xi.x - pj.x, xi.y - pj.y, xi.z - pj.z) < S.thr2)
-          fn(j);
-      }
-    });
+  const uint32_t info = L.info[i];
+  if (!(info & kListOverflow))
+    list_visit(L, Spans::of(L.blk[i / kBlock]), static_cast<size_t>(S.nf), i, info, fn);
+  else
+    grid_walk(S, i, xi, fn);
+}
+
+// the three x-runs of slab s (rows dz = s - 1, dy = -1..1) of a cell, in
+// stencil order: fn(b, e); rows outside the box are skipped
+template <class Fn>
+__device__ __forceinline__ void slab_runs(const GridDims &g, uint32_t cellc,
+                                          const uint32_t *__restrict__ start, int s, Fn &&fn) {
+  const int cx = static_cast<int>(cellc & 0x7FFu);
+  const int cy = static_cast<int>((cellc >> 11) & 0x7FFu);
+  const int z = static_cast<int>(cellc >> 22) + s - 1;
+  if (z < 0 || z >= g.nz)
+    return;
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx + 1 < g.nx ? cx + 1 : g.nx - 1;
+#pragma unroll
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int y = cy + dy;
+    if (y < 0 || y >= g.ny)
+      continue;
+    const uint32_t row = static_cast<uint32_t>(g.nx) *
+                         (static_cast<uint32_t>(y) +
+                          static_cast<uint32_t>(g.ny) * static_cast<uint32_t>(z));
+    fn(start[row + static_cast<uint32_t>(x0)], start[row + static_cast<uint32_t>(x1) + 1u]);
   }
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const uint32_t u = __shfl_xor(v, o, kWave);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const uint32_t u = __shfl_xor(v, o, kWave);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+// Spans of a block (all threads call it): span s runs from the first start
+// to the last end of the slab-s runs of the block's particles. Written to R
+// (LDS).
+__device__ __forceinline__ void block_spans(const GridDims &g, const uint32_t *fstart,
+                                            bool valid, uint32_t cell, BlockRanges &R) {
+  __shared__ uint32_t lo[3], hi[3];
+  if (threadIdx.x < 3) {
+    lo[threadIdx.x] = 0xFFFFFFFFu;
+    hi[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    uint32_t bm = 0xFFFFFFFFu, em = 0u;
+    if (valid)
+      slab_runs(g, cell, fstart, s, [&](uint32_t b, uint32_t e) {
+        if (e > b) {
+          bm = b < bm ? b : bm;
+          em = e > em ? e : em;
+        }
+      });
+    bm = wave_min_u32(bm);
+    em = wave_max_u32(em);
+    if (__lane_id() == 0) {
+      atomicMin(&lo[s], bm);
+      atomicMax(&hi[s], em);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const uint32_t b = lo[threadIdx.x], e = hi[threadIdx.x];
+    R.base[threadIdx.x] = b < e ? b : 0u;
+    R.len[threadIdx.x] = b < e ? e - b : 0u;
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // setup pass: density + neighbour list + IISPH c, AA (+ s when the velocity
 // of every neighbour is known without another pass, i.e. no explicit
 // viscosity: v*_j = v_j + dt g H_j)
+//
+// Phase 1 tests the 27-cell candidates with the exact predicate and appends
+// the accepted neighbours (entries relative to the slab's span) to this
+// thread's column of an LDS table; phase 2 runs the pair sums over the
+// compacted entries, so the lanes that reject a candidate do not wait for the
+// kernel arithmetic of the ones that accept it. The table is written to the
+// list row by row (coalesced). With kStage the block stages each slab's span
+// in LDS and runs both phases per slab against the staged copy; otherwise
+// the candidates are read from global memory (through L2) and both phases
+// run once over all 27 cells.
 // ---------------------------------------------------------------------------
-template <class T, bool kWithSource>
-__device__ __forceinline__ void k_setup_body(SceneView<T> S, NbrList L,
-                                                  vec4_t<T> *vout, unsigned lb) {
-  
-  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
-  if (i >= S.nf)
-    return;
+template <class T, bool kWithSource, bool kStage>
+__device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrList &L,
+                                             vec4_t<T> *vout, unsigned lb) {
+  constexpr uint32_t kCap = slab_cap<T>();
+  __shared__ vec4_t<T> sx[kStage ? kCap : 1];
+  __shared__ uint16_t lst[kNbrCap][kBlock];
+  __shared__ BlockRanges sR;
+  const unsigned t = threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + t);
+  const bool valid = i < S.nf;
+  const uint32_t cell = valid ? S.fcell[i] : 0u;
+  block_spans(S.grid, S.fstart, valid, cell, sR);
+  if (t < 6)
+    reinterpret_cast<uint32_t *>(L.blk + lb)[t] = t < 3 ? sR.base[t] : sR.len[t - 3];
+  const Spans R = Spans::of(sR);
+  const uint32_t smax = R.max_len();
+  const bool staged = kStage && smax <= kCap;
+  const bool encodable = smax <= 0x10000u;
+
   const FastSpline<T> K = fast_spline(S);
-  const vec4_t<T> xi = S.xm[i];
-  const vec4_t<T> vi = S.v[i];
+  const vec4_t<T> zero4 = make4<vec4_t<T>>(T(0), T(0), T(0), T(0));
+  const vec4_t<T> xi = valid ? S.xm[i] : zero4;
+  const vec4_t<T> vi = valid ? S.v[i] : zero4;
   const T Hi = fade_step(S, vi.w);
   const T dtg0 = S.dt * S.g0, dtg1 = S.dt * S.g1, dtg2 = S.dt * S.g2;
   const T vs0 = vi.x + dtg0 * Hi, vs1 = vi.y + dtg1 * Hi, vs2 = vi.z + dtg2 * Hi;
-  const uint32_t cell = S.fcell[i];
-  const int n = S.nf;
-  // phase 1: exact predicate over the 27-cell candidates, accepted fluid
-  // neighbours appended to the list (the kernel arithmetic of the accepted
-  // pairs runs in phase 2 over the compacted list, so the SIMD lanes that
-  // reject a candidate do not wait for the ones that accept it)
-  // The accepted indices go to this thread's column of an LDS table first
-  // (conflict-free: entry k of thread t at lst[k][t]); phase 2 copies them
-  // to the list row by row, so every global store is a coalesced row write.
-  __shared__ uint32_t lst[kSetupLdsCap][kBlock];
-  const unsigned t = threadIdx.x;
-  uint32_t cnt = 0;
-  runs(S.grid, cell, S.fstart, [&](uint32_t b, uint32_t e) {
-    // four candidates per trip, loads issued together (indices clamped to the
-    // run; the clamped duplicates are rejected by the bound check)
-    for (uint32_t j = b; j < e; j += 4) {
-      vec4_t<T> pk[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        pk[u] = S.xm[min(j + u, e - 1)];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t ju = j + u;
-        if (ju < e && ju != static_cast<uint32_t>(i) &&
-            sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) < S.thr2) {
-          if (cnt < kSetupLdsCap)
-            lst[cnt][t] = ju;
-          ++cnt;
-        }
-      }
-    }
-  });
-  L.count[i] = cnt;
-  // phase 2: density (self term W(0) = 4 alpha / h^3), G_f, sum |gradW|^2,
-  // and the velocity-divergence sum of the source term
+  // density (self term W(0) = 4 alpha / h^3), G_f, sum |gradW|^2 and the
+  // velocity-divergence sum of the source term
   T rho = xi.w * K.aw * T(4), gf0 = 0, gf1 = 0, gf2 = 0, s2 = 0, dsum = 0;
-  auto pair = [&](uint32_t j) {
-    const vec4_t<T> pj = S.xm[j];
+  // pair sums of neighbour (x_j, m_j) = pj with velocity vj (read only
+  // with kWithSource)
+  auto pair2 = [&](const vec4_t<T> &pj, const vec4_t<T> &vj) {
     const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
     T w, f;
     K.eval(d0 * d0 + d1 * d1 + d2 * d2, w, f);
@@ -227,7 +290,6 @@ __device__ __forceinline__ void k_setup_body(SceneView<T> S, NbrList L,
     gf2 += pj.w * g2;
     s2 += pj.w * (g0 * g0 + g1 * g1 + g2 * g2);
     if (kWithSource) {
-      const vec4_t<T> vj = S.v[j];
       const T Hj = fade_step(S, vj.w);
       const T u0 = vs0 - (vj.x + dtg0 * Hj);
       const T u1 = vs1 - (vj.y + dtg1 * Hj);
@@ -235,31 +297,114 @@ __device__ __forceinline__ void k_setup_body(SceneView<T> S, NbrList L,
       dsum += pj.w * (u0 * g0 + u1 * g1 + u2 * g2);
     }
   };
-  const size_t nn = static_cast<size_t>(n);
-  if (cnt <= kSetupLdsCap) {
-#pragma unroll 4
-    for (uint32_t k = 0; k < cnt; ++k) {
-      const uint32_t j = lst[k][t];
-      L.idx[k * nn + i] = j;  // row k of the list, written by all lanes together
-      pair(j);
-    }
-  } else {
-    // more neighbours than the LDS table holds: walk the grid again in the
-    // same candidate order, writing the first L.cap entries
-    uint32_t k = 0;
-    runs(S.grid, cell, S.fstart, [&](uint32_t b, uint32_t e) {
-      for (uint32_t j = b; j < e; ++j) {
-        const vec4_t<T> pj = S.xm[j];
-        if (j != static_cast<uint32_t>(i) &&
-            sqdist_exact(xi.x - pj.x, xi.y - pj.y, xi.z - pj.z) < S.thr2) {
-          if (k < L.cap)
-            L.idx[k * nn + i] = j;
-          ++k;
-          pair(j);
+  auto load_v = [&](uint32_t j) { return kWithSource ? S.v[j] : zero4; };
+  auto pair = [&](const vec4_t<T> &pj, uint32_t j) { pair2(pj, load_v(j)); };
+  uint32_t cnt = 0;
+  uint32_t cs[3] = {0u, 0u, 0u};  // entries per slab
+  // candidate test of the slab-s runs, positions from global memory
+  // (kLds = false) or from the staged span; accepted entries are appended
+  auto scan = [&](int s, auto lds_tag) {
+    constexpr bool kLds = decltype(lds_tag)::value;
+    const uint32_t base = R.base(s);
+    slab_runs(S.grid, cell, S.fstart, s, [&](uint32_t b, uint32_t e) {
+      // four candidates per trip, loads issued together (indices clamped to
+      // the run; the clamped duplicates are rejected by the bound check)
+      for (uint32_t j = b; j < e; j += 4) {
+        vec4_t<T> pk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t jj = min(j + u, e - 1);
+          pk[u] = kLds ? sx[jj - base] : S.xm[jj];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t ju = j + u;
+          if (ju < e && ju != static_cast<uint32_t>(i) &&
+              sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) < S.thr2) {
+            if (cnt < kNbrCap)
+              lst[cnt][t] = static_cast<uint16_t>(ju - base);
+            ++cnt;
+          }
         }
       }
     });
+  };
+  bool over = !encodable;
+  if (staged) {
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      stage_span(R, s, S.xm, sx);
+      __syncthreads();
+      if (valid && !over) {
+        const uint32_t c0 = cnt;
+        scan(s, std::true_type{});
+        if (cnt > kNbrCap) {
+          over = true;
+        } else {
+          cs[s] = cnt - c0;
+          const uint32_t base = R.base(s);
+#pragma unroll 4
+          for (uint32_t e = c0; e < cnt; ++e) {
+            const uint32_t u = lst[e][t];
+            pair(sx[u], base + u);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  } else if (valid && !over) {
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const uint32_t c0 = cnt;
+      scan(s, std::false_type{});
+      cs[s] = cnt - c0;
+    }
+    if (cnt > kNbrCap) {
+      over = true;
+    } else {
+      const uint32_t c0 = cs[0], c01 = cs[0] + cs[1];
+      const uint32_t d1 = R.b1 - R.b0, d2 = R.b2 - R.b1;
+      auto slot = [&](uint32_t e) {
+        return R.b0 + lst[e][t] + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u);
+      };
+      // groups of four: every gather of a group ahead of its arithmetic
+      uint32_t e = 0;
+      for (; e + 4 <= cnt; e += 4) {
+        vec4_t<T> px[4], pv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t j = slot(e + k);
+          px[k] = S.xm[j];
+          pv[k] = load_v(j);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          pair2(px[k], pv[k]);
+      }
+      for (; e < cnt; ++e) {
+        const uint32_t j = slot(e);
+        pair2(S.xm[j], load_v(j));
+      }
+    }
   }
+  if (!valid)
+    return;
+  const size_t nn = static_cast<size_t>(S.nf);
+  uint32_t info;
+  if (over) {
+    // more neighbours than the list holds: all pair sums again over the grid
+    rho = xi.w * K.aw * T(4);
+    gf0 = gf1 = gf2 = s2 = dsum = T(0);
+    grid_walk(S, i, xi, [&](uint32_t j) { pair(S.xm[j], j); });
+    info = kListOverflow;
+  } else {
+    for (uint32_t e = 0; e < cnt; ++e)
+      L.idx[e * nn + i] = lst[e][t];  // row e, written by all lanes together
+    info = cs[0] | (cs[1] << 8) | (cs[2] << 16);
+  }
+  L.info[i] = info;
   T rhob = 0, gb0 = 0, gb1 = 0, gb2 = 0;
   runs(S.grid, cell, S.bstart, [&](uint32_t b, uint32_t e) {
     for (uint32_t j = b; j < e; j += 4) {
@@ -311,10 +456,10 @@ __device__ __forceinline__ void k_setup_body(SceneView<T> S, NbrList L,
   }
 }
 
-template <class T, bool kWithSource>
+template <class T, bool kWithSource, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
                                                   vec4_t<T> *vout) {
-  k_setup_body<T, kWithSource>(S, L, vout, xcd_block(blockIdx.x, gridDim.x));
+  k_setup_body<T, kWithSource, kStage>(S, L, vout, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // explicit viscosity (viscosity.hpp:171-240) over the list, then
@@ -389,65 +534,132 @@ __global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList 
            S.dt * rho0 / rhoi * (dsum + rho0 * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z));
 }
 
+// Walk of the list entries of particle i (all threads of the block call it).
+// kStage and spans that fit LDS: per slab, stage(s) fills LDS (all threads),
+// then use(lds(u)) for the neighbour at LDS index u. Otherwise
+// use(load(j)) over the whole list with global slots, kUnroll gathers in
+// flight.
+template <bool kStage, uint32_t kCap, int kUnroll, class Stage, class Lds, class Load,
+          class Use>
+__device__ __forceinline__ void list_walk(const NbrList &L, const Spans &R, size_t n, int i,
+                                          uint32_t info, Stage &&stage, Lds &&lds, Load &&load,
+                                          Use &&use) {
+  const uint32_t live = (info & kListOverflow) ? 0u : info;
+  if (kStage && R.max_len() <= kCap) {
+    const uint16_t *__restrict__ idx = L.idx + i;
+    uint32_t e = 0;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const uint32_t c = (live >> (8 * s)) & 0xFFu;
+      stage(s);
+      __syncthreads();
+      uint32_t q = 0;
+      for (; q + 4 <= c; q += 4) {
+        const uint32_t u0 = idx[(e + q) * n], u1 = idx[(e + q + 1) * n];
+        const uint32_t u2 = idx[(e + q + 2) * n], u3 = idx[(e + q + 3) * n];
+        use(lds(u0));
+        use(lds(u1));
+        use(lds(u2));
+        use(lds(u3));
+      }
+      for (; q < c; ++q)
+        use(lds(idx[(e + q) * n]));
+      __syncthreads();
+      e += c;
+    }
+  } else {
+    list_gather<kUnroll>(L, R, n, i, live, load, use);
+  }
+}
+
 // IISPH pressure acceleration over the list: a_i = p_i c_i - sum m_j pr_j gradW
-template <class T>
-__device__ __forceinline__ void k_accel_list_body(SceneView<T> S, NbrList L, unsigned lb) {
+template <class T, bool kStage>
+__device__ __forceinline__ void k_accel_list_body(const SceneView<T> &S, const NbrList &L,
+                                                  unsigned lb) {
   if (S.ctl->done)
     return;
-  
+  constexpr uint32_t kCap = slab_cap<T>();
+  __shared__ vec4_t<T> sq[kStage ? kCap : 1];
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
-  if (i >= S.nf)
-    return;
+  const bool valid = i < S.nf;
+  const Spans R = Spans::of(L.blk[lb]);
   const FastSpline<T> K = fast_spline(S);
-  const vec4_t<T> xi = S.xm[i];
-  const vec4_t<T> ci = S.c[i];
-  const T pi = S.p[i];
+  const vec4_t<T> zero4 = make4<vec4_t<T>>(T(0), T(0), T(0), T(0));
+  const vec4_t<T> xi = valid ? S.xm[i] : zero4;
+  const uint32_t info = valid ? L.info[i] : kListOverflow;
+  // own fields loaded ahead of the walk so their latency overlaps it
+  const vec4_t<T> ci = valid ? S.c[i] : zero4;
+  const T pi = valid ? S.p[i] : T(0);
   T a0 = 0, a1 = 0, a2 = 0;
-  auto pair = [&](uint32_t j) {
-    const vec4_t<T> qj = S.xq[j];  // (x_j, m_j p_j / rho_j^2)
+  auto pair = [&](const vec4_t<T> &qj) {  // (x_j, m_j p_j / rho_j^2)
     const T d0 = xi.x - qj.x, d1 = xi.y - qj.y, d2 = xi.z - qj.z;
     const T s = qj.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2);
     a0 += s * d0;
     a1 += s * d1;
     a2 += s * d2;
   };
-  walk(S, L, i, xi, pair);
+  list_walk<kStage, kCap, 8>(
+      L, R, static_cast<size_t>(S.nf), i, info, [&](int s) { stage_span(R, s, S.xq, sq); },
+      [&](uint32_t u) { return sq[u]; }, [&](uint32_t j) { return S.xq[j]; }, pair);
+  if (!valid)
+    return;
+  if (info & kListOverflow)
+    grid_walk(S, i, xi, [&](uint32_t j) { pair(S.xq[j]); });
   S.a[i] = make4<vec4_t<T>>(pi * ci.x - K.ag3 * a0, pi * ci.y - K.ag3 * a1,
                             pi * ci.z - K.ag3 * a2, T(0));
 }
 
-template <class T>
+template <class T, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_accel_list(SceneView<T> S, NbrList L) {
-  k_accel_list_body<T>(S, L, xcd_block(blockIdx.x, gridDim.x));
+  k_accel_list_body<T, kStage>(S, L, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // IISPH relaxed Jacobi update over the list, FP64 aprde reduction and the
 // device-side convergence decision (example.lua:409-421)
-template <class T>
-__device__ __forceinline__ void k_pressure_list_body(SceneView<T> S, NbrList L, unsigned lb) {
+template <class T, bool kStage>
+__device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, const NbrList &L,
+                                                     unsigned lb) {
   if (S.ctl->done)
     return;
-  
+  constexpr uint32_t kCap = slab_cap<T>();
+  __shared__ vec4_t<T> sx[kStage ? kCap : 1];
+  __shared__ vec4_t<T> sa[kStage ? kCap : 1];
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
+  const bool valid = i < S.nf;
+  const Spans R = Spans::of(L.blk[lb]);
+  const FastSpline<T> K = fast_spline(S);
+  const vec4_t<T> zero4 = make4<vec4_t<T>>(T(0), T(0), T(0), T(0));
+  const vec4_t<T> xi = valid ? S.xm[i] : zero4;
+  const uint32_t info = valid ? L.info[i] : kListOverflow;
+  // own fields loaded ahead of the walk so their latency overlaps it
+  const vec4_t<T> ai = valid ? S.a[i] : zero4, gs = valid ? S.G[i] : zero4;
+  const T rhoi = valid ? S.rho[i] : T(1), AA = valid ? S.AA[i] : T(0);
+  const T si = valid ? S.s[i] : T(0), p_old = valid ? S.p[i] : T(0);
+  T sum = 0;
+  struct XA {
+    vec4_t<T> x, a;
+  };
+  auto pair = [&](const XA &q) {
+    const T d0 = xi.x - q.x.x, d1 = xi.y - q.x.y, d2 = xi.z - q.x.z;
+    sum += q.x.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2) *
+           (q.a.x * d0 + q.a.y * d1 + q.a.z * d2);
+  };
+  list_walk<kStage, kCap, 4>(
+      L, R, static_cast<size_t>(S.nf), i, info,
+      [&](int s) {
+        stage_span(R, s, S.xm, sx);
+        stage_span(R, s, S.a, sa);
+      },
+      [&](uint32_t u) { return XA{sx[u], sa[u]}; }, [&](uint32_t j) { return XA{S.xm[j], S.a[j]}; },
+      pair);
   double term = 0.0;
-  if (i < S.nf) {
-    const FastSpline<T> K = fast_spline(S);
-    const vec4_t<T> xi = S.xm[i], ai = S.a[i], gs = S.G[i];
-    const T rhoi = S.rho[i];
-    T sum = 0;
-    auto pair = [&](uint32_t j) {
-      const vec4_t<T> pj = S.xm[j];
-      const vec4_t<T> aj = S.a[j];
-      const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-      sum += pj.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2) *
-             (aj.x * d0 + aj.y * d1 + aj.z * d2);
-    };
-    walk(S, L, i, xi, pair);
+  if (valid) {
+    if (info & kListOverflow)
+      grid_walk(S, i, xi, [&](uint32_t j) { pair(XA{S.xm[j], S.a[j]}); });
     const T Ap = S.dt * S.dt * S.rho0 / rhoi *
                  ((ai.x * gs.x + ai.y * gs.y + ai.z * gs.z) - K.ag3 * sum);
-    const T AA = S.AA[i], si = S.s[i];
     const T rcp = (fabs(AA) > T(1e-9)) ? T(1) / AA : T(0);
-    const T upd = S.p[i] + S.omega * (si - Ap) * rcp;
+    const T upd = p_old + S.omega * (si - Ap) * rcp;
     const T pn = ((T(0) < upd) ? upd : T(0)) * ((T(1e-9) < fabs(AA)) ? T(1) : T(0));
     S.Ap[i] = Ap;
     S.p[i] = pn;
@@ -460,9 +672,9 @@ __device__ __forceinline__ void k_pressure_list_body(SceneView<T> S, NbrList L, 
     S.partials[lb] = bs;
 }
 
-template <class T>
+template <class T, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_pressure_list(SceneView<T> S, NbrList L) {
-  k_pressure_list_body<T>(S, L, xcd_block(blockIdx.x, gridDim.x));
+  k_pressure_list_body<T, kStage>(S, L, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Sum of the block partials in block order (FP64, fixed tree) and the
@@ -561,28 +773,29 @@ __global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double 
 }
 
 // ---- batched wrappers (prtcl_hip_step over several scenes)
-template <class T>
+template <class T, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_setup_b(const BatchEntry<T> *tab,
                                                     const uint32_t *pmap) {
   const uint32_t u = pmap[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  k_setup_body<T, true>(E.S, NbrList{E.nidx, E.ncnt, kNbrCap}, E.vtmp, u & kBatchLocalMask);
+  k_setup_body<T, true, kStage>(E.S, NbrList{E.nidx, E.ninfo, E.nblk}, E.vtmp,
+                                u & kBatchLocalMask);
 }
 
-template <class T>
+template <class T, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_accel_b(const BatchEntry<T> *tab,
                                                     const uint32_t *pmap) {
   const uint32_t u = pmap[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  k_accel_list_body<T>(E.S, NbrList{E.nidx, E.ncnt, kNbrCap}, u & kBatchLocalMask);
+  k_accel_list_body<T, kStage>(E.S, NbrList{E.nidx, E.ninfo, E.nblk}, u & kBatchLocalMask);
 }
 
-template <class T>
+template <class T, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_pressure_b(const BatchEntry<T> *tab,
                                                        const uint32_t *pmap) {
   const uint32_t u = pmap[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  k_pressure_list_body<T>(E.S, NbrList{E.nidx, E.ncnt, kNbrCap}, u & kBatchLocalMask);
+  k_pressure_list_body<T, kStage>(E.S, NbrList{E.nidx, E.ninfo, E.nblk}, u & kBatchLocalMask);
 }
 
 // one block per scene
@@ -610,15 +823,21 @@ __global__ __launch_bounds__(kBlock) void k_speed_finalize_b(const BatchEntry<T>
 
 template <class T>
 void FusedOps<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
-                        bool explicit_viscosity, vec4_t<T> *vtmp) {
+                        bool explicit_viscosity, vec4_t<T> *vtmp, bool lds) {
   if (S.nf <= 0)
     return;
   const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
   if (!explicit_viscosity) {
-    hipLaunchKernelGGL((k_setup<T, true>), gr, bl, 0, st, S, L, vtmp);
+    if (lds)
+      hipLaunchKernelGGL((k_setup<T, true, true>), gr, bl, 0, st, S, L, vtmp);
+    else
+      hipLaunchKernelGGL((k_setup<T, true, false>), gr, bl, 0, st, S, L, vtmp);
     return;
   }
-  hipLaunchKernelGGL((k_setup<T, false>), gr, bl, 0, st, S, L, vtmp);
+  if (lds)
+    hipLaunchKernelGGL((k_setup<T, false, true>), gr, bl, 0, st, S, L, vtmp);
+  else
+    hipLaunchKernelGGL((k_setup<T, false, false>), gr, bl, 0, st, S, L, vtmp);
   hipLaunchKernelGGL(k_viscosity_list<T>, gr, bl, 0, st, S, L, vtmp);
   // the integrated velocities replace v once every thread has read the old ones
   SceneView<T> S2 = S;
@@ -627,17 +846,25 @@ void FusedOps<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
 }
 
 template <class T>
-void FusedOps<T>::accel(hipStream_t st, const SceneView<T> &S, const NbrList &L) {
-  if (S.nf > 0)
-    hipLaunchKernelGGL(k_accel_list<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, L);
+void FusedOps<T>::accel(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds) {
+  if (S.nf <= 0)
+    return;
+  const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
+  if (lds)
+    hipLaunchKernelGGL((k_accel_list<T, true>), gr, bl, 0, st, S, L);
+  else
+    hipLaunchKernelGGL((k_accel_list<T, false>), gr, bl, 0, st, S, L);
 }
 
 template <class T>
-void FusedOps<T>::pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L) {
+void FusedOps<T>::pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds) {
   if (S.nf <= 0)
     return;
   const unsigned nb = grid_blocks(S.nf);
-  hipLaunchKernelGGL(k_pressure_list<T>, dim3(nb), dim3(kBlock), 0, st, S, L);
+  if (lds)
+    hipLaunchKernelGGL((k_pressure_list<T, true>), dim3(nb), dim3(kBlock), 0, st, S, L);
+  else
+    hipLaunchKernelGGL((k_pressure_list<T, false>), dim3(nb), dim3(kBlock), 0, st, S, L);
   hipLaunchKernelGGL(k_pressure_finalize, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials, nb);
 }
 
@@ -652,24 +879,35 @@ void FusedOps<T>::integrate(hipStream_t st, const SceneView<T> &S) {
 
 template <class T>
 void FusedOps<T>::setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                              unsigned npblocks) {
-  if (npblocks > 0)
-    hipLaunchKernelGGL(k_setup_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+                              unsigned npblocks, bool lds) {
+  if (npblocks == 0)
+    return;
+  if (lds)
+    hipLaunchKernelGGL((k_setup_b<T, true>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  else
+    hipLaunchKernelGGL((k_setup_b<T, false>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
 }
 
 template <class T>
 void FusedOps<T>::accel_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                              unsigned npblocks) {
-  if (npblocks > 0)
-    hipLaunchKernelGGL(k_accel_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+                              unsigned npblocks, bool lds) {
+  if (npblocks == 0)
+    return;
+  if (lds)
+    hipLaunchKernelGGL((k_accel_b<T, true>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  else
+    hipLaunchKernelGGL((k_accel_b<T, false>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
 }
 
 template <class T>
 void FusedOps<T>::pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                                 unsigned npblocks, unsigned nscenes) {
+                                 unsigned npblocks, unsigned nscenes, bool lds) {
   if (npblocks == 0)
     return;
-  hipLaunchKernelGGL(k_pressure_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  if (lds)
+    hipLaunchKernelGGL((k_pressure_b<T, true>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+  else
+    hipLaunchKernelGGL((k_pressure_b<T, false>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
   hipLaunchKernelGGL(k_pressure_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
 }
 

@@ -1,34 +1,96 @@
 // Launchers of the fused step kernels (fused.hip).
 #pragma once
 
-#include "scene.hpp"
+#include "nbrlist.hpp"
 
 namespace prtcl_amd {
 
-// per-step fluid neighbour list, slot-interleaved: entry k of particle i at
-// idx[k * n + i]; count[i] may exceed cap (the particle is then walked over
-// the grid)
-struct NbrList {
-  uint32_t *idx;
-  uint32_t *count;
-  uint32_t cap;
+// Schedule of the three neighbour passes of the fused step: the neighbours'
+// fields are gathered from global memory (through L2) or staged slab by slab
+// in LDS first. Both schedules visit the same neighbours in the same order
+// and give bit-identical results; which one is faster depends on the scene.
+struct FusedSchedule {
+  bool setup_lds = false;
+  bool accel_lds = false;
+  bool pressure_lds = false;
+  int mask() const { return (setup_lds ? 1 : 0) | (accel_lds ? 2 : 0) | (pressure_lds ? 4 : 0); }
+  static FusedSchedule from_mask(int m) {
+    FusedSchedule s;
+    s.setup_lds = (m & 1) != 0;
+    s.accel_lds = (m & 2) != 0;
+    s.pressure_lds = (m & 4) != 0;
+    return s;
+  }
+};
+
+// Picks each pass's schedule from timed trial steps: after one warm-up step,
+// kTrials steps alternate the all-global and the all-LDS schedule; per pass
+// the variant with the lower mean launch time wins. A forced mask (>= 0)
+// bypasses the trials.
+struct ScheduleTuner {
+  static constexpr int kTrials = 4;
+  int forced = -1;
+  int steps = 0;
+  double ms[3][2] = {};
+  double launches[3][2] = {};
+  FusedSchedule chosen;
+  bool trial() const { return forced < 0 && steps >= 1 && steps <= kTrials; }
+  int trial_variant() const { return (steps - 1) & 1; }
+  FusedSchedule current() const {
+    if (forced >= 0)
+      return FusedSchedule::from_mask(forced);
+    if (trial())
+      return FusedSchedule::from_mask(trial_variant() ? 7 : 0);
+    return chosen;
+  }
+  // pass: 0 setup, 1 pressure acceleration, 2 pressure
+  void record(int pass, double total_ms, double n) {
+    ms[pass][trial_variant()] += total_ms;
+    launches[pass][trial_variant()] += n;
+  }
+  void finish_step() {
+    if (forced >= 0)
+      return;
+    if (trial() && steps == kTrials) {
+      bool lds[3];
+      for (int p = 0; p < 3; ++p) {
+        const double g = launches[p][0] > 0 ? ms[p][0] / launches[p][0] : 0.0;
+        const double l = launches[p][1] > 0 ? ms[p][1] / launches[p][1] : 1e30;
+        lds[p] = l < g;
+      }
+      chosen.setup_lds = lds[0];
+      chosen.accel_lds = lds[1];
+      chosen.pressure_lds = lds[2];
+    }
+    ++steps;
+  }
+  void force(int mask) {
+    forced = mask;
+    if (mask < 0) {  // restart the trials
+      steps = 0;
+      for (auto &r : ms)
+        r[0] = r[1] = 0;
+      for (auto &r : launches)
+        r[0] = r[1] = 0;
+    }
+  }
 };
 
 template <class T>
 struct FusedOps {
   // density, neighbour list, IISPH c / AA / s, integrated velocity into vtmp
   static void setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
-                    bool explicit_viscosity, vec4_t<T> *vtmp);
-  static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L);
-  static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L);
+                    bool explicit_viscosity, vec4_t<T> *vtmp, bool lds);
+  static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
+  static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
   static void integrate(hipStream_t st, const SceneView<T> &S);
   // the same passes over every scene of a batch (pmap: block map)
   static void setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                          unsigned npblocks);
+                          unsigned npblocks, bool lds);
   static void accel_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                          unsigned npblocks);
+                          unsigned npblocks, bool lds);
   static void pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                             unsigned npblocks, unsigned nscenes);
+                             unsigned npblocks, unsigned nscenes, bool lds);
   static void integrate_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
                               unsigned npblocks, unsigned nscenes);
 };

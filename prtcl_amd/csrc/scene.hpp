@@ -101,13 +101,17 @@ struct SceneView {
   V4 *cx, *cr, *cp, *cq, *cy, *cb;
 };
 
+struct BlockRanges;
+
 // One scene of a batch (prtcl_hip_step over several scenes): its view and
 // the buffers the batched sort and setup use. A batched kernel maps its block
 // to (scene, local block) through a block map of words (scene << 20 | block).
 template <class T>
 struct BatchEntry {
   SceneView<T> S;
-  uint32_t *nidx, *ncnt;      // neighbour list (NbrList of the scene)
+  uint16_t *nidx;             // neighbour list of the scene (NbrList)
+  uint32_t *ninfo;
+  BlockRanges *nblk;
   vec4_t<T> *vtmp;            // integrated velocity written by the setup pass
   uint32_t *count, *start;    // fluid grid counts / starts
   uint32_t *rank, *perm;      // sort buffers (result in rank)

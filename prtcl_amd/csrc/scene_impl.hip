@@ -601,7 +601,7 @@ class SceneImpl final : public SceneBase {
     return S;
   }
 
-  NbrList nbr_list() const { return NbrList{P_.nidx, P_.ncnt, kNbrCap}; }
+  NbrList nbr_list() const { return NbrList{P_.nidx, P_.ninfo, P_.nblk}; }
 
   void require_grid() {
     if (!grid_built_)
@@ -766,7 +766,10 @@ class SceneImpl final : public SceneBase {
     }
   }
 
-  int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused) {
+  // IISPH iterations until the device reports convergence or the cap. With
+  // `tune`, the fused passes are also timed for the schedule tuner.
+  int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused,
+                 const FusedSchedule &sc = FusedSchedule{}, bool tune = false) {
     require_grid();
     hipStream_t st = res_->stream;
     const unsigned seq = ++seq_;
@@ -774,20 +777,25 @@ class SceneImpl final : public SceneBase {
     const SceneView<T> S = view();
     const NbrList L = nbr_list();
     Profiler &pf = res_->prof;
+    tprof_.on = tune;
     int enq = 0;
     auto enqueue = [&](int n) {
       for (int k = 0; k < n; ++k) {
         int r = pf.begin(st, kStagePressureAccel, enq + k);
+        int q = tprof_.begin(st, kStagePressureAccel, enq + k);
         if (fused)
-          FusedOps<T>::accel(st, S, L);
+          FusedOps<T>::accel(st, S, L, sc.accel_lds);
         else
           SphOps<T>::iisph_accel(st, S, true);
+        tprof_.end(st, q);
         pf.end(st, r);
         r = pf.begin(st, kStagePressure, enq + k);
+        q = tprof_.begin(st, kStagePressure, enq + k);
         if (fused)
-          FusedOps<T>::pressure(st, S, L);
+          FusedOps<T>::pressure(st, S, L, sc.pressure_lds);
         else
           SphOps<T>::iisph_pressure(st, S, true);
+        tprof_.end(st, q);
         pf.end(st, r);
       }
       enq += n;
@@ -814,6 +822,16 @@ class SceneImpl final : public SceneBase {
     if (pf.on) {
       PRTCL_HIP_CHECK(hipStreamSynchronize(st));
       pf.resolve(p.iters);
+    }
+    if (tune) {
+      tprof_.resolve(p.iters);
+      tuner_.record(0, tprof_.ms[kStageSetup], static_cast<double>(tprof_.launches[kStageSetup]));
+      tuner_.record(1, tprof_.ms[kStagePressureAccel],
+                    static_cast<double>(tprof_.launches[kStagePressureAccel]));
+      tuner_.record(2, tprof_.ms[kStagePressure],
+                    static_cast<double>(tprof_.launches[kStagePressure]));
+      tprof_.reset();
+      tprof_.on = false;
     }
     predicted_iters_ = std::max(1, p.iters);
     last_iisph_iters_ = p.iters;
@@ -849,11 +867,17 @@ class SceneImpl final : public SceneBase {
         steps_since_permute_ = 0;
       }
       const bool fused = desc.step_mode == 0;
+      const bool tune = fused && tuner_.trial();
+      const FusedSchedule sc = tuner_.current();
       if (fused) {
         // one neighbour pass for density + list + IISPH setup (three with
         // explicit viscosity); the integrated velocity lands in tmpV4
         r = pf.begin(st, kStageSetup);
-        FusedOps<T>::setup(st, view(), nbr_list(), desc.explicit_viscosity != 0, P_.tmpV4);
+        tprof_.on = tune;
+        const int q = tprof_.begin(st, kStageSetup);
+        FusedOps<T>::setup(st, view(), nbr_list(), desc.explicit_viscosity != 0, P_.tmpV4,
+                           sc.setup_lds);
+        tprof_.end(st, q);
         pf.end(st, r);
         std::swap(P_.v, P_.tmpV4);
       } else {
@@ -880,7 +904,9 @@ class SceneImpl final : public SceneBase {
       int it = 0;
       double rel = 0;
       int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
-                          desc.iisph_max_iterations, &it, &rel, fused);
+                          desc.iisph_max_iterations, &it, &rel, fused, sc, tune);
+      if (fused)
+        tuner_.finish_step();
       if (rc != PRTCL_HIP_OK)
         status = rc;
       const SceneView<T> S = view();
@@ -979,7 +1005,8 @@ class SceneImpl final : public SceneBase {
       BatchEntry<T> E{};
       E.S = s.view();
       E.nidx = s.P_.nidx;
-      E.ncnt = s.P_.ncnt;
+      E.ninfo = s.P_.ninfo;
+      E.nblk = s.P_.nblk;
       E.count = s.G_.count;
       E.start = s.G_.fstart;
       E.rank = s.P_.frank;
@@ -1036,8 +1063,15 @@ class SceneImpl final : public SceneBase {
           hipLaunchKernelGGL(k_iota, dim3(grid_blocks(s.nb_)), dim3(kBlock), 0, st, s.P_.bid,
                              s.nb_);
       }
+    ScheduleTuner &tn = res->batch_tuner[sizeof(T) == 8 ? 1 : 0];
+    const bool tune = tn.trial();
+    const FusedSchedule sc = tn.current();
+    Profiler &tp = res->batch_prof;
+    tp.on = tune;
     r = pf.begin(st, kStageSetup);
-    FusedOps<T>::setup_batch(st, d_setup, d_pmap, np);
+    int q = tp.begin(st, kStageSetup);
+    FusedOps<T>::setup_batch(st, d_setup, d_pmap, np, sc.setup_lds);
+    tp.end(st, q);
     pf.end(st, r);
     hipLaunchKernelGGL(k_begin_solve_b<T>, dim3(grid_blocks(ns, 64)), dim3(64), 0, st, d_solve,
                        ns);
@@ -1053,12 +1087,16 @@ class SceneImpl final : public SceneBase {
     int enq = 0;
     auto enqueue = [&](int n) {
       for (int j = 0; j < n; ++j) {
-        int q = pf.begin(st, kStagePressureAccel, enq + j);
-        FusedOps<T>::accel_batch(st, d_solve, d_pmap, np);
-        pf.end(st, q);
-        q = pf.begin(st, kStagePressure, enq + j);
-        FusedOps<T>::pressure_batch(st, d_solve, d_pmap, np, ns);
-        pf.end(st, q);
+        int q1 = pf.begin(st, kStagePressureAccel, enq + j);
+        int q2 = tp.begin(st, kStagePressureAccel, enq + j);
+        FusedOps<T>::accel_batch(st, d_solve, d_pmap, np, sc.accel_lds);
+        tp.end(st, q2);
+        pf.end(st, q1);
+        q1 = pf.begin(st, kStagePressure, enq + j);
+        q2 = tp.begin(st, kStagePressure, enq + j);
+        FusedOps<T>::pressure_batch(st, d_solve, d_pmap, np, ns, sc.pressure_lds);
+        tp.end(st, q2);
+        pf.end(st, q1);
       }
       enq += n;
     };
@@ -1083,6 +1121,19 @@ class SceneImpl final : public SceneBase {
         break;
     }
     PRTCL_HIP_CHECK(hipGetLastError());
+    if (tune) {
+      int it = 0;
+      for (const Progress &p : prog)
+        it = std::max(it, p.iters);
+      tp.resolve(it);
+      tn.record(0, tp.ms[kStageSetup], static_cast<double>(tp.launches[kStageSetup]));
+      tn.record(1, tp.ms[kStagePressureAccel],
+                static_cast<double>(tp.launches[kStagePressureAccel]));
+      tn.record(2, tp.ms[kStagePressure], static_cast<double>(tp.launches[kStagePressure]));
+      tp.reset();
+      tp.on = false;
+    }
+    tn.finish_step();
     r = pf.begin(st, kStageIntegrateX);
     FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
     pf.end(st, r);
@@ -1117,6 +1168,9 @@ class SceneImpl final : public SceneBase {
     const unsigned x = b % kXcd, k = b / kXcd;
     return x * per + (x < rem ? x : rem) + k;
   }
+
+  void set_schedule(int mask) override { tuner_.force(mask); }
+  int schedule() const override { return tuner_.current().mask(); }
 
   void step_info(int *it, double *rel, int *wk) override {
     if (it)
@@ -1174,7 +1228,9 @@ class SceneImpl final : public SceneBase {
     V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq;
     T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp;
     int32_t *id, *tmpI;
-    uint32_t *fcell, *frank, *fperm, *nidx, *ncnt;
+    uint32_t *fcell, *frank, *fperm, *ninfo;
+    uint16_t *nidx;
+    BlockRanges *nblk;
     V4 *bxV, *btmpV4;
     int32_t *bid, *btmpI;
     uint32_t *bcell, *brank, *bperm;
@@ -1205,7 +1261,8 @@ class SceneImpl final : public SceneBase {
     A.take(P.frank, f);
     A.take(P.fperm, f);
     A.take(P.nidx, static_cast<size_t>(kNbrCap) * f);
-    A.take(P.ncnt, f);
+    A.take(P.ninfo, f);
+    A.take(P.nblk, grid_blocks(static_cast<long>(f)));
     A.take(P.bxV, b);
     A.take(P.btmpV4, b);
     A.take(P.bid, b);
@@ -1235,7 +1292,9 @@ class SceneImpl final : public SceneBase {
       Arena::rebase(*q, base);
     for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
       Arena::rebase(*q, base);
-    for (uint32_t **q : {&P_.fcell, &P_.frank, &P_.fperm, &P_.nidx, &P_.ncnt, &P_.bcell,
+    Arena::rebase(P_.nidx, base);
+    Arena::rebase(P_.nblk, base);
+    for (uint32_t **q : {&P_.fcell, &P_.frank, &P_.fperm, &P_.ninfo, &P_.bcell,
                          &P_.brank, &P_.bperm})
       Arena::rebase(*q, base);
     Arena::rebase(P_.partials, base);
@@ -1272,6 +1331,8 @@ class SceneImpl final : public SceneBase {
   bool grid_built_ = false, bgrid_dirty_ = true, fgrid_dirty_ = true;
   double clock_ = 0;
   int steps_since_permute_ = 0;
+  ScheduleTuner tuner_;
+  Profiler tprof_;  // timing of the tuner's trial steps
   int predicted_iters_ = 2;
   int predicted_cg_ = 4;
   unsigned seq_ = 0;

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/prtcl_hip.h"
+#include "fused.hpp"
 #include "scene.hpp"
 
 namespace prtcl_amd {
@@ -62,6 +63,9 @@ struct CtxResources {
   void *batch_dev = nullptr;
   size_t batch_bytes = 0;
   hipEvent_t batch_ev = nullptr;
+  // schedule of the batched fused passes (FP32, FP64) and its trial timing
+  ScheduleTuner batch_tuner[2];
+  Profiler batch_prof;
   void ensure_batch(size_t bytes);
   void release();
 };
@@ -88,6 +92,10 @@ class SceneBase {
   virtual void last_permutation(int group, uint32_t *perm) = 0;
   virtual void step_info(int *iisph_iters, double *rel, int *wkbb18_iters) = 0;
   virtual uint64_t grid_bytes_for_domain() const = 0;
+  // fused-pass schedule: mask >= 0 forces it (bit 0 setup, 1 pressure
+  // acceleration, 2 pressure in LDS), -1 returns to the tuner
+  virtual void set_schedule(int mask) = 0;
+  virtual int schedule() const = 0;
 };
 
 SceneBase *make_scene(const prtcl_hip_scene_desc &desc, CtxResources *res);

@@ -206,10 +206,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L)
     const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
     add(cf * pj.w / S.rho[j] * gradf_fast(S, r2) / (r2 + S.hh001), d0, d1, d2);
   };
-  const uint32_t cnt = L.count[i];
-  if (cnt <= L.cap) {
-    for (uint32_t k = 0; k < cnt; ++k)
-      fluid(L.idx[k * n + i]);
+  const uint32_t info = L.info[i];
+  if (!(info & kListOverflow)) {
+    list_visit(L, Spans::of(L.blk[i / kBlock]), n, i, info, fluid);
   } else {
     nbrs_exact(S, xi, S.fcell[i], S.fstart, S.xm,
                [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
@@ -267,16 +266,9 @@ __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrLis
     l1 += s * d1;
     l2 += s * d2;
   };
-  const uint32_t cnt = L.count[i];
-  if (cnt <= L.cap) {
-    uint32_t k = 0;
-    for (; k + 2 <= cnt; k += 2) {
-      const uint32_t j0 = L.idx[k * n + i], j1 = L.idx[(k + 1) * n + i];
-      fluid(j0);
-      fluid(j1);
-    }
-    if (k < cnt)
-      fluid(L.idx[k * n + i]);
+  const uint32_t info = L.info[i];
+  if (!(info & kListOverflow)) {
+    list_visit(L, Spans::of(L.blk[i / kBlock]), n, i, info, fluid);
   } else {
     nbrs_exact(S, xq, S.fcell[i], S.fstart, S.xq,
                [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {

@@ -48,6 +48,7 @@ EXPORTS = [
     "prtcl_hip_upload_field", "prtcl_hip_download_field", "prtcl_hip_field_id",
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
     "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
+    "prtcl_hip_set_schedule", "prtcl_hip_get_schedule",
     "prtcl_hip_copy_neighbors", "prtcl_hip_last_permutation", "prtcl_hip_profile_enable",
     "prtcl_hip_profile_read",
 ]
@@ -121,6 +122,8 @@ def lib():
         L.prtcl_hip_iisph_solve.argtypes = [vp, i32, C.c_double, i32, i32, C.POINTER(i32),
                                             C.POINTER(C.c_double)]
         L.prtcl_hip_step.argtypes = [vp, C.POINTER(i32), i32, i32]
+        L.prtcl_hip_set_schedule.argtypes = [vp, i32, i32]
+        L.prtcl_hip_get_schedule.argtypes = [vp, i32, C.POINTER(i32)]
         L.prtcl_hip_last_step_info.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(C.c_double),
                                                C.POINTER(i32)]
         L.prtcl_hip_copy_neighbors.argtypes = [vp, i32, i32, i32, u32p, u32p,
@@ -211,6 +214,15 @@ class Context:
             k += 1
         return out
 
+    def set_batch_schedule(self, mask: int):
+        """Force the LDS/global schedule of batched steps (-1: autotune)."""
+        self.check(self.L.prtcl_hip_set_schedule(self.h, -1, mask), "set_schedule")
+
+    def batch_schedule(self) -> int:
+        m = C.c_int32()
+        self.check(self.L.prtcl_hip_get_schedule(self.h, -1, C.byref(m)), "get_schedule")
+        return m.value
+
     def step(self, scenes: Sequence["HipScene"], k: int = 1) -> int:
         ids = (C.c_int32 * len(scenes))(*[s.id for s in scenes])
         rc = self.L.prtcl_hip_step(self.h, ids, len(scenes), k)
@@ -296,6 +308,17 @@ class HipScene:
 
     def step(self, k: int = 1) -> int:
         return self.ctx.step([self], k)
+
+    def set_schedule(self, mask: int):
+        """Force the LDS/global schedule of the fused passes: bit 0 setup,
+        bit 1 pressure acceleration, bit 2 pressure (-1: autotune)."""
+        self.ctx.check(self.L.prtcl_hip_set_schedule(self.ctx.h, self.id, mask), "set_schedule")
+
+    def schedule(self) -> int:
+        m = C.c_int32()
+        self.ctx.check(self.L.prtcl_hip_get_schedule(self.ctx.h, self.id, C.byref(m)),
+                       "get_schedule")
+        return m.value
 
     def step_info(self) -> Dict[str, float]:
         it, rel, wk = C.c_int32(), C.c_double(), C.c_int32()
