@@ -178,6 +178,30 @@ __device__ __forceinline__ void slab_runs(const GridDims &g, uint32_t cellc,
   }
 }
 
+// bounds of the three x-runs of slab s of a cell, loaded together (rows
+// outside the box give empty runs)
+__device__ __forceinline__ void slab_bounds(const GridDims &g, uint32_t cellc,
+                                            const uint32_t *__restrict__ start, int s,
+                                            uint32_t rb[3], uint32_t re[3]) {
+  const int cx = static_cast<int>(cellc & 0x7FFu);
+  const int cy = static_cast<int>((cellc >> 11) & 0x7FFu);
+  const int z = static_cast<int>(cellc >> 22) + s - 1;
+  const bool zok = z >= 0 && z < g.nz;
+  const uint32_t x0 = static_cast<uint32_t>(cx > 0 ? cx - 1 : 0);
+  const uint32_t x1 = static_cast<uint32_t>(cx + 1 < g.nx ? cx + 1 : g.nx - 1);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int y = cy + r - 1;
+    const bool ok = zok && y >= 0 && y < g.ny;
+    const uint32_t row = ok ? static_cast<uint32_t>(g.nx) *
+                                  (static_cast<uint32_t>(y) +
+                                   static_cast<uint32_t>(g.ny) * static_cast<uint32_t>(z))
+                            : 0u;
+    rb[r] = ok ? start[row + x0] : 0u;
+    re[r] = ok ? start[row + x1 + 1u] : 0u;
+  }
+}
+
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -253,7 +277,7 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
                                              vec4_t<T> *vout, unsigned lb) {
   constexpr uint32_t kCap = slab_cap<T>();
   __shared__ vec4_t<T> sx[kStage ? kCap : 1];
-  __shared__ uint16_t lst[kNbrCap][kBlock];
+  __shared__ uint16_t lst[kNbrCap + 1][kBlock];  // row kNbrCap: rejected candidates
   __shared__ BlockRanges sR;
   const unsigned t = threadIdx.x;
   const int i = static_cast<int>(lb * kBlock + t);
@@ -306,29 +330,35 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
   auto scan = [&](int s, auto lds_tag) {
     constexpr bool kLds = decltype(lds_tag)::value;
     const uint32_t base = R.base(s);
-    slab_runs(S.grid, cell, S.fstart, s, [&](uint32_t b, uint32_t e) {
-      // four candidates per trip, loads issued together (indices clamped to
-      // the run; the clamped duplicates are rejected by the bound check)
-      for (uint32_t j = b; j < e; j += 4) {
-        vec4_t<T> pk[4];
+    uint32_t rb[3], re[3];
+    slab_bounds(S.grid, cell, S.fstart, s, rb, re);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+    for (int r = 0; r < 3; ++r) {
+      const uint32_t b = rb[r], e = re[r];
+      // eight candidates per trip, loads issued together (indices clamped to
+      // the run; the clamped duplicates are rejected by the bound check).
+      // Branch-free append: every candidate is written to the next free slot
+      // of the LDS table (row kNbrCap once full) and the count advances only
+      // for an accepted one.
+      for (uint32_t j = b; j < e; j += 8) {
+        vec4_t<T> pk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
           const uint32_t jj = min(j + u, e - 1);
           pk[u] = kLds ? sx[jj - base] : S.xm[jj];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
           const uint32_t ju = j + u;
-          if (ju < e && ju != static_cast<uint32_t>(i) &&
-              sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) < S.thr2) {
-            if (cnt < kNbrCap)
-              lst[cnt][t] = static_cast<uint16_t>(ju - base);
-            ++cnt;
-          }
+          const bool ok = (ju < e) & (ju != static_cast<uint32_t>(i)) &
+                          (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
+                           S.thr2);
+          lst[cnt < kNbrCap ? cnt : kNbrCap][t] = static_cast<uint16_t>(ju - base);
+          cnt += ok ? 1u : 0u;
         }
       }
-    });
+    }
   };
   bool over = !encodable;
   if (staged) {
