@@ -216,6 +216,29 @@ int prtcl_hip_copy_neighbors(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
 int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
                                uint32_t *perm);
 
+/* ---- residency and serving ----------------------------------------------
+ * No reference counterpart (the reference runs one scene per process from
+ * its Lua loop). Device state of a scene can be evicted to host memory and is
+ * brought back bit-identically (same slot order and buffers) by the next
+ * operation on the scene. The request loop advances resident scenes by k
+ * steps per request: prtcl_hip_serve_tick steps every scene with pending
+ * work once, all of them in one batched step (continuous batching at step
+ * granularity); a scene with work that does not fit the HBM budget is brought
+ * back after idle scenes (no pending work, least recently stepped first) are
+ * evicted, or waits. A request completes when its last step has finished on
+ * the device; prtcl_hip_serve_poll returns completed tickets in completion
+ * order. Each scene's results do not depend on the other requests. */
+int prtcl_hip_scene_evict(prtcl_hip_ctx *ctx, int32_t scene);
+int prtcl_hip_scene_resident(prtcl_hip_ctx *ctx, int32_t scene, int32_t *resident);
+/* evicts idle scenes until `bytes` more fit the budget (admission of a new
+ * scene); PRTCL_HIP_ADMISSION_DENIED if they cannot */
+int prtcl_hip_serve_make_room(prtcl_hip_ctx *ctx, uint64_t bytes);
+int prtcl_hip_serve_submit(prtcl_hip_ctx *ctx, int32_t scene, int32_t k_steps,
+                           int64_t *ticket);
+int prtcl_hip_serve_tick(prtcl_hip_ctx *ctx, int32_t *n_stepped);
+int prtcl_hip_serve_poll(prtcl_hip_ctx *ctx, int64_t *tickets, int32_t cap, int32_t *n);
+int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *steps);
+
 /* ---- schedule of the fused passes ---------------------------------------
  * No reference counterpart (tuning of this implementation; results do not
  * depend on it). The setup, pressure-acceleration and pressure passes of

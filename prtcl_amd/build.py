@@ -34,6 +34,7 @@ SOURCES = {
     "fused.hip": [],
     "wkbb18.hip": ["-ffp-contract=off"],
     "scene_impl.hip": [],
+    "serve.hip": [],
     "capi.hip": [],
 }
 

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "scene_impl.hpp"
+#include "serve.hpp"
 
 using prtcl_amd::CtxResources;
 using prtcl_amd::HipError;
@@ -22,6 +23,7 @@ struct prtcl_hip_ctx {
   std::map<int32_t, std::unique_ptr<SceneBase>> scenes;
   int32_t next_id = 0;
   std::string last_error;
+  prtcl_amd::Server server;
 };
 
 namespace {
@@ -241,6 +243,7 @@ int prtcl_hip_scene_destroy(prtcl_hip_ctx *ctx, int32_t scene) {
       return fail(ctx, PRTCL_HIP_NOT_FOUND, "scene does not exist");
     PRTCL_HIP_CHECK(hipStreamSynchronize(ctx->res.stream));
     ctx->scenes.erase(it);
+    ctx->server.forget(scene);
     return PRTCL_HIP_OK;
   });
 }
@@ -359,6 +362,84 @@ int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
     if (!perm)
       return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "null permutation pointer");
     find(ctx, scene)->last_permutation(group, perm);
+    return PRTCL_HIP_OK;
+  });
+}
+
+namespace {
+prtcl_amd::Server::Hooks hooks(prtcl_hip_ctx *ctx) {
+  prtcl_amd::Server::Hooks h;
+  h.find = [ctx](int32_t id) { return find(ctx, id); };
+  h.scene_ids = [ctx]() {
+    std::vector<int32_t> v;
+    for (const auto &kv : ctx->scenes)
+      v.push_back(kv.first);
+    return v;
+  };
+  h.bytes_in_use = [ctx]() { return prtcl_hip_ctx_bytes_in_use(ctx); };
+  h.budget = ctx->budget;
+  return h;
+}
+}  // namespace
+
+int prtcl_hip_scene_evict(prtcl_hip_ctx *ctx, int32_t scene) {
+  return guarded(ctx, [&]() -> int {
+    find(ctx, scene)->evict();
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_scene_resident(prtcl_hip_ctx *ctx, int32_t scene, int32_t *resident) {
+  return guarded(ctx, [&]() -> int {
+    if (!resident)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "resident is NULL");
+    *resident = find(ctx, scene)->resident() ? 1 : 0;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_serve_make_room(prtcl_hip_ctx *ctx, uint64_t bytes) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx->server.make_room(hooks(ctx), bytes))
+      return fail(ctx, PRTCL_HIP_ADMISSION_DENIED,
+                  "not enough idle scenes to evict for the requested bytes");
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_serve_submit(prtcl_hip_ctx *ctx, int32_t scene, int32_t k_steps,
+                           int64_t *ticket) {
+  return guarded(ctx, [&]() -> int {
+    if (!ticket || k_steps < 0)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid request");
+    (void)find(ctx, scene);
+    *ticket = ctx->server.submit(scene, k_steps);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_serve_tick(prtcl_hip_ctx *ctx, int32_t *n_stepped) {
+  return guarded(ctx, [&]() -> int {
+    int status = PRTCL_HIP_OK;
+    const int n = ctx->server.tick(hooks(ctx), &ctx->res, &status);
+    if (n_stepped)
+      *n_stepped = n;
+    return status;
+  });
+}
+
+int prtcl_hip_serve_poll(prtcl_hip_ctx *ctx, int64_t *tickets, int32_t cap, int32_t *n) {
+  return guarded(ctx, [&]() -> int {
+    if (!n || (cap > 0 && !tickets) || cap < 0)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid poll buffer");
+    *n = ctx->server.poll(tickets, cap);
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *steps) {
+  return guarded(ctx, [&]() -> int {
+    ctx->server.pending(requests, steps);
     return PRTCL_HIP_OK;
   });
 }

@@ -305,6 +305,7 @@ class SceneImpl final : public SceneBase {
   }
 
   void upload(int group, int field, const void *host, size_t count) override {
+    ensure_resident();
     Loc L = loc(group, field);
     const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
     if (!L.base)
@@ -347,6 +348,7 @@ class SceneImpl final : public SceneBase {
   }
 
   void download(int group, int field, void *host, size_t count) override {
+    ensure_resident();
     Loc L = loc(group, field);
     const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
     if (!L.base)
@@ -378,6 +380,7 @@ class SceneImpl final : public SceneBase {
   }
 
   double get_global(int g) override {
+    ensure_resident();
     if (g == PRTCL_HIP_G_CURRENT_TIME)
       return static_cast<double>(static_cast<T>(clock_));
     Ctl c = read_ctl();
@@ -402,6 +405,7 @@ class SceneImpl final : public SceneBase {
     }
   }
   void set_global(int g, double v) override {
+    ensure_resident();
     switch (g) {
       case PRTCL_HIP_G_CURRENT_TIME: clock_ = v; break;
       case PRTCL_HIP_G_IISPH_APRDE: ctl_op(kCtlSetAprde, v); break;
@@ -413,9 +417,8 @@ class SceneImpl final : public SceneBase {
   }
 
   // ------------------------------------------------------------------- grid
-  void ensure_grid() {
-    if (grid_mem_)
-      return;
+  // dense cell box around the uploaded particles (one cell of margin)
+  GridDims plan_grid() const {
     double lo[3], hi[3];
     bool any = false;
     for (int k = 0; k < 3; ++k) {
@@ -441,21 +444,34 @@ class SceneImpl final : public SceneBase {
       org[k] = static_cast<int>(std::max(a, -1.0e9));
       dim[k] = static_cast<int>(std::min(std::max(b - a + 1.0, 1.0), static_cast<double>(maxd[k])));
     }
-    gd_.nx = dim[0];
-    gd_.ny = dim[1];
-    gd_.nz = dim[2];
-    gd_.ox = org[0];
-    gd_.oy = org[1];
-    gd_.oz = org[2];
-    gd_.ncells = static_cast<uint32_t>(dim[0]) * dim[1] * dim[2];
-    gd_.radius = radius_;
+    GridDims g{};
+    g.nx = dim[0];
+    g.ny = dim[1];
+    g.nz = dim[2];
+    g.ox = org[0];
+    g.oy = org[1];
+    g.oz = org[2];
+    g.ncells = static_cast<uint32_t>(dim[0]) * dim[1] * dim[2];
+    g.radius = radius_;
+    return g;
+  }
+
+  template <class GP>
+  static uint64_t grid_layout(Arena &A, GP &G, size_t nc) {
+    A.take(G.fstart, nc + 1);
+    A.take(G.bstart, nc + 1);
+    A.take(G.count, nc);
+    A.take(G.scan_tmp, scan_tmp_words(static_cast<uint32_t>(nc)));
+    return A.off;
+  }
+
+  void ensure_grid() {
+    if (grid_mem_)
+      return;
+    gd_ = plan_grid();
     const size_t nc = gd_.ncells;
     Arena A;
-    A.take(G_.fstart, nc + 1);
-    A.take(G_.bstart, nc + 1);
-    A.take(G_.count, nc);
-    A.take(G_.scan_tmp, scan_tmp_words(static_cast<uint32_t>(nc)));
-    grid_bytes_ = A.off;
+    grid_bytes_ = grid_layout(A, G_, nc);
     if (budget_check && !budget_check(grid_bytes_))
       throw std::length_error("grid exceeds the HBM budget");
     PRTCL_HIP_CHECK(hipMalloc(&grid_mem_, grid_bytes_));
@@ -609,6 +625,7 @@ class SceneImpl final : public SceneBase {
   }
 
   void run(int proc) override {
+    ensure_resident();
     hipStream_t st = res_->stream;
     switch (proc) {
       case PRTCL_HIP_P_NSEARCH_UPDATE: nsearch_update(true); break;
@@ -717,6 +734,7 @@ class SceneImpl final : public SceneBase {
   // convergence exits at its first instruction.
   int iisph_solve(double tol, int min_it, int max_it, int *iters, double *rel,
                   bool) override {
+    ensure_resident();
     int rc = solve_impl(tol, min_it, max_it, iters, rel, false);
     PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
     return rc;
@@ -846,6 +864,7 @@ class SceneImpl final : public SceneBase {
 
   // ----------------------------------------------------------------- a step
   int step(int k) override {
+    ensure_resident();
     hipStream_t st = res_->stream;
     int status = PRTCL_HIP_OK;
     for (int s = 0; s < k; ++s) {
@@ -1169,6 +1188,75 @@ class SceneImpl final : public SceneBase {
     return x * per + (x < rem ? x : rem) + k;
   }
 
+  // ------------------------------------------------------------ residency
+  // Eviction copies the device state byte for byte to host memory and frees
+  // the HBM; restoring allocates again, copies back and rebases every device
+  // pointer, so the scene continues bit-identically (same slot order, same
+  // buffers in use).
+  bool resident() const override { return arena_ != nullptr; }
+
+  void evict() override {
+    if (!arena_)
+      return;
+    hipStream_t st = res_->stream;
+    PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+    host_arena_.resize(arena_bytes_);
+    PRTCL_HIP_CHECK(hipMemcpy(host_arena_.data(), arena_, arena_bytes_, hipMemcpyDeviceToHost));
+    if (grid_mem_) {
+      host_grid_.resize(grid_bytes_);
+      PRTCL_HIP_CHECK(hipMemcpy(host_grid_.data(), grid_mem_, grid_bytes_, hipMemcpyDeviceToHost));
+    }
+    rebase_all(static_cast<char *>(arena_), nullptr, static_cast<char *>(grid_mem_), nullptr);
+    PRTCL_HIP_CHECK(hipFree(arena_));
+    arena_ = nullptr;
+    if (grid_mem_) {
+      PRTCL_HIP_CHECK(hipFree(grid_mem_));
+      grid_mem_ = nullptr;
+      grid_evicted_ = true;
+    }
+    bytes = 0;
+  }
+
+  void ensure_resident() override {
+    if (arena_)
+      return;
+    const uint64_t need = arena_bytes_ + (grid_evicted_ ? grid_bytes_ : 0);
+    if (budget_check && !budget_check(need))
+      throw std::length_error("restoring the scene exceeds the HBM budget");
+    void *a = nullptr, *g = nullptr;
+    PRTCL_HIP_CHECK(hipMalloc(&a, arena_bytes_));
+    if (grid_evicted_) {
+      hipError_t e = hipMalloc(&g, grid_bytes_);
+      if (e != hipSuccess) {
+        (void)hipFree(a);
+        throw HipError{e, "hipMalloc(grid)"};
+      }
+    }
+    PRTCL_HIP_CHECK(hipMemcpy(a, host_arena_.data(), arena_bytes_, hipMemcpyHostToDevice));
+    if (g)
+      PRTCL_HIP_CHECK(hipMemcpy(g, host_grid_.data(), grid_bytes_, hipMemcpyHostToDevice));
+    rebase_all(nullptr, static_cast<char *>(a), nullptr, static_cast<char *>(g));
+    arena_ = a;
+    grid_mem_ = g;
+    grid_evicted_ = false;
+    bytes = arena_bytes_ + (g ? grid_bytes_ : 0);
+    std::vector<char>().swap(host_arena_);
+    std::vector<char>().swap(host_grid_);
+  }
+
+  // device bytes this scene still needs before its next step
+  uint64_t resident_bytes_needed() const override {
+    uint64_t need = arena_ ? 0 : arena_bytes_;
+    if (grid_evicted_) {
+      need += grid_bytes_;
+    } else if (!grid_mem_) {
+      Arena A;
+      GridPtrs G;
+      need += grid_layout(A, G, plan_grid().ncells);
+    }
+    return need;
+  }
+
   void set_schedule(int mask) override { tuner_.force(mask); }
   int schedule() const override { return tuner_.current().mask(); }
 
@@ -1184,6 +1272,7 @@ class SceneImpl final : public SceneBase {
   // --------------------------------------------------------- neighbour export
   void copy_neighbors(int group, int ngroup, uint32_t *offsets, uint32_t *indices,
                       uint64_t *total) override {
+    ensure_resident();
     require_grid();
     if ((group != 0 && group != 1) || (ngroup != 0 && ngroup != 1))
       throw std::invalid_argument("group");
@@ -1278,11 +1367,48 @@ class SceneImpl final : public SceneBase {
       A.take(*q, nv);
   }
 
+  // every device pointer into the arena (resp. the grid memory)
+  template <class F>
+  void each_arena_ptr(F &&f) {
+    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
+                   &P_.btmpV4, &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb})
+      f(reinterpret_cast<void **>(q));
+    for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp})
+      f(reinterpret_cast<void **>(q));
+    for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
+      f(reinterpret_cast<void **>(q));
+    for (uint32_t **q : {&P_.fcell, &P_.frank, &P_.fperm, &P_.ninfo, &P_.bcell, &P_.brank,
+                         &P_.bperm})
+      f(reinterpret_cast<void **>(q));
+    f(reinterpret_cast<void **>(&P_.nidx));
+    f(reinterpret_cast<void **>(&P_.nblk));
+    f(reinterpret_cast<void **>(&P_.partials));
+    f(reinterpret_cast<void **>(&P_.cg_part));
+    f(reinterpret_cast<void **>(&P_.ctl));
+  }
+  template <class F>
+  void each_grid_ptr(F &&f) {
+    for (uint32_t **q : {&G_.fstart, &G_.bstart, &G_.count, &G_.scan_tmp})
+      f(reinterpret_cast<void **>(q));
+  }
+  // pointers relative to old base (or offsets when old is null) -> new base
+  // (or offsets when new is null)
+  void rebase_all(char *old_a, char *new_a, char *old_g, char *new_g) {
+    auto mv = [](void **q, char *from, char *to) {
+      const uintptr_t off = reinterpret_cast<uintptr_t>(*q) - reinterpret_cast<uintptr_t>(from);
+      *q = reinterpret_cast<void *>(reinterpret_cast<uintptr_t>(to) + off);
+    };
+    each_arena_ptr([&](void **q) { mv(q, old_a, new_a); });
+    if (grid_bytes_ && (old_g || new_g))
+      each_grid_ptr([&](void **q) { mv(q, old_g, new_g); });
+  }
+
   void allocate() {
     Arena A;
     layout(A, P_, nf_, nb_, desc.implicit_viscosity != 0);
     PRTCL_HIP_CHECK(hipMalloc(&arena_, A.off));
     bytes = A.off;
+    arena_bytes_ = A.off;
     char *base = static_cast<char *>(arena_);
     for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
                    &P_.btmpV4,
@@ -1323,6 +1449,9 @@ class SceneImpl final : public SceneBase {
   double radius_ = 1;
   void *arena_ = nullptr;
   void *grid_mem_ = nullptr;
+  uint64_t arena_bytes_ = 0;
+  std::vector<char> host_arena_, host_grid_;  // state of an evicted scene
+  bool grid_evicted_ = false;
   uint64_t grid_bytes_ = 0;
   Ptrs P_{};
   GridPtrs G_{};
@@ -1361,6 +1490,7 @@ int step_scenes(const std::vector<SceneBase *> &scenes, int k, CtxResources *res
     for (size_t j = 0; j < i; ++j)
       if (scenes[j] == scenes[i])
         throw std::invalid_argument("a scene is listed twice");
+    scenes[i]->ensure_resident();
     if (auto *f = dynamic_cast<SceneImpl<float> *>(scenes[i]); f && f->batchable())
       bf.push_back(f);
     else if (auto *d = dynamic_cast<SceneImpl<double> *>(scenes[i]); d && d->batchable())

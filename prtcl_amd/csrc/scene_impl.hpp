@@ -96,6 +96,13 @@ class SceneBase {
   // acceleration, 2 pressure in LDS), -1 returns to the tuner
   virtual void set_schedule(int mask) = 0;
   virtual int schedule() const = 0;
+  // residency: evict() moves the device state to host memory and frees the
+  // HBM; ensure_resident() brings it back (throws std::length_error when the
+  // budget does not allow it)
+  virtual bool resident() const = 0;
+  virtual void evict() = 0;
+  virtual void ensure_resident() = 0;
+  virtual uint64_t resident_bytes_needed() const = 0;
 };
 
 SceneBase *make_scene(const prtcl_hip_scene_desc &desc, CtxResources *res);

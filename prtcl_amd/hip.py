@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Dict, Optional, Sequence
+from typing import List, Dict, Optional, Sequence
 
 import numpy as np
 
@@ -49,6 +49,9 @@ EXPORTS = [
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
     "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
     "prtcl_hip_set_schedule", "prtcl_hip_get_schedule",
+    "prtcl_hip_scene_evict", "prtcl_hip_scene_resident", "prtcl_hip_serve_make_room",
+    "prtcl_hip_serve_submit", "prtcl_hip_serve_tick", "prtcl_hip_serve_poll",
+    "prtcl_hip_serve_pending",
     "prtcl_hip_copy_neighbors", "prtcl_hip_last_permutation", "prtcl_hip_profile_enable",
     "prtcl_hip_profile_read",
 ]
@@ -123,6 +126,13 @@ def lib():
                                             C.POINTER(C.c_double)]
         L.prtcl_hip_step.argtypes = [vp, C.POINTER(i32), i32, i32]
         L.prtcl_hip_set_schedule.argtypes = [vp, i32, i32]
+        L.prtcl_hip_scene_evict.argtypes = [vp, i32]
+        L.prtcl_hip_scene_resident.argtypes = [vp, i32, C.POINTER(i32)]
+        L.prtcl_hip_serve_make_room.argtypes = [vp, C.c_uint64]
+        L.prtcl_hip_serve_submit.argtypes = [vp, i32, i32, C.POINTER(i64)]
+        L.prtcl_hip_serve_tick.argtypes = [vp, C.POINTER(i32)]
+        L.prtcl_hip_serve_poll.argtypes = [vp, C.POINTER(i64), i32, C.POINTER(i32)]
+        L.prtcl_hip_serve_pending.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
         L.prtcl_hip_get_schedule.argtypes = [vp, i32, C.POINTER(i32)]
         L.prtcl_hip_last_step_info.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(C.c_double),
                                                C.POINTER(i32)]
@@ -213,6 +223,36 @@ class Context:
             out[name.value.decode()] = (ms.value, n.value)
             k += 1
         return out
+
+    # ---- residency and serving
+    def make_room(self, nbytes: int):
+        """Evict idle scenes until nbytes more fit the HBM budget."""
+        self.check(self.L.prtcl_hip_serve_make_room(self.h, int(nbytes)), "make_room")
+
+    def submit(self, scene: "HipScene", k: int) -> int:
+        t = C.c_int64()
+        self.check(self.L.prtcl_hip_serve_submit(self.h, scene.id, int(k), C.byref(t)),
+                   "serve_submit")
+        return t.value
+
+    def tick(self) -> int:
+        """One step of every scene with pending requests (one batched step)."""
+        n = C.c_int32()
+        rc = self.L.prtcl_hip_serve_tick(self.h, C.byref(n))
+        if rc not in (OK, NOT_CONVERGED):
+            self.check(rc, "serve_tick")
+        return n.value
+
+    def poll(self, cap: int = 4096) -> List[int]:
+        buf = (C.c_int64 * cap)()
+        n = C.c_int32()
+        self.check(self.L.prtcl_hip_serve_poll(self.h, buf, cap, C.byref(n)), "serve_poll")
+        return [buf[k] for k in range(n.value)]
+
+    def pending(self):
+        r, st = C.c_int64(), C.c_int64()
+        self.check(self.L.prtcl_hip_serve_pending(self.h, C.byref(r), C.byref(st)), "pending")
+        return r.value, st.value
 
     def set_batch_schedule(self, mask: int):
         """Force the LDS/global schedule of batched steps (-1: autotune)."""
@@ -308,6 +348,16 @@ class HipScene:
 
     def step(self, k: int = 1) -> int:
         return self.ctx.step([self], k)
+
+    def evict(self):
+        """Move the device state to host memory (restored by the next use)."""
+        self.ctx.check(self.L.prtcl_hip_scene_evict(self.ctx.h, self.id), "scene_evict")
+
+    def resident(self) -> bool:
+        r = C.c_int32()
+        self.ctx.check(self.L.prtcl_hip_scene_resident(self.ctx.h, self.id, C.byref(r)),
+                       "scene_resident")
+        return bool(r.value)
 
     def set_schedule(self, mask: int):
         """Force the LDS/global schedule of the fused passes: bit 0 setup,

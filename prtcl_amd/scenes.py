@@ -95,9 +95,19 @@ def box_boundary(nx: int, ny: int, nz: int, spacing: float) -> np.ndarray:
     """Single-layer closed box: every lattice point (i, j, k) * spacing with
     0 <= i < nx, ... that lies on the box surface.
     Count = nx*ny*nz - (nx-2)*(ny-2)*(nz-2)."""
-    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
-    surf = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
-    pts = np.stack([i[surf], j[surf], k[surf]], axis=1).astype(np.float64) * spacing
+    # lexicographic (i, j, k) order, built slab by slab: the two end slabs
+    # are full faces, every slab between them is the same (j, k) ring
+    j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    face = np.stack([j.ravel(), k.ravel()], axis=1)
+    on_ring = (face[:, 0] == 0) | (face[:, 0] == ny - 1) | (face[:, 1] == 0) | (face[:, 1] == nz - 1)
+    ring = face[on_ring]
+    blocks = [np.column_stack([np.zeros(len(face), np.int64), face])]
+    if nx > 2:
+        ii = np.repeat(np.arange(1, nx - 1), len(ring))
+        blocks.append(np.column_stack([ii, np.tile(ring, (nx - 2, 1))]))
+    if nx > 1:
+        blocks.append(np.column_stack([np.full(len(face), nx - 1, np.int64), face]))
+    pts = np.concatenate(blocks).astype(np.float64) * spacing
     return pts.astype(np.float32)
 
 
