@@ -36,6 +36,10 @@ STAGE_BYTES = {
     "iisph_setup": (44, 16),                  # two passes: 32 + 56 B, per launch mean
     "integrate_position": (36, 0),            # x, v -> x
     "nsearch_sort": (96, 0),
+    "gravity_integrate_v": (40, 0),           # v, a, t_birth -> v
+    "wkbb18_diagonal": (56, 16),              # x, rho, m -> sp
+    "pcg_spmv": (44, 16),                     # q = A p and p.q, per CG iteration
+    "pcg_vector": (156, 0),                   # x, r, y updates, dots, new p
 }
 
 
@@ -45,7 +49,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dambreak_1m",
-                    choices=["dambreak_1m", "dambreak_8k"])
+                    choices=["dambreak_1m", "dambreak_8k", "viscous_250k", "served_stream"])
+    ap.add_argument("--duration", type=float, default=30.0,
+                    help="served stream: length of the arrival trace in seconds")
+    ap.add_argument("--rate", type=float, default=200.0,
+                    help="served stream: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-scenes", type=int, default=512)
+    ap.add_argument("--serve-warm-steps", type=int, default=3)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU baseline sample")
@@ -67,9 +78,13 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.workload == "served_stream":
+        bench_serve(args)
+        return
     from prtcl_amd import hip, scenes
 
-    sc = scenes.config1() if args.workload == "dambreak_1m" else scenes.config0()
+    sc = {"dambreak_1m": scenes.config1, "dambreak_8k": scenes.config0,
+          "viscous_250k": scenes.config2}[args.workload]()
     sc.permute_every = 8
     ctx = hip.Context(0)
     g = hip.HipScene(ctx, sc)
@@ -180,11 +195,67 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, g, args.cpu_seconds)
 
-    if args.batch_scenes > 0:
+    if args.batch_scenes > 0 and args.workload == "dambreak_1m":
         g.destroy()
         out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
 
     print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Served stream (config 4): Poisson arrivals of "advance scene s by
+    k = 10 steps" over resident scenes with N_f log-uniform in [5k, 500k], 30%
+    with implicit viscosity; scenes admitted against the HBM budget on their
+    first request. Reports request latency percentiles and the aggregate
+    particle-steps/s of the run (scene creation included in the latency of a
+    scene's first request)."""
+    from prtcl_amd import hip, scenes, serving
+    t0 = time.perf_counter()
+    scs = scenes.config4_scenes(args.serve_scenes)
+    gen_s = time.perf_counter() - t0
+    ctx = hip.Context(0, budget=int(args.budget_gb * 1e9))
+    ss = serving.SceneSet(ctx, lambda i: scs[i], warm_steps=args.serve_warm_steps)
+    # admission and set-up of every scene before the trace (the scenes are
+    # resident and running when the stream starts: each has taken
+    # serve_warm_steps steps, past the start-up transient of its first solves)
+    t0 = time.perf_counter()
+    for i in range(len(scs)):
+        ss.get(i)
+    ctx.synchronize()
+    admit_s = time.perf_counter() - t0
+    arr = scenes.poisson_arrivals(4, args.rate, args.duration, len(scs))
+    out = serving.replay(ctx, ss, arr, k=10)
+    lat = out["latencies"]
+    visc = np.array([scs[i].implicit_viscosity for i in out["latency_scene"]], bool)
+    tk = out["tick_ms"]
+    nf = sum(s.n_fluid for s in scs)
+    nvisc = sum(1 for s in scs if s.implicit_viscosity)
+    res = {
+        "metric": "served stream: request latency p50/p99 and particle-steps/s",
+        "value": round(out["particle_steps"] / out["wall"], 1),
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": out["ticks"], "warmup": 0,
+        "ms_per_step": round(1e3 * out["wall"] / max(out["ticks"], 1), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "served_stream", "scenes": len(scs), "viscous_scenes": nvisc,
+                   "fluid_particles_total": nf, "rate_per_s": args.rate, "k_steps": 10,
+                   "trace_s": args.duration, "hbm_budget_gb": args.budget_gb},
+        "requests": out["requests"],
+        "p50_request_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+        "p99_request_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+        "mean_request_ms": round(1e3 * float(np.mean(lat)), 3),
+        "p50_request_ms_inviscid": round(1e3 * float(np.percentile(lat[~visc], 50)), 3),
+        "p50_request_ms_viscous": round(1e3 * float(np.percentile(lat[visc], 50)), 3)
+        if visc.any() else None,
+        "tick_ms_p50": round(float(np.percentile(tk, 50)), 3),
+        "tick_ms_p99": round(float(np.percentile(tk, 99)), 3),
+        "tick_ms_max": round(float(tk.max()), 3),
+        "scenes_per_tick_mean": round(float(out["active"].mean()), 2),
+        "wall_s": round(out["wall"], 3), "scene_generation_s": round(gen_s, 2),
+        "admission_s": round(admit_s, 2), "warm_steps_per_scene": args.serve_warm_steps,
+        "device_gb": round(ctx.bytes_in_use() / 1e9, 3),
+    }
+    print(json.dumps(res), flush=True)
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):

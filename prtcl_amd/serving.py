@@ -24,10 +24,12 @@ from . import hip
 class SceneSet:
     """Scenes of a served workload, created on first use."""
 
-    def __init__(self, ctx: hip.Context, make: Callable[[int], object], prepare=True):
+    def __init__(self, ctx: hip.Context, make: Callable[[int], object], prepare=True,
+                 warm_steps: int = 0):
         self.ctx = ctx
         self.make = make
         self.prepare = prepare
+        self.warm_steps = warm_steps  # steps run at admission, before any request
         self.live: Dict[int, hip.HipScene] = {}
         self.nf: Dict[int, int] = {}
 
@@ -50,6 +52,8 @@ class SceneSet:
             h = hip.HipScene(self.ctx, sc)
             if self.prepare:
                 h.prepare()
+            if self.warm_steps:
+                h.step(self.warm_steps)
             self.live[index] = h
             self.nf[index] = sc.n_fluid
         return h
@@ -62,6 +66,9 @@ def replay(ctx: hip.Context, scenes: SceneSet, arrivals: Sequence[Tuple[float, i
     t0 = time.perf_counter()
     inflight: Dict[int, Tuple[float, int]] = {}
     lat: List[float] = []
+    lat_scene: List[int] = []
+    tick_ms: List[float] = []
+    active: List[int] = []
     psteps = 0
     ticks = 0
     i = 0
@@ -74,7 +81,10 @@ def replay(ctx: hip.Context, scenes: SceneSet, arrivals: Sequence[Tuple[float, i
             tk = ctx.submit(scenes.get(idx), k)
             inflight[tk] = (t_arr, idx)
         if inflight:
-            ctx.tick()
+            tt = time.perf_counter()
+            nstep = ctx.tick()
+            tick_ms.append(1e3 * (time.perf_counter() - tt))
+            active.append(nstep)
             ticks += 1
             if on_tick:
                 on_tick()
@@ -82,9 +92,11 @@ def replay(ctx: hip.Context, scenes: SceneSet, arrivals: Sequence[Tuple[float, i
             for tk in ctx.poll():
                 t_arr, idx = inflight.pop(tk)
                 lat.append(t_done - t_arr)
+                lat_scene.append(idx)
                 psteps += k * scenes.nf[idx]
         elif i < n:
             time.sleep(max(0.0, min(arrivals[i][0] - (time.perf_counter() - t0), 0.01)))
     wall = time.perf_counter() - t0
-    return {"latencies": np.asarray(lat), "particle_steps": psteps, "wall": wall,
-            "ticks": ticks, "requests": len(lat)}
+    return {"latencies": np.asarray(lat), "latency_scene": np.asarray(lat_scene),
+            "particle_steps": psteps, "wall": wall, "ticks": ticks, "requests": len(lat),
+            "tick_ms": np.asarray(tick_ms), "active": np.asarray(active)}
