@@ -33,14 +33,34 @@ STAGE_BYTES = {
     "iisph_pressure_acceleration": (36, 16),  # x, p, rho, m -> a
     "iisph_pressure": (52, 16),               # x, a, m, rho, s, AA, p -> Ap, p
     "density": (20, 16),                      # x, m -> rho
-    "iisph_setup": (44, 16),                  # two passes: 32 + 56 B, per launch mean
-    "integrate_position": (36, 0),            # x, v -> x
+    # the fused setup pass does the work of density (20), a = g (12),
+    # integrate v (40), setup c (32) and setup s/AA (56), three boundary passes
+    "iisph_setup": (160, 48),
+    # the fused integration: integrate v (40) + integrate x (36)
+    "integrate_position": (76, 0),
     "nsearch_sort": (96, 0),
     "gravity_integrate_v": (40, 0),           # v, a, t_birth -> v
     "wkbb18_diagonal": (56, 16),              # x, rho, m -> sp
     "pcg_spmv": (44, 16),                     # q = A p and p.q, per CG iteration
     "pcg_vector": (156, 0),                   # x, r, y updates, dots, new p
 }
+
+
+def profiled_traffic(stage):
+    """HBM bytes per launch of a stage of the 1M dam break, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a plain copy
+    kernel); None when the summary has no row for it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_dambreak_1m.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == stage:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
 
 
 def parse():
@@ -154,7 +174,8 @@ def main():
     achieved = bytes_launch / avg_s / 1e9
     roofline = {
         "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": profiled_traffic(dom) if args.workload == "dambreak_1m" else None,
         "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5),
         "launches": dom_n,
     }
