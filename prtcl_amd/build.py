@@ -71,7 +71,29 @@ def build(verbose: bool = False, jobs: int = 6) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_compat(verbose)
     return LIB
+
+
+COMPAT_SRC = os.path.join(HERE, "compat", "prtcl_compat.cpp")
+COMPAT_LIB = os.path.join(LIBDIR, "libprtcl_compat.so")
+
+
+def build_compat(verbose: bool = False) -> str:
+    """The C++ host interface (Model, Neighborhood, SchemeBase, ... and the
+    GPU schemes) as libprtcl_compat.so over libprtcl_hip.so; host-only code,
+    built with g++."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = COMPAT_LIB + ".tmp"
+    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", COMPAT_SRC,
+           "-L" + LIBDIR, "-lprtcl_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compat build failed:\n{r.stderr}")
+    os.replace(tmp, COMPAT_LIB)
+    return COMPAT_LIB
 
 
 if __name__ == "__main__":

@@ -1,0 +1,681 @@
+// Implementation of the source-compatible host interface (prtcl_compat.hpp)
+// and of the GPU schemes over the C ABI.
+#include "prtcl_compat.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+namespace prtcl {
+
+// ---- fields ----------------------------------------------------------------
+void FieldStore::Permute(const size_t *perm, size_t n) {
+  const size_t item = elem_ * extent_;
+  if (n != Items())
+    throw std::invalid_argument("permutation size differs from the item count");
+  std::vector<unsigned char> out(bytes_.size());
+  for (size_t i = 0; i < n; ++i) {
+    if (perm[i] >= n)
+      throw std::out_of_range("permutation index out of range");
+    std::memcpy(out.data() + i * item, bytes_.data() + perm[i] * item, item);
+  }
+  bytes_.swap(out);
+}
+
+FieldStore *UniformManager::Find(std::string_view name) const {
+  auto it = fields_.find(std::string(name));
+  return it == fields_.end() ? nullptr : it->second.get();
+}
+
+FieldStore *UniformManager::Add(std::string_view name, const std::type_info &t, size_t elem,
+                                size_t extent) {
+  auto &slot = fields_[std::string(name)];
+  if (slot) {
+    if (!slot->Same(t, extent))
+      throw FieldOfDifferentTypeAlreadyExistsError{};
+    return slot.get();
+  }
+  slot = std::make_unique<FieldStore>(t, elem, extent);
+  slot->Resize(1);
+  return slot.get();
+}
+
+std::vector<std::string> UniformManager::GetFieldNames() const {
+  std::vector<std::string> v;
+  for (const auto &kv : fields_)
+    v.push_back(kv.first);
+  return v;
+}
+
+FieldStore *VaryingManager::Find(std::string_view name) const {
+  auto it = fields_.find(std::string(name));
+  return it == fields_.end() ? nullptr : it->second.get();
+}
+
+FieldStore *VaryingManager::Add(std::string_view name, const std::type_info &t, size_t elem,
+                                size_t extent) {
+  auto &slot = fields_[std::string(name)];
+  if (slot) {
+    if (!slot->Same(t, extent))
+      throw FieldOfDifferentTypeAlreadyExistsError{};
+    return slot.get();
+  }
+  slot = std::make_unique<FieldStore>(t, elem, extent);
+  slot->Resize(count_);
+  return slot.get();
+}
+
+void VaryingManager::ResizeItems(size_t n) {
+  for (auto &kv : fields_)
+    kv.second->Resize(n);
+  count_ = n;
+  dirty_ = true;
+}
+
+void VaryingManager::PermuteItems(const size_t *perm, size_t n) {
+  if (n != count_)
+    throw std::invalid_argument("permutation size differs from the item count");
+  for (auto &kv : fields_)
+    kv.second->Permute(perm, n);
+  dirty_ = true;
+}
+
+std::vector<std::string> VaryingManager::GetFieldNames() const {
+  std::vector<std::string> v;
+  for (const auto &kv : fields_)
+    v.push_back(kv.first);
+  return v;
+}
+
+// ---- model -----------------------------------------------------------------
+Model::~Model() { hip::Forget(this); }
+
+Group &Model::AddGroup(std::string_view name, std::string_view type) {
+  if (name.empty())
+    throw std::invalid_argument("invalid identifier");
+  auto it = groups_.find(name);
+  if (it == groups_.end()) {
+    it = groups_.emplace(std::string(name), std::make_unique<Group>(*this, name, type)).first;
+  } else if (it->second->GetGroupType() != type) {
+    throw GroupOfDifferentTypeAlreadyExists{};
+  }
+  Reindex();
+  return *it->second;
+}
+
+void Model::Reindex() {
+  by_index_.clear();
+  for (auto &kv : groups_) {
+    kv.second->SetGroupIndex(static_cast<GroupIndex>(by_index_.size()));
+    by_index_.push_back(kv.second.get());
+  }
+}
+
+Group *Model::TryGetGroup(std::string_view name) {
+  auto it = groups_.find(name);
+  return it == groups_.end() ? nullptr : it->second.get();
+}
+
+void Model::RemoveGroup(std::string_view name) {
+  auto it = groups_.find(name);
+  if (it != groups_.end()) {
+    groups_.erase(it);
+    Reindex();
+  }
+}
+
+std::vector<Group *> Model::GetGroups() { return by_index_; }
+
+std::vector<std::string> Model::GetGroupNames() const {
+  std::vector<std::string> v;
+  for (const auto &kv : groups_)
+    v.push_back(kv.first);
+  return v;
+}
+
+bool Model::IsDirty() const {
+  for (const auto &kv : groups_)
+    if (kv.second->IsDirty())
+      return true;
+  return false;
+}
+
+void Model::SetDirty(bool v) {
+  for (auto &kv : groups_)
+    kv.second->SetDirty(v);
+}
+
+// ---- schemes registry --------------------------------------------------------
+SchemeRegistry &GetSchemeRegistry() {
+  static SchemeRegistry registry;
+  return registry;
+}
+
+// ---- device mirror -----------------------------------------------------------
+namespace hip {
+namespace {
+
+void check(prtcl_hip_ctx *ctx, int rc, const char *what) {
+  if (rc != PRTCL_HIP_OK && rc != PRTCL_HIP_NOT_CONVERGED)
+    throw std::runtime_error(std::string(what) + ": " + prtcl_hip_last_error(ctx));
+}
+
+struct Mirror {
+  int32_t scene = -1;
+  prtcl_hip_scene_desc desc{};
+  double radius = 0;  // from the Neighborhood; 0: 2h
+};
+
+std::mutex g_mu;
+std::map<const Model *, Mirror> g_mirrors;
+
+struct Mapped {
+  int group;
+  int field;
+  const char *name;
+  size_t extent;
+};
+const Mapped kMapped[] = {
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_POSITION, "position", 3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_VELOCITY, "velocity", 3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_ACCELERATION, "acceleration", 3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_DENSITY, "density", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PRESSURE, "pressure", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_MASS, "mass", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_IISPH_HELPER_C, "iisph_helper_c", 3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_IISPH_SOURCE_TERM, "iisph_source_term", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_IISPH_DIAGONAL_ELEMENT, "iisph_diagonal_element", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_IISPH_RIGHT_HAND_SIDE, "iisph_right_hand_side", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_TIME_OF_BIRTH, "time_of_birth", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_WKBB18_VELOCITY_DELTA, "wkbb18_velocity_delta", 3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL, "wkbb18_system_diagonal", 9},
+    {PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_POSITION, "position", 3},
+    {PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_VOLUME, "volume", 1},
+};
+
+Group *group_of_type(Model &m, std::string_view type) {
+  Group *found = nullptr;
+  for (Group *g : m.GetGroups())
+    if (g->GetGroupType() == type) {
+      if (found)
+        throw std::invalid_argument("the device mirror supports one group of each type");
+      found = g;
+    }
+  return found;
+}
+
+template <class T>
+bool read_uniform(const UniformManager &u, const char *name, T *out, size_t k = 0) {
+  FieldStore *f = u.Find(name);
+  if (!f)
+    return false;
+  if (f->Same(typeid(float), f->Extent()) && k < f->Extent())
+    *out = static_cast<T>(static_cast<const float *>(f->Data())[k]);
+  else if (f->Same(typeid(double), f->Extent()) && k < f->Extent())
+    *out = static_cast<T>(static_cast<const double *>(f->Data())[k]);
+  else if (f->Same(typeid(int32_t), f->Extent()) && k < f->Extent())
+    *out = static_cast<T>(static_cast<const int32_t *>(f->Data())[k]);
+  else
+    return false;
+  return true;
+}
+
+template <class T>
+void write_uniform(const UniformManager &u, const char *name, T v) {
+  FieldStore *f = u.Find(name);
+  if (!f || f->Extent() != 1)
+    return;
+  if (f->Same(typeid(float), 1))
+    *static_cast<float *>(f->Data()) = static_cast<float>(v);
+  else if (f->Same(typeid(double), 1))
+    *static_cast<double *>(f->Data()) = static_cast<double>(v);
+  else if (f->Same(typeid(int32_t), 1))
+    *static_cast<int32_t *>(f->Data()) = static_cast<int32_t>(v);
+}
+
+prtcl_hip_scene_desc desc_of(Model &m, double radius) {
+  prtcl_hip_scene_desc d;
+  prtcl_hip_scene_desc_init(&d);
+  Group *f = group_of_type(m, "fluid");
+  Group *b = group_of_type(m, "boundary");
+  d.n_fluid = f ? static_cast<int64_t>(f->GetItemCount()) : 0;
+  d.n_boundary = b ? static_cast<int64_t>(b->GetItemCount()) : 0;
+  const UniformManager &g = m.GetGlobal();
+  read_uniform(g, "smoothing_scale", &d.smoothing_scale);
+  read_uniform(g, "time_step", &d.time_step);
+  read_uniform(g, "iisph_relaxation", &d.iisph_relaxation);
+  read_uniform(g, "fade_duration", &d.fade_duration);
+  for (int k = 0; k < 3; ++k)
+    read_uniform(g, "gravity", &d.gravity[k], k);
+  if (f) {
+    read_uniform(f->GetUniform(), "rest_density", &d.rest_density);
+    read_uniform(f->GetUniform(), "dynamic_viscosity", &d.fluid_viscosity);
+    read_uniform(f->GetUniform(), "wkbb18_maximum_error", &d.wkbb18_maximum_error);
+    read_uniform(f->GetUniform(), "wkbb18_maximum_iterations", &d.wkbb18_maximum_iterations);
+  }
+  if (b)
+    read_uniform(b->GetUniform(), "dynamic_viscosity", &d.boundary_viscosity);
+  d.radius = radius;
+  d.permute_every = 0;
+  d.step_mode = 1;
+  return d;
+}
+
+bool same_desc(const prtcl_hip_scene_desc &a, const prtcl_hip_scene_desc &b) {
+  return std::memcmp(&a, &b, sizeof(a)) == 0;
+}
+
+// the scene of a model, created or re-created when the model's shape or
+// parameters changed (all mirrored fields uploaded then)
+int32_t scene_locked(Model &m, Mirror &mr) {
+  prtcl_hip_ctx *ctx = Context();
+  const prtcl_hip_scene_desc d = desc_of(m, mr.radius);
+  if (mr.scene >= 0 && same_desc(d, mr.desc))
+    return mr.scene;
+  if (mr.scene >= 0)
+    check(ctx, prtcl_hip_scene_destroy(ctx, mr.scene), "scene_destroy");
+  mr.scene = -1;
+  int32_t id = -1;
+  prtcl_hip_scene_desc dd = d;
+  check(ctx, prtcl_hip_scene_create(ctx, &dd, &id), "scene_create");
+  mr.scene = id;
+  mr.desc = d;
+  return id;
+}
+
+void transfer(Model &m, int32_t scene, bool up) {
+  prtcl_hip_ctx *ctx = Context();
+  Group *groups[2] = {group_of_type(m, "boundary"), group_of_type(m, "fluid")};
+  for (const Mapped &mf : kMapped) {
+    Group *g = groups[mf.group];
+    if (!g || g->GetItemCount() == 0)
+      continue;
+    FieldStore *fs = g->GetVarying().Find(mf.name);
+    if (!fs || !fs->Same(typeid(float), mf.extent))
+      continue;
+    const size_t count = g->GetItemCount() * mf.extent;
+    if (up)
+      check(ctx, prtcl_hip_upload_field(ctx, scene, mf.group, mf.field, fs->Data(), count),
+            "upload_field");
+    else
+      check(ctx, prtcl_hip_download_field(ctx, scene, mf.group, mf.field, fs->Data(), count),
+            "download_field");
+  }
+  const UniformManager &gl = m.GetGlobal();
+  if (up) {
+    // globals the host loop sets (example.lua:368-369, 414)
+    double v = 0;
+    if (read_uniform(gl, "current_time", &v))
+      check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_CURRENT_TIME, v), "set_global");
+    if (read_uniform(gl, "iisph_aprde", &v))
+      check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_IISPH_APRDE, v), "set_global");
+    if (read_uniform(gl, "iisph_nprde", &v))
+      check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_IISPH_NPRDE, v), "set_global");
+  } else {
+    double v = 0;
+    check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_IISPH_APRDE, &v), "get_global");
+    write_uniform(gl, "iisph_aprde", v);
+    check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_IISPH_NPRDE, &v), "get_global");
+    write_uniform(gl, "iisph_nprde", v);
+    check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_MAXIMUM_SPEED, &v), "get_global");
+    write_uniform(gl, "maximum_speed", v);
+    if (Group *f = groups[PRTCL_HIP_FLUID]) {
+      check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_WKBB18_ITERATIONS, &v),
+            "get_global");
+      write_uniform(f->GetUniform(), "wkbb18_iterations", v);
+    }
+  }
+}
+
+int device_group(Model &m, size_t group_index) {
+  Group *g = m.GetGroupByIndex(group_index);
+  if (g->GetGroupType() == "fluid")
+    return PRTCL_HIP_FLUID;
+  if (g->GetGroupType() == "boundary")
+    return PRTCL_HIP_BOUNDARY;
+  return -1;
+}
+
+}  // namespace
+
+prtcl_hip_ctx *Context() {
+  static prtcl_hip_ctx *ctx = [] {
+    prtcl_hip_ctx *c = nullptr;
+    if (prtcl_hip_ctx_create(0, 0, &c) != PRTCL_HIP_OK)
+      throw std::runtime_error("prtcl_hip_ctx_create failed (no MI355X device?)");
+    return c;
+  }();
+  return ctx;
+}
+
+int32_t SceneOf(Model &model) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Mirror &mr = g_mirrors[&model];
+  const int32_t old = mr.scene;
+  const int32_t id = scene_locked(model, mr);
+  if (id != old)
+    transfer(model, id, true);
+  return id;
+}
+
+void Upload(Model &model) { transfer(model, SceneOf(model), true); }
+void Download(Model &model) { transfer(model, SceneOf(model), false); }
+
+void Forget(const Model *model) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_mirrors.find(model);
+  if (it == g_mirrors.end())
+    return;
+  if (it->second.scene >= 0)
+    (void)prtcl_hip_scene_destroy(Context(), it->second.scene);
+  g_mirrors.erase(it);
+}
+
+namespace {
+
+void set_radius(Model &m, double r) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_mirrors[&m].radius = r;
+}
+
+// run one procedure with the host spans authoritative before and after
+void run(Model &m, int proc) {
+  prtcl_hip_ctx *ctx = Context();
+  const int32_t sc = SceneOf(m);
+  transfer(m, sc, true);
+  check(ctx, prtcl_hip_run(ctx, sc, proc), "run");
+  transfer(m, sc, false);
+}
+
+}  // namespace
+}  // namespace hip
+
+// ---- neighbourhood -----------------------------------------------------------
+void Neighborhood::SetRadius(double radius) {
+  radius_ = radius;
+  if (model_)
+    hip::set_radius(*model_, radius);
+}
+
+void Neighborhood::Load(const Model &model) {
+  model_ = const_cast<Model *>(&model);
+  hip::set_radius(*model_, radius_);
+  csr_.clear();
+}
+
+void Neighborhood::Update() {
+  if (!model_)
+    throw std::logic_error("neighborhood has no model");
+  hip::run(*model_, PRTCL_HIP_P_NSEARCH_UPDATE);
+  csr_.clear();
+}
+
+void Neighborhood::Permute(Model &model) {
+  if (&model != model_)
+    Load(model);
+  prtcl_hip_ctx *ctx = hip::Context();
+  const int32_t sc = hip::SceneOf(model);
+  hip::transfer(model, sc, true);
+  hip::check(ctx, prtcl_hip_run(ctx, sc, PRTCL_HIP_P_NSEARCH_PERMUTE), "permute");
+  // the device's host order is now its slot order: apply the same gather to
+  // every varying field of the mirrored groups
+  for (const char *type : {"boundary", "fluid"}) {
+    Group *g = hip::group_of_type(model, type);
+    if (!g || g->GetItemCount() == 0)
+      continue;
+    const int dg = std::string_view(type) == "fluid" ? PRTCL_HIP_FLUID : PRTCL_HIP_BOUNDARY;
+    std::vector<uint32_t> p(g->GetItemCount());
+    hip::check(ctx, prtcl_hip_last_permutation(ctx, sc, dg, p.data()), "last_permutation");
+    std::vector<size_t> perm(p.begin(), p.end());
+    g->Permute(perm.data(), perm.size());
+  }
+  csr_.clear();
+}
+
+void Neighborhood::CopyNeighbors(size_t g_, size_t i_,
+                                 std::vector<std::vector<size_t>> &neighbors) const {
+  if (!model_)
+    throw std::logic_error("neighborhood has no model");
+  prtcl_hip_ctx *ctx = hip::Context();
+  const int32_t sc = hip::SceneOf(*model_);
+  const size_t ng = model_->GetGroupCount();
+  neighbors.resize(ng);
+  const int qg = hip::device_group(*model_, g_);
+  for (size_t h = 0; h < ng; ++h) {
+    neighbors[h].clear();
+    const int dg = hip::device_group(*model_, h);
+    if (qg < 0 || dg < 0)
+      continue;
+    auto key = std::make_pair(g_, h);
+    auto it = csr_.find(key);
+    if (it == csr_.end()) {
+      const size_t nq = model_->GetGroupByIndex(g_)->GetItemCount();
+      std::vector<uint32_t> off(nq + 1);
+      uint64_t total = 0;
+      hip::check(ctx, prtcl_hip_copy_neighbors(ctx, sc, qg, dg, off.data(), nullptr, &total),
+                 "copy_neighbors");
+      std::vector<uint32_t> idx(total);
+      hip::check(ctx, prtcl_hip_copy_neighbors(ctx, sc, qg, dg, off.data(), idx.data(), &total),
+                 "copy_neighbors");
+      it = csr_.emplace(key, std::make_pair(std::move(off), std::move(idx))).first;
+    }
+    const auto &off = it->second.first;
+    const auto &idx = it->second.second;
+    if (i_ + 1 >= off.size())
+      throw std::out_of_range("item index out of range");
+    neighbors[h].assign(idx.begin() + off[i_], idx.begin() + off[i_ + 1]);
+  }
+}
+
+// ---- GPU schemes ---------------------------------------------------------------
+namespace hip {
+namespace {
+
+using Binder = std::function<void(Model &)>;
+
+class HipScheme final : public SchemeBase {
+ public:
+  HipScheme(std::string name, const std::vector<std::pair<const char *, int>> &procs,
+            Binder bind)
+      : name_(std::move(name)), bind_(std::move(bind)) {
+    for (const auto &p : procs) {
+      const int proc = p.second;
+      RegisterProcedureFunction(p.first, [proc](SchemeBase &self, const Neighborhood &nh) {
+        static_cast<HipScheme &>(self).Run(proc, nh);
+      });
+    }
+  }
+  std::string GetFullName() const override { return name_; }
+  void Load(Model &model) override {
+    bind_(model);
+    model_ = &model;
+  }
+  void Run(int proc, const Neighborhood &nh) {
+    if (!model_)
+      throw std::logic_error("scheme is not loaded");
+    if (nh.GetModel() && nh.GetModel() != model_)
+      throw std::logic_error("neighborhood belongs to another model");
+    run(*model_, proc);
+  }
+
+ private:
+  std::string name_;
+  Binder bind_;
+  Model *model_ = nullptr;
+};
+
+template <class Fn>
+void each_of_type(Model &m, std::string_view type, Fn &&fn) {
+  for (Group *g : m.GetGroups())
+    if (g->GetGroupType() == type)
+      fn(*g);
+}
+
+template <class Fn>
+void each_dynamic(Model &m, Fn &&fn) {
+  for (Group *g : m.GetGroups())
+    if (g->HasTag("dynamic"))
+      fn(*g);
+}
+
+std::string name_of(const char *scheme) {
+  return std::string("prtcl::schemes::") + scheme + "[T=f32, N=3, K=Cubic Spline, B=hip]";
+}
+
+}  // namespace
+
+void RegisterSchemes() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    SchemeRegistry &R = GetSchemeRegistry();
+    // field bindings of the reference schemes' Load (iisph.hpp:134-185,
+    // wkbb18.hpp:135-188 and the glue schemes)
+    R.RegisterFactory(name_of("iisph"), [] {
+      return new HipScheme(
+          name_of("iisph"),
+          {{"setup", PRTCL_HIP_P_IISPH_SETUP},
+           {"iteration_pressure_acceleration", PRTCL_HIP_P_IISPH_ITERATION_PRESSURE_ACCELERATION},
+           {"iteration_pressure", PRTCL_HIP_P_IISPH_ITERATION_PRESSURE}},
+          [](Model &m) {
+            m.AddGlobalFieldImpl<float, 3>("gravity");
+            m.AddGlobalFieldImpl<float>("smoothing_scale");
+            m.AddGlobalFieldImpl<float>("time_step");
+            m.AddGlobalFieldImpl<float>("iisph_relaxation");
+            m.AddGlobalFieldImpl<float>("iisph_aprde");
+            m.AddGlobalFieldImpl<int32_t>("iisph_nprde");
+            each_of_type(m, "fluid", [](Group &g) {
+              g.AddUniformFieldImpl<float>("rest_density");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float, 3>("velocity");
+              g.AddVaryingFieldImpl<float, 3>("acceleration");
+              g.AddVaryingFieldImpl<float>("density");
+              g.AddVaryingFieldImpl<float>("pressure");
+              g.AddVaryingFieldImpl<float>("mass");
+              g.AddVaryingFieldImpl<float, 3>("iisph_helper_c");
+              g.AddVaryingFieldImpl<float>("iisph_source_term");
+              g.AddVaryingFieldImpl<float>("iisph_diagonal_element");
+              g.AddVaryingFieldImpl<float>("iisph_right_hand_side");
+            });
+            each_of_type(m, "boundary", [](Group &g) {
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float>("volume");
+            });
+          });
+    });
+    R.RegisterFactory(name_of("wkbb18"), [] {
+      return new HipScheme(
+          name_of("wkbb18"),
+          {{"compute_diagonal", PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL},
+           {"accumulate_acceleration", PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION}},
+          [](Model &m) {
+            m.AddGlobalFieldImpl<float>("smoothing_scale");
+            m.AddGlobalFieldImpl<float>("time_step");
+            m.AddGlobalFieldImpl<float>("current_time");
+            m.AddGlobalFieldImpl<float>("fade_duration");
+            each_of_type(m, "fluid", [](Group &g) {
+              g.AddUniformFieldImpl<float>("rest_density");
+              g.AddUniformFieldImpl<float>("dynamic_viscosity");
+              g.AddUniformFieldImpl<float>("wkbb18_maximum_error");
+              g.AddUniformFieldImpl<int32_t>("wkbb18_maximum_iterations");
+              g.AddUniformFieldImpl<int32_t>("wkbb18_iterations");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float, 3>("velocity");
+              g.AddVaryingFieldImpl<float, 3>("acceleration");
+              g.AddVaryingFieldImpl<float>("density");
+              g.AddVaryingFieldImpl<float>("mass");
+              g.AddVaryingFieldImpl<float, 3>("wkbb18_velocity_delta");
+              g.AddVaryingFieldImpl<float, 3, 3>("wkbb18_system_diagonal");
+              g.AddVaryingFieldImpl<float>("time_of_birth");
+            });
+            each_of_type(m, "boundary", [](Group &g) {
+              g.AddUniformFieldImpl<float>("dynamic_viscosity");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float>("volume");
+            });
+          });
+    });
+    R.RegisterFactory(name_of("density"), [] {
+      return new HipScheme(name_of("density"),
+                           {{"compute_density", PRTCL_HIP_P_DENSITY_COMPUTE_DENSITY}},
+                           [](Model &m) {
+                             m.AddGlobalFieldImpl<float>("smoothing_scale");
+                             each_of_type(m, "fluid", [](Group &g) {
+                               g.AddUniformFieldImpl<float>("rest_density");
+                               g.AddVaryingFieldImpl<float, 3>("position");
+                               g.AddVaryingFieldImpl<float>("density");
+                               g.AddVaryingFieldImpl<float>("mass");
+                             });
+                             each_of_type(m, "boundary", [](Group &g) {
+                               g.AddVaryingFieldImpl<float, 3>("position");
+                               g.AddVaryingFieldImpl<float>("volume");
+                             });
+                           });
+    });
+    R.RegisterFactory(name_of("gravity"), [] {
+      return new HipScheme(name_of("gravity"),
+                           {{"initialize_acceleration",
+                             PRTCL_HIP_P_GRAVITY_INITIALIZE_ACCELERATION}},
+                           [](Model &m) {
+                             m.AddGlobalFieldImpl<float, 3>("gravity");
+                             each_dynamic(m, [](Group &g) {
+                               g.AddVaryingFieldImpl<float, 3>("position");
+                               g.AddVaryingFieldImpl<float, 3>("acceleration");
+                             });
+                           });
+    });
+    R.RegisterFactory(name_of("viscosity"), [] {
+      return new HipScheme(name_of("viscosity"),
+                           {{"accumulate_acceleration",
+                             PRTCL_HIP_P_VISCOSITY_ACCUMULATE_ACCELERATION}},
+                           [](Model &m) {
+                             m.AddGlobalFieldImpl<float>("smoothing_scale");
+                             each_of_type(m, "fluid", [](Group &g) {
+                               g.AddUniformFieldImpl<float>("rest_density");
+                               g.AddUniformFieldImpl<float>("dynamic_viscosity");
+                               g.AddVaryingFieldImpl<float, 3>("position");
+                               g.AddVaryingFieldImpl<float, 3>("velocity");
+                               g.AddVaryingFieldImpl<float, 3>("acceleration");
+                               g.AddVaryingFieldImpl<float>("density");
+                               g.AddVaryingFieldImpl<float>("mass");
+                             });
+                             each_of_type(m, "boundary", [](Group &g) {
+                               g.AddUniformFieldImpl<float>("dynamic_viscosity");
+                               g.AddVaryingFieldImpl<float, 3>("position");
+                               g.AddVaryingFieldImpl<float>("volume");
+                             });
+                           });
+    });
+    R.RegisterFactory(name_of("symplectic_euler"), [] {
+      return new HipScheme(
+          name_of("symplectic_euler"),
+          {{"integrate_velocity_with_hard_fade", PRTCL_HIP_P_SE_INTEGRATE_VELOCITY_WITH_HARD_FADE},
+           {"integrate_position", PRTCL_HIP_P_SE_INTEGRATE_POSITION}},
+          [](Model &m) {
+            m.AddGlobalFieldImpl<float>("time_step");
+            m.AddGlobalFieldImpl<float>("current_time");
+            m.AddGlobalFieldImpl<float>("fade_duration");
+            m.AddGlobalFieldImpl<float>("maximum_speed");
+            each_dynamic(m, [](Group &g) {
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float, 3>("velocity");
+              g.AddVaryingFieldImpl<float, 3>("acceleration");
+              g.AddVaryingFieldImpl<float>("time_of_birth");
+            });
+          });
+    });
+    R.RegisterFactory(name_of("aiast12"), [] {
+      return new HipScheme(name_of("aiast12"),
+                           {{"compute_volume", PRTCL_HIP_P_AIAST12_COMPUTE_VOLUME}},
+                           [](Model &m) {
+                             m.AddGlobalFieldImpl<float>("smoothing_scale");
+                             each_of_type(m, "boundary", [](Group &g) {
+                               g.AddVaryingFieldImpl<float, 3>("position");
+                               g.AddVaryingFieldImpl<float>("volume");
+                             });
+                           });
+    });
+  });
+}
+
+}  // namespace hip
+}  // namespace prtcl

@@ -1,0 +1,341 @@
+// Checks of the C++ host interface (prtcl_amd/compat). Built and run by
+// tests/test_compat.py.
+//   compat_check cpu             -- model / group / registry semantics, no GPU
+//   compat_check gpu FILE STEPS  -- the example.lua step loop through the GPU
+//                                   schemes vs prtcl_hip_step on the same scene
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <functional>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../prtcl_amd/compat/prtcl_compat.hpp"
+
+using namespace prtcl;
+
+static int failures = 0;
+#define CHECK(cond)                                                   \
+  do {                                                                \
+    if (!(cond)) {                                                    \
+      std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);     \
+      ++failures;                                                     \
+    }                                                                 \
+  } while (0)
+
+template <class E, class Fn>
+static bool throws(Fn &&fn) {
+  try {
+    fn();
+  } catch (const E &) {
+    return true;
+  } catch (...) {
+    return false;
+  }
+  return false;
+}
+
+static std::string thrown_literal(std::function<void()> fn) {
+  try {
+    fn();
+  } catch (const char *s) {
+    return s;
+  } catch (...) {
+    return "other";
+  }
+  return "none";
+}
+
+static int cpu_checks() {
+  Model model;
+  Group &f = model.AddGroup("f", "fluid");
+  Group &b = model.AddGroup("b", "boundary");
+  // group index = rank of the name (model.cpp:12-33): b = 0, f = 1
+  CHECK(b.GetGroupIndex() == static_cast<GroupIndex>(0));
+  CHECK(f.GetGroupIndex() == static_cast<GroupIndex>(1));
+  CHECK(throws<GroupOfDifferentTypeAlreadyExists>([&] { model.AddGroup("f", "boundary"); }));
+  CHECK(&model.AddGroup("f", "fluid") == &f);
+
+  // known-answer test of group.test.cpp:102-121 (gather semantics)
+  Group &g = model.AddGroup("g", "other");
+  g.Resize(5);
+  auto vb = g.AddVaryingFieldImpl<float, 3, 2>("b");
+  CHECK(vb.GetSize() == 5);
+  CHECK(vb[3][5] == 0.0f);  // zero-filled
+  for (int i = 0; i < 5; ++i)
+    vb[i][0] = static_cast<float>(i + 1);
+  g.SetDirty(false);
+  const size_t perm[5] = {3, 4, 1, 2, 0};
+  g.Permute(perm, 5);
+  const float expect[5] = {4, 5, 2, 3, 1};
+  for (int i = 0; i < 5; ++i)
+    CHECK(vb[i][0] == expect[i]);
+  CHECK(g.IsDirty());
+  CHECK(throws<FieldOfDifferentTypeAlreadyExistsError>([&] { g.AddVaryingFieldImpl<double, 3>("b"); }));
+  CHECK(throws<FieldOfDifferentKindAlreadyExistsError>([&] { g.AddUniformFieldImpl<float>("b"); }));
+  g.Resize(7);
+  CHECK(vb.GetSize() == 7);
+  CHECK(vb[6][0] == 0.0f);
+
+  // registry and procedures
+  hip::RegisterSchemes();
+  const std::string iisph = "prtcl::schemes::iisph[T=f32, N=3, K=Cubic Spline, B=hip]";
+  auto names = GetSchemeRegistry().GetSchemeNames();
+  CHECK(names.size() >= 7);
+  auto s = GetSchemeRegistry().NewScheme(iisph);
+  CHECK(s->GetFullName() == iisph);
+  auto procs = s->GetProcedureNames();
+  CHECK(procs.size() == 3);
+  CHECK(procs[0] == "iteration_pressure" && procs[1] == "iteration_pressure_acceleration" &&
+        procs[2] == "setup");
+  CHECK(thrown_literal([] { GetSchemeRegistry().NewScheme("prtcl::schemes::none"); }) ==
+        "scheme is not registered");
+  Neighborhood nh;
+  CHECK(thrown_literal([&] { s->RunProcedure("no_such_procedure", nh); }) ==
+        "procedure does not exist");
+  // Load binds the reference scheme's fields (iisph.hpp:134-185)
+  s->Load(model);
+  CHECK(f.GetVarying().HasField("iisph_helper_c"));
+  CHECK(f.GetVarying().HasField("iisph_right_hand_side"));
+  CHECK(f.GetUniform().HasField("rest_density"));
+  CHECK(b.GetVarying().HasField("volume"));
+  CHECK(model.GetGlobal().HasField("iisph_nprde"));
+  CHECK(!g.GetVarying().HasField("pressure"));
+  auto x = f.GetVarying().FieldSpan<float, 3>("position");
+  CHECK(x.GetSize() == 0);
+  std::printf(failures ? "FAILED\n" : "OK\n");
+  return failures ? 1 : 0;
+}
+
+struct SceneFile {
+  int64_t nf = 0, nb = 0;
+  std::vector<float> fx, fv, fm, ftb, bx;
+  double h, dt, rho0, omega, g[3], tol, max_it, fade;
+};
+
+static SceneFile read_scene(const char *path) {
+  SceneFile s;
+  std::ifstream in(path, std::ios::binary);
+  in.read(reinterpret_cast<char *>(&s.nf), 8);
+  in.read(reinterpret_cast<char *>(&s.nb), 8);
+  auto rd = [&](std::vector<float> &v, size_t n) {
+    v.resize(n);
+    in.read(reinterpret_cast<char *>(v.data()), static_cast<std::streamsize>(4 * n));
+  };
+  rd(s.fx, 3 * s.nf);
+  rd(s.fv, 3 * s.nf);
+  rd(s.fm, s.nf);
+  rd(s.ftb, s.nf);
+  rd(s.bx, 3 * s.nb);
+  double p[10];
+  in.read(reinterpret_cast<char *>(p), sizeof(p));
+  s.h = p[0];
+  s.dt = p[1];
+  s.rho0 = p[2];
+  s.omega = p[3];
+  s.g[0] = p[4];
+  s.g[1] = p[5];
+  s.g[2] = p[6];
+  s.tol = p[7];
+  s.max_it = p[8];
+  s.fade = p[9];
+  if (!in)
+    throw std::runtime_error("short scene file");
+  return s;
+}
+
+static int gpu_check(const char *path, int steps) {
+  const SceneFile sf = read_scene(path);
+  hip::RegisterSchemes();
+  Model model;
+  Group &f = model.AddGroup("f", "fluid");
+  f.AddTag("dynamic");
+  Group &b = model.AddGroup("b", "boundary");
+  auto &R = GetSchemeRegistry();
+  auto name = [](const char *s) {
+    return std::string("prtcl::schemes::") + s + "[T=f32, N=3, K=Cubic Spline, B=hip]";
+  };
+  auto iisph = R.NewScheme(name("iisph"));
+  auto density = R.NewScheme(name("density"));
+  auto gravity = R.NewScheme(name("gravity"));
+  auto se = R.NewScheme(name("symplectic_euler"));
+  auto aiast12 = R.NewScheme(name("aiast12"));
+  f.Resize(static_cast<size_t>(sf.nf));
+  b.Resize(static_cast<size_t>(sf.nb));
+  for (auto *s : {iisph.get(), density.get(), gravity.get(), se.get(), aiast12.get()})
+    s->Load(model);
+  *model.AddGlobalFieldImpl<float>("smoothing_scale") = static_cast<float>(sf.h);
+  *model.AddGlobalFieldImpl<float>("time_step") = static_cast<float>(sf.dt);
+  *model.AddGlobalFieldImpl<float>("iisph_relaxation") = static_cast<float>(sf.omega);
+  *model.AddGlobalFieldImpl<float>("fade_duration") = static_cast<float>(sf.fade);
+  auto grav = model.AddGlobalFieldImpl<float, 3>("gravity");
+  for (int k = 0; k < 3; ++k)
+    grav[k] = static_cast<float>(sf.g[k]);
+  auto t_now = model.AddGlobalFieldImpl<float>("current_time");
+  *f.AddUniformFieldImpl<float>("rest_density") = static_cast<float>(sf.rho0);
+  std::memcpy(f.GetVarying().FieldSpan<float, 3>("position").data(), sf.fx.data(), 4 * sf.fx.size());
+  std::memcpy(f.GetVarying().FieldSpan<float, 3>("velocity").data(), sf.fv.data(), 4 * sf.fv.size());
+  std::memcpy(f.GetVarying().FieldSpan<float>("mass").data(), sf.fm.data(), 4 * sf.fm.size());
+  std::memcpy(f.GetVarying().FieldSpan<float>("time_of_birth").data(), sf.ftb.data(),
+              4 * sf.ftb.size());
+  std::memcpy(b.GetVarying().FieldSpan<float, 3>("position").data(), sf.bx.data(), 4 * sf.bx.size());
+
+  // example.lua:342-344 then the step loop of example.lua:363-462
+  Neighborhood nh;
+  nh.SetRadius(2.0 * static_cast<double>(static_cast<float>(sf.h)));
+  nh.Load(model);
+  nh.Update();
+  aiast12->RunProcedure("compute_volume", nh);
+  double t = 0;
+  auto aprde = model.AddGlobalFieldImpl<float>("iisph_aprde");
+  auto nprde = model.AddGlobalFieldImpl<int32_t>("iisph_nprde");
+  std::vector<int> its;
+  for (int step = 0; step < steps; ++step) {
+    *t_now = static_cast<float>(t);
+    nh.Update();
+    density->RunProcedure("compute_density", nh);
+    gravity->RunProcedure("initialize_acceleration", nh);
+    se->RunProcedure("integrate_velocity_with_hard_fade", nh);
+    iisph->RunProcedure("setup", nh);
+    int it = 0;
+    for (;;) {
+      *aprde = 0.0f;  // aprde:set(0), example.lua:414
+      iisph->RunProcedure("iteration_pressure_acceleration", nh);
+      iisph->RunProcedure("iteration_pressure", nh);
+      ++it;
+      const double rel = static_cast<double>(*aprde) / static_cast<double>(*nprde);
+      if ((it >= 2 && rel < sf.tol) || it >= static_cast<int>(sf.max_it))
+        break;
+    }
+    its.push_back(it);
+    se->RunProcedure("integrate_velocity_with_hard_fade", nh);
+    se->RunProcedure("integrate_position", nh);
+    t += static_cast<double>(static_cast<float>(sf.dt));
+  }
+
+  // the same scene through prtcl_hip_step (per-procedure kernels, no permute)
+  prtcl_hip_ctx *ctx = hip::Context();
+  prtcl_hip_scene_desc d;
+  prtcl_hip_scene_desc_init(&d);
+  d.n_fluid = sf.nf;
+  d.n_boundary = sf.nb;
+  d.smoothing_scale = static_cast<float>(sf.h);
+  d.time_step = static_cast<float>(sf.dt);
+  d.rest_density = static_cast<float>(sf.rho0);
+  d.iisph_relaxation = static_cast<float>(sf.omega);
+  d.fade_duration = static_cast<float>(sf.fade);
+  for (int k = 0; k < 3; ++k)
+    d.gravity[k] = static_cast<float>(sf.g[k]);
+  d.iisph_tolerance = sf.tol;
+  d.iisph_max_iterations = static_cast<int32_t>(sf.max_it);
+  d.permute_every = 0;
+  d.step_mode = 1;
+  int32_t id = -1;
+  if (prtcl_hip_scene_create(ctx, &d, &id) != PRTCL_HIP_OK) {
+    std::printf("FAIL scene_create: %s\n", prtcl_hip_last_error(ctx));
+    return 1;
+  }
+  prtcl_hip_upload_field(ctx, id, PRTCL_HIP_FLUID, PRTCL_HIP_F_POSITION, sf.fx.data(), sf.fx.size());
+  prtcl_hip_upload_field(ctx, id, PRTCL_HIP_FLUID, PRTCL_HIP_F_VELOCITY, sf.fv.data(), sf.fv.size());
+  prtcl_hip_upload_field(ctx, id, PRTCL_HIP_FLUID, PRTCL_HIP_F_MASS, sf.fm.data(), sf.fm.size());
+  prtcl_hip_upload_field(ctx, id, PRTCL_HIP_FLUID, PRTCL_HIP_F_TIME_OF_BIRTH, sf.ftb.data(),
+                         sf.ftb.size());
+  prtcl_hip_upload_field(ctx, id, PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_POSITION, sf.bx.data(),
+                         sf.bx.size());
+  prtcl_hip_run(ctx, id, PRTCL_HIP_P_NSEARCH_UPDATE);
+  prtcl_hip_run(ctx, id, PRTCL_HIP_P_AIAST12_COMPUTE_VOLUME);
+  std::vector<int> its2;
+  for (int step = 0; step < steps; ++step) {
+    const int rc = prtcl_hip_step(ctx, &id, 1, 1);
+    if (rc != PRTCL_HIP_OK && rc != PRTCL_HIP_NOT_CONVERGED) {
+      std::printf("FAIL step: %s\n", prtcl_hip_last_error(ctx));
+      return 1;
+    }
+    int32_t it = 0;
+    prtcl_hip_last_step_info(ctx, id, &it, nullptr, nullptr);
+    its2.push_back(it);
+  }
+  if (its != its2) {
+    std::printf("iterations through the schemes:");
+    for (int v : its)
+      std::printf(" %d", v);
+    std::printf("\niterations through prtcl_hip_step:");
+    for (int v : its2)
+      std::printf(" %d", v);
+    std::printf("\n");
+  }
+  CHECK(its == its2);
+  struct Cmp {
+    const char *name;
+    int field;
+    size_t extent;
+  };
+  for (const Cmp &c : {Cmp{"position", PRTCL_HIP_F_POSITION, 3}, Cmp{"velocity", PRTCL_HIP_F_VELOCITY, 3},
+                       Cmp{"density", PRTCL_HIP_F_DENSITY, 1}, Cmp{"pressure", PRTCL_HIP_F_PRESSURE, 1}}) {
+    std::vector<float> ref(static_cast<size_t>(sf.nf) * c.extent);
+    prtcl_hip_download_field(ctx, id, PRTCL_HIP_FLUID, c.field, ref.data(), ref.size());
+    const float *host = nullptr;
+    if (c.extent == 3)
+      host = f.GetVarying().FieldSpan<float, 3>(c.name).data();
+    else
+      host = f.GetVarying().FieldSpan<float>(c.name).data();
+    const bool same = std::memcmp(ref.data(), host, 4 * ref.size()) == 0;
+    if (!same) {
+      double md = 0, mx = 0;
+      for (size_t k = 0; k < ref.size(); ++k) {
+        md = std::max(md, std::fabs(static_cast<double>(ref[k]) - host[k]));
+        mx = std::max(mx, std::fabs(static_cast<double>(ref[k])));
+      }
+      std::printf("field %s differs: max |diff| %.3e (max |value| %.3e)\n", c.name, md, mx);
+    }
+    CHECK(same);
+  }
+  prtcl_hip_scene_destroy(ctx, id);
+
+  // Neighborhood::Permute: host fields follow the device's new order
+  std::vector<float> before(f.GetVarying().FieldSpan<float, 3>("position").data(),
+                            f.GetVarying().FieldSpan<float, 3>("position").data() + 3 * sf.nf);
+  nh.Permute(model);
+  auto xp = f.GetVarying().FieldSpan<float, 3>("position");
+  std::vector<std::vector<size_t>> nbrs;
+  nh.Update();
+  nh.CopyNeighbors(1, 0, nbrs);
+  CHECK(nbrs.size() == 2);
+  bool self = false;
+  for (size_t j : nbrs[1])
+    self |= j == 0;
+  CHECK(self);  // the neighbour set includes the particle itself
+  // same multiset of positions after the permutation
+  std::vector<float> after(xp.data(), xp.data() + 3 * sf.nf);
+  auto key = [](std::vector<float> v) {
+    std::vector<std::array<float, 3>> a(v.size() / 3);
+    for (size_t i = 0; i < a.size(); ++i)
+      a[i] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    std::sort(a.begin(), a.end());
+    return a;
+  };
+  CHECK(key(before) == key(after));
+  std::printf(failures ? "FAILED\n" : "OK\n");
+  return failures ? 1 : 0;
+}
+
+int main(int argc, char **argv) {
+  try {
+    if (argc >= 2 && std::string(argv[1]) == "cpu")
+      return cpu_checks();
+    if (argc >= 4 && std::string(argv[1]) == "gpu")
+      return gpu_check(argv[2], std::atoi(argv[3]));
+  } catch (const std::exception &e) {
+    std::printf("FAIL exception: %s\n", e.what());
+    return 1;
+  } catch (const char *s) {
+    std::printf("FAIL exception: %s\n", s);
+    return 1;
+  }
+  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS\n");
+  return 2;
+}
