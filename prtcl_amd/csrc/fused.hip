@@ -430,8 +430,19 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
     grid_walk(S, i, xi, [&](uint32_t j) { pair(S.xm[j], j); });
     info = kListOverflow;
   } else {
-    for (uint32_t e = 0; e < cnt; ++e)
-      L.idx[e * nn + i] = lst[e][t];  // row e, written by all lanes together
+    // chunks of 8 entries, one 16-byte store per lane (1 KB per wavefront)
+    for (uint32_t g = 0; g * kChunk < cnt; ++g) {
+      uint32_t w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t e0 = g * kChunk + 2 * k;
+        const uint32_t lo = e0 < cnt ? lst[e0][t] : 0u;
+        const uint32_t hi = e0 + 1 < cnt ? lst[e0 + 1][t] : 0u;
+        w[k] = lo | (hi << 16);
+      }
+      reinterpret_cast<uint4 *>(L.idx)[static_cast<size_t>(g) * nn + static_cast<size_t>(i)] =
+          make_uint4(w[0], w[1], w[2], w[3]);
+    }
     info = cs[0] | (cs[1] << 8) | (cs[2] << 16);
   }
   L.info[i] = info;
@@ -476,6 +487,7 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
   S.p[i] = T(0);
   S.pr[i] = T(0);
   S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, T(0));
+  S.xa[2 * static_cast<size_t>(i)] = xi;
   if (kWithSource) {
     const T src = (rho0 - rho) -
                   S.dt * rho0 * irho * (dsum + rho0 * (vs0 * gb0 + vs1 * gb1 + vs2 * gb2));
@@ -567,16 +579,14 @@ __global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList 
 // Walk of the list entries of particle i (all threads of the block call it).
 // kStage and spans that fit LDS: per slab, stage(s) fills LDS (all threads),
 // then use(lds(u)) for the neighbour at LDS index u. Otherwise
-// use(load(j)) over the whole list with global slots, kUnroll gathers in
-// flight.
-template <bool kStage, uint32_t kCap, int kUnroll, class Stage, class Lds, class Load,
-          class Use>
+// use(load(j)) over the whole list with global slots, a chunk of 8 gathers
+// in flight.
+template <bool kStage, uint32_t kCap, class Stage, class Lds, class Load, class Use>
 __device__ __forceinline__ void list_walk(const NbrList &L, const Spans &R, size_t n, int i,
                                           uint32_t info, Stage &&stage, Lds &&lds, Load &&load,
                                           Use &&use) {
   const uint32_t live = (info & kListOverflow) ? 0u : info;
   if (kStage && R.max_len() <= kCap) {
-    const uint16_t *__restrict__ idx = L.idx + i;
     uint32_t e = 0;
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
@@ -585,20 +595,20 @@ __device__ __forceinline__ void list_walk(const NbrList &L, const Spans &R, size
       __syncthreads();
       uint32_t q = 0;
       for (; q + 4 <= c; q += 4) {
-        const uint32_t u0 = idx[(e + q) * n], u1 = idx[(e + q + 1) * n];
-        const uint32_t u2 = idx[(e + q + 2) * n], u3 = idx[(e + q + 3) * n];
+        const uint32_t u0 = list_entry(L, n, i, e + q), u1 = list_entry(L, n, i, e + q + 1);
+        const uint32_t u2 = list_entry(L, n, i, e + q + 2), u3 = list_entry(L, n, i, e + q + 3);
         use(lds(u0));
         use(lds(u1));
         use(lds(u2));
         use(lds(u3));
       }
       for (; q < c; ++q)
-        use(lds(idx[(e + q) * n]));
+        use(lds(list_entry(L, n, i, e + q)));
       __syncthreads();
       e += c;
     }
   } else {
-    list_gather<kUnroll>(L, R, n, i, live, load, use);
+    list_gather(L, R, n, i, live, load, use);
   }
 }
 
@@ -628,15 +638,17 @@ __device__ __forceinline__ void k_accel_list_body(const SceneView<T> &S, const N
     a1 += s * d1;
     a2 += s * d2;
   };
-  list_walk<kStage, kCap, 8>(
+  list_walk<kStage, kCap>(
       L, R, static_cast<size_t>(S.nf), i, info, [&](int s) { stage_span(R, s, S.xq, sq); },
       [&](uint32_t u) { return sq[u]; }, [&](uint32_t j) { return S.xq[j]; }, pair);
   if (!valid)
     return;
   if (info & kListOverflow)
     grid_walk(S, i, xi, [&](uint32_t j) { pair(S.xq[j]); });
-  S.a[i] = make4<vec4_t<T>>(pi * ci.x - K.ag3 * a0, pi * ci.y - K.ag3 * a1,
-                            pi * ci.z - K.ag3 * a2, T(0));
+  const vec4_t<T> ai = make4<vec4_t<T>>(pi * ci.x - K.ag3 * a0, pi * ci.y - K.ag3 * a1,
+                                        pi * ci.z - K.ag3 * a2, T(0));
+  S.a[i] = ai;
+  S.xa[2 * static_cast<size_t>(i) + 1] = ai;
 }
 
 template <class T, bool kStage>
@@ -652,8 +664,11 @@ __device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, cons
   if (S.ctl->done)
     return;
   constexpr uint32_t kCap = slab_cap<T>();
-  __shared__ vec4_t<T> sx[kStage ? kCap : 1];
-  __shared__ vec4_t<T> sa[kStage ? kCap : 1];
+  struct XA {
+    vec4_t<T> x, a;  // (x, m), (a, 0): one 32-byte record of xa
+  };
+  __shared__ XA sxa[kStage ? kCap / 2 : 1];
+  const XA *__restrict__ xa = reinterpret_cast<const XA *>(S.xa);
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   const bool valid = i < S.nf;
   const Spans R = Spans::of(L.blk[lb]);
@@ -666,26 +681,18 @@ __device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, cons
   const T rhoi = valid ? S.rho[i] : T(1), AA = valid ? S.AA[i] : T(0);
   const T si = valid ? S.s[i] : T(0), p_old = valid ? S.p[i] : T(0);
   T sum = 0;
-  struct XA {
-    vec4_t<T> x, a;
-  };
   auto pair = [&](const XA &q) {
     const T d0 = xi.x - q.x.x, d1 = xi.y - q.x.y, d2 = xi.z - q.x.z;
     sum += q.x.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2) *
            (q.a.x * d0 + q.a.y * d1 + q.a.z * d2);
   };
-  list_walk<kStage, kCap, 4>(
-      L, R, static_cast<size_t>(S.nf), i, info,
-      [&](int s) {
-        stage_span(R, s, S.xm, sx);
-        stage_span(R, s, S.a, sa);
-      },
-      [&](uint32_t u) { return XA{sx[u], sa[u]}; }, [&](uint32_t j) { return XA{S.xm[j], S.a[j]}; },
-      pair);
+  list_walk<kStage, kCap / 2>(
+      L, R, static_cast<size_t>(S.nf), i, info, [&](int s) { stage_span(R, s, xa, sxa); },
+      [&](uint32_t u) { return sxa[u]; }, [&](uint32_t j) { return xa[j]; }, pair);
   double term = 0.0;
   if (valid) {
     if (info & kListOverflow)
-      grid_walk(S, i, xi, [&](uint32_t j) { pair(XA{S.xm[j], S.a[j]}); });
+      grid_walk(S, i, xi, [&](uint32_t j) { pair(xa[j]); });
     const T Ap = S.dt * S.dt * S.rho0 / rhoi *
                  ((ai.x * gs.x + ai.y * gs.y + ai.z * gs.z) - K.ag3 * sum);
     const T rcp = (fabs(AA) > T(1e-9)) ? T(1) / AA : T(0);

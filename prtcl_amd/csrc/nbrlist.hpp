@@ -11,8 +11,10 @@
 // span base + entry, and a pass that stages a span in LDS reads the
 // neighbour at LDS index = entry.
 //
-// Layout: entry e of particle i at idx[e * n + i] (a wavefront reads or writes
-// 64 consecutive entries), entries ordered by slab; info[i] holds the entry
+// Layout: chunks of 8 entries; entry e of particle i at
+// idx[((e / 8) * n + i) * 8 + e % 8], so one 16-byte load per lane fetches 8
+// entries and a wavefront's load is 1 KB contiguous. Entries are ordered by
+// slab; info[i] holds the entry
 // count of each slab (8 bits each) and kListOverflow when the particle has
 // more than kNbrCap neighbours or a span of its block exceeds 16 bits (its
 // passes then walk the grid with the exact predicate instead).
@@ -70,56 +72,76 @@ __device__ __forceinline__ void stage_span(const Spans &R, int s, const V *__res
     dst[u] = p[u];
 }
 
+constexpr uint32_t kChunk = 8;  // list entries per 16-byte chunk
+
+// the 8 entries of chunk g of particle i
+__device__ __forceinline__ uint4 list_chunk(const NbrList &L, size_t n, int i, uint32_t g) {
+  return reinterpret_cast<const uint4 *>(L.idx)[static_cast<size_t>(g) * n +
+                                                static_cast<size_t>(i)];
+}
+
+// entry k (0..7) of a chunk
+__device__ __forceinline__ uint32_t chunk_entry(const uint4 &q, int k) {
+  const uint32_t w = k < 2 ? q.x : (k < 4 ? q.y : (k < 6 ? q.z : q.w));
+  return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+// entry e of particle i (single load)
+__device__ __forceinline__ uint32_t list_entry(const NbrList &L, size_t n, int i, uint32_t e) {
+  return L.idx[((static_cast<size_t>(e) / kChunk) * n + static_cast<size_t>(i)) * kChunk +
+               e % kChunk];
+}
+
 // Visits the global slots of the list entries of particle i (not
-// overflowed) in list order: fn(j). R is the span table of i's block.
+// overflowed) in list order: fn(j). R is the span table of i's block. The
+// slot is b0 + entry + [e >= c0] (b1 - b0) + [e >= c01] (b2 - b1) (two selects
+// against register values).
 template <class Fn>
 __device__ __forceinline__ void list_visit(const NbrList &L, const Spans &R, size_t n, int i,
                                            uint32_t info, Fn &&fn) {
-  const uint16_t *__restrict__ idx = L.idx + i;
   const uint32_t c0 = info & 0xFFu;
   const uint32_t c01 = c0 + ((info >> 8) & 0xFFu);
   const uint32_t c = c01 + ((info >> 16) & 0xFFu);
   const uint32_t d1 = R.b1 - R.b0, d2 = R.b2 - R.b1;
-  for (uint32_t e = 0; e < c; ++e)
-    fn(R.b0 + idx[e * n] + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u));
+  for (uint32_t g = 0; g * kChunk < c; ++g) {
+    const uint4 q = list_chunk(L, n, i, g);
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(kChunk); ++k) {
+      const uint32_t e = g * kChunk + k;
+      if (e < c)
+        fn(R.b0 + chunk_entry(q, k) + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u));
+    }
+  }
 }
 
-// The same walk in groups of kUnroll entries, with the memory traffic of a
-// group issued before its arithmetic: the indices first, then every gather
-// (v[k] = load(j_k)), then use(v[k]) in order. The slot is
-// b0 + entry + [e >= c0] (b1 - b0) + [e >= c01] (b2 - b1) (two selects
-// against register values).
-template <int kUnroll, class Load, class Use>
+// The same walk one chunk (8 entries) at a time with the memory traffic of
+// a chunk issued before its arithmetic: the chunk, then every gather
+// (v[k] = load(j_k)), then use(v[k]) in order.
+template <class Load, class Use>
 __device__ __forceinline__ void list_gather(const NbrList &L, const Spans &R, size_t n, int i,
                                             uint32_t info, Load &&load, Use &&use) {
-  const uint16_t *__restrict__ idx = L.idx + i;
   const uint32_t c0 = info & 0xFFu;
   const uint32_t c01 = c0 + ((info >> 8) & 0xFFu);
   const uint32_t c = c01 + ((info >> 16) & 0xFFu);
   const uint32_t d1 = R.b1 - R.b0, d2 = R.b2 - R.b1;
-  auto slot = [&](uint32_t e, uint32_t u) {
-    return R.b0 + u + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u);
-  };
   using V = decltype(load(0u));
-  uint32_t e = 0;
-  for (; e + kUnroll <= c; e += kUnroll) {
-    uint32_t u[kUnroll];
+  for (uint32_t g = 0; g * kChunk < c; ++g) {
+    const uint4 q = list_chunk(L, n, i, g);
+    V v[kChunk];
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k)
-      u[k] = idx[(e + k) * n];
-    V v[kUnroll];
-#pragma unroll
-    for (int k = 0; k < kUnroll; ++k)
-      v[k] = load(slot(e + k, u[k]));
-    // keep every gather of the group ahead of the arithmetic (the scheduler
+    for (int k = 0; k < static_cast<int>(kChunk); ++k) {
+      const uint32_t e = g * kChunk + k;
+      if (e < c)
+        v[k] = load(R.b0 + chunk_entry(q, k) + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u));
+    }
+    // keep every gather of the chunk ahead of the arithmetic (the scheduler
     // would otherwise consume each gather before issuing the next one)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k)
-      use(v[k]);
+    for (int k = 0; k < static_cast<int>(kChunk); ++k)
+      if (g * kChunk + k < c)
+        use(v[k]);
   }
-  for (; e < c; ++e)
-    use(load(slot(e, idx[e * n])));
 }
 
 }  // namespace prtcl_amd
