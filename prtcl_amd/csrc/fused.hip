@@ -487,7 +487,6 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
   S.p[i] = T(0);
   S.pr[i] = T(0);
   S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, T(0));
-  S.xa[2 * static_cast<size_t>(i)] = xi;
   if (kWithSource) {
     const T src = (rho0 - rho) -
                   S.dt * rho0 * irho * (dsum + rho0 * (vs0 * gb0 + vs1 * gb1 + vs2 * gb2));
@@ -581,7 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList 
 // then use(lds(u)) for the neighbour at LDS index u. Otherwise
 // use(load(j)) over the whole list with global slots, a chunk of 8 gathers
 // in flight.
-template <bool kStage, uint32_t kCap, class Stage, class Lds, class Load, class Use>
+template <bool kStage, uint32_t kCap, int kGroup = 8, class Stage, class Lds, class Load,
+          class Use>
 __device__ __forceinline__ void list_walk(const NbrList &L, const Spans &R, size_t n, int i,
                                           uint32_t info, Stage &&stage, Lds &&lds, Load &&load,
                                           Use &&use) {
@@ -608,7 +608,7 @@ __device__ __forceinline__ void list_walk(const NbrList &L, const Spans &R, size
       e += c;
     }
   } else {
-    list_gather(L, R, n, i, live, load, use);
+    list_gather<kGroup>(L, R, n, i, live, load, use);
   }
 }
 
@@ -648,7 +648,6 @@ __device__ __forceinline__ void k_accel_list_body(const SceneView<T> &S, const N
   const vec4_t<T> ai = make4<vec4_t<T>>(pi * ci.x - K.ag3 * a0, pi * ci.y - K.ag3 * a1,
                                         pi * ci.z - K.ag3 * a2, T(0));
   S.a[i] = ai;
-  S.xa[2 * static_cast<size_t>(i) + 1] = ai;
 }
 
 template <class T, bool kStage>
@@ -665,10 +664,10 @@ __device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, cons
     return;
   constexpr uint32_t kCap = slab_cap<T>();
   struct XA {
-    vec4_t<T> x, a;  // (x, m), (a, 0): one 32-byte record of xa
+    vec4_t<T> x, a;  // (x, m) and (a, 0) of a neighbour
   };
-  __shared__ XA sxa[kStage ? kCap / 2 : 1];
-  const XA *__restrict__ xa = reinterpret_cast<const XA *>(S.xa);
+  __shared__ vec4_t<T> sx[kStage ? kCap : 1];
+  __shared__ vec4_t<T> sa[kStage ? kCap : 1];
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   const bool valid = i < S.nf;
   const Spans R = Spans::of(L.blk[lb]);
@@ -686,13 +685,18 @@ __device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, cons
     sum += q.x.w * K.gradf_unscaled(d0 * d0 + d1 * d1 + d2 * d2) *
            (q.a.x * d0 + q.a.y * d1 + q.a.z * d2);
   };
-  list_walk<kStage, kCap / 2>(
-      L, R, static_cast<size_t>(S.nf), i, info, [&](int s) { stage_span(R, s, xa, sxa); },
-      [&](uint32_t u) { return sxa[u]; }, [&](uint32_t j) { return xa[j]; }, pair);
+  list_walk<kStage, kCap, 4>(
+      L, R, static_cast<size_t>(S.nf), i, info,
+      [&](int s) {
+        stage_span(R, s, S.xm, sx);
+        stage_span(R, s, S.a, sa);
+      },
+      [&](uint32_t u) { return XA{sx[u], sa[u]}; }, [&](uint32_t j) { return XA{S.xm[j], S.a[j]}; },
+      pair);
   double term = 0.0;
   if (valid) {
     if (info & kListOverflow)
-      grid_walk(S, i, xi, [&](uint32_t j) { pair(xa[j]); });
+      grid_walk(S, i, xi, [&](uint32_t j) { pair(XA{S.xm[j], S.a[j]}); });
     const T Ap = S.dt * S.dt * S.rho0 / rhoi *
                  ((ai.x * gs.x + ai.y * gs.y + ai.z * gs.z) - K.ag3 * sum);
     const T rcp = (fabs(AA) > T(1e-9)) ? T(1) / AA : T(0);

@@ -115,32 +115,43 @@ __device__ __forceinline__ void list_visit(const NbrList &L, const Spans &R, siz
 }
 
 // The same walk one chunk (8 entries) at a time with the memory traffic of
-// a chunk issued before its arithmetic: the chunk, then every gather
-// (v[k] = load(j_k)), then use(v[k]) in order.
-template <class Load, class Use>
+// a chunk issued before its arithmetic: the chunk, then the gathers
+// (v[k] = load(j_k)) of kGroup entries, then use(v[k]) in order; the next
+// chunk is fetched while the current one is in flight. kGroup < 8 keeps fewer
+// gathered values live (more waves per SIMD) for wide records.
+template <int kGroup = 8, class Load, class Use>
 __device__ __forceinline__ void list_gather(const NbrList &L, const Spans &R, size_t n, int i,
                                             uint32_t info, Load &&load, Use &&use) {
+  static_assert(kChunk % kGroup == 0, "group must divide the chunk");
   const uint32_t c0 = info & 0xFFu;
   const uint32_t c01 = c0 + ((info >> 8) & 0xFFu);
   const uint32_t c = c01 + ((info >> 16) & 0xFFu);
   const uint32_t d1 = R.b1 - R.b0, d2 = R.b2 - R.b1;
   using V = decltype(load(0u));
+  uint4 q = c > 0 ? list_chunk(L, n, i, 0) : make_uint4(0u, 0u, 0u, 0u);
   for (uint32_t g = 0; g * kChunk < c; ++g) {
-    const uint4 q = list_chunk(L, n, i, g);
-    V v[kChunk];
+    const uint4 cur = q;
+    if ((g + 1) * kChunk < c)
+      q = list_chunk(L, n, i, g + 1);
 #pragma unroll
-    for (int k = 0; k < static_cast<int>(kChunk); ++k) {
-      const uint32_t e = g * kChunk + k;
-      if (e < c)
-        v[k] = load(R.b0 + chunk_entry(q, k) + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u));
+    for (int h = 0; h < static_cast<int>(kChunk); h += kGroup) {
+      V v[kGroup];
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) {
+        const uint32_t e = g * kChunk + h + k;
+        if (e < c)
+          v[k] = load(R.b0 + chunk_entry(cur, h + k) + (e >= c0 ? d1 : 0u) +
+                      (e >= c01 ? d2 : 0u));
+      }
+      // keep every gather of the group ahead of the arithmetic (the
+      // scheduler would otherwise consume each gather before issuing the
+      // next one)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k)
+        if (g * kChunk + h + k < c)
+          use(v[k]);
     }
-    // keep every gather of the chunk ahead of the arithmetic (the scheduler
-    // would otherwise consume each gather before issuing the next one)
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < static_cast<int>(kChunk); ++k)
-      if (g * kChunk + k < c)
-        use(v[k]);
   }
 }
 

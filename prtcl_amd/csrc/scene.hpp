@@ -89,9 +89,6 @@ struct SceneView {
   const uint32_t *bstart;
   V4 *xm, *v, *a, *c, *G, *vd;
   V4 *xq;  // (x, y, z, m p / rho^2): the one gather of the pressure-acceleration pass
-  // (x, y, z, m) and (a, 0) of a particle side by side (2 real4 per slot):
-  // the one 32-byte gather, one cache line, of the pressure pass
-  V4 *xa;
   T *rho, *p, *pr, *s, *AA, *Ap, *sp;
   int32_t *id;
   uint32_t *fcell;

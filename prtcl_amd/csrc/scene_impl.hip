@@ -594,7 +594,6 @@ class SceneImpl final : public SceneBase {
     S.G = P_.G;
     S.vd = P_.vd;
     S.xq = P_.xq;
-    S.xa = P_.xa;
     S.rho = P_.rho;
     S.p = P_.p;
     S.pr = P_.pr;
@@ -1315,7 +1314,7 @@ class SceneImpl final : public SceneBase {
 
  private:
   struct Ptrs {
-    V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq, *xa;
+    V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq;
     T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp;
     int32_t *id, *tmpI;
     uint32_t *fcell, *frank, *fperm, *ninfo;
@@ -1341,7 +1340,6 @@ class SceneImpl final : public SceneBase {
     const size_t b = static_cast<size_t>(std::max(nb, 1));
     for (V4 **q : {&P.xm, &P.v, &P.a, &P.c, &P.G, &P.vd, &P.tmpV4, &P.xq})
       A.take(*q, f);
-    A.take(P.xa, 2 * f);
     for (T **q : {&P.rho, &P.p, &P.pr, &P.s, &P.AA, &P.Ap, &P.tmpT})
       A.take(*q, f);
     A.take(P.sp, 9 * f);
@@ -1372,7 +1370,7 @@ class SceneImpl final : public SceneBase {
   // every device pointer into the arena (resp. the grid memory)
   template <class F>
   void each_arena_ptr(F &&f) {
-    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.xa, &P_.bxV,
+    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
                    &P_.btmpV4, &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb})
       f(reinterpret_cast<void **>(q));
     for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp})
@@ -1412,7 +1410,7 @@ class SceneImpl final : public SceneBase {
     bytes = A.off;
     arena_bytes_ = A.off;
     char *base = static_cast<char *>(arena_);
-    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.xa, &P_.bxV,
+    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
                    &P_.btmpV4,
                    &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb})
       Arena::rebase(*q, base);
