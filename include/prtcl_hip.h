@@ -246,7 +246,9 @@ int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *step
  * stage them slab by slab in LDS; an autotuner times both on the first fused
  * steps of each scene (and of batched steps) and keeps the faster per pass.
  * mask: bit 0 setup, bit 1 pressure acceleration, bit 2 pressure in LDS;
- * -1 restarts the tuner. scene < 0 addresses the schedule of batched steps. */
+ * bit 3 launches each IISPH iteration kernel by kernel instead of replaying
+ * it from a captured hipGraph; -1 restarts the tuner. scene < 0 addresses
+ * the schedule of batched steps. */
 int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask);
 int prtcl_hip_get_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t *mask);
 

@@ -32,8 +32,10 @@ void CtxResources::ensure_batch(size_t bytes) {
     PRTCL_HIP_CHECK(hipEventCreateWithFlags(&batch_ev, hipEventDisableTiming));
   if (bytes <= batch_bytes)
     return;
-  // kernels of earlier batches may still read the old tables
+  // kernels of earlier batches may still read the old tables, and the
+  // captured batch iterations point at them
   PRTCL_HIP_CHECK(hipStreamSynchronize(stream));
+  batch_graphs.clear();
   if (batch_host)
     PRTCL_HIP_CHECK(hipHostFree(batch_host));
   if (batch_dev)
@@ -47,6 +49,7 @@ void CtxResources::ensure_batch(size_t bytes) {
 }
 
 void CtxResources::release() {
+  batch_graphs.clear();
   if (batch_host)
     (void)hipHostFree(batch_host);
   if (batch_dev)
@@ -56,6 +59,57 @@ void CtxResources::release() {
   batch_host = batch_dev = nullptr;
   batch_ev = nullptr;
   batch_bytes = 0;
+}
+
+void GraphCache::clear() {
+  for (Entry &e : entries_) {
+    if (e.exec)
+      (void)hipGraphExecDestroy(e.exec);
+    if (e.graph)
+      (void)hipGraphDestroy(e.graph);
+  }
+  entries_.clear();
+}
+
+void GraphCache::launch(hipStream_t st, const std::string &key,
+                        const std::function<void()> &enqueue) {
+  Entry *hit = nullptr;
+  for (Entry &e : entries_)
+    if (e.key == key)
+      hit = &e;
+  if (!hit) {
+    if (entries_.size() >= kMaxEntries) {
+      // evict the least recently used graph once nothing is in flight
+      PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+      auto lru = std::min_element(entries_.begin(), entries_.end(),
+                                  [](const Entry &a, const Entry &b) { return a.used < b.used; });
+      (void)hipGraphExecDestroy(lru->exec);
+      (void)hipGraphDestroy(lru->graph);
+      entries_.erase(lru);
+    }
+    Entry e;
+    e.key = key;
+    PRTCL_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      throw;
+    }
+    PRTCL_HIP_CHECK(hipStreamEndCapture(st, &e.graph));
+    hipError_t err = hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+      (void)hipGraphDestroy(e.graph);
+      throw HipError{err, "hipGraphInstantiate"};
+    }
+    entries_.push_back(e);
+    hit = &entries_.back();
+  }
+  hit->used = ++clock_;
+  PRTCL_HIP_CHECK(hipGraphLaunch(hit->exec, st));
 }
 
 const char *stage_name(int s) {
@@ -797,7 +851,24 @@ class SceneImpl final : public SceneBase {
     Profiler &pf = res_->prof;
     tprof_.on = tune;
     int enq = 0;
+    // one IISPH iteration (pressure acceleration, pressure, finalize) replayed
+    // from a captured graph unless timed or eager
+    const bool graph = fused && !sc.eager && !pf.on && !tune;
+    // the iteration kernels do not read the clock: a view without it keeps
+    // one graph per buffer arrangement (two, alternating step by step)
+    SceneView<T> Sg = S;
+    Sg.t = T(0);
+    const std::string key = graph ? graph_key(Sg, L, sc) : std::string();
     auto enqueue = [&](int n) {
+      if (graph) {
+        for (int k = 0; k < n; ++k)
+          graphs_.launch(st, key, [&] {
+            FusedOps<T>::accel(st, Sg, L, sc.accel_lds);
+            FusedOps<T>::pressure(st, Sg, L, sc.pressure_lds);
+          });
+        enq += n;
+        return;
+      }
       for (int k = 0; k < n; ++k) {
         int r = pf.begin(st, kStagePressureAccel, enq + k);
         int q = tprof_.begin(st, kStagePressureAccel, enq + k);
@@ -1104,7 +1175,18 @@ class SceneImpl final : public SceneBase {
       max_it = std::max(max_it, static_cast<int>(sc->desc.iisph_max_iterations));
     }
     int enq = 0;
+    const bool graph = !sc.eager && !pf.on && !tune;
+    const std::string key = graph ? graph_key(d_solve, d_pmap, np, ns, sc) : std::string();
     auto enqueue = [&](int n) {
+      if (graph) {
+        for (int j = 0; j < n; ++j)
+          res->batch_graphs.launch(st, key, [&] {
+            FusedOps<T>::accel_batch(st, d_solve, d_pmap, np, sc.accel_lds);
+            FusedOps<T>::pressure_batch(st, d_solve, d_pmap, np, ns, sc.pressure_lds);
+          });
+        enq += n;
+        return;
+      }
       for (int j = 0; j < n; ++j) {
         int q1 = pf.begin(st, kStagePressureAccel, enq + j);
         int q2 = tp.begin(st, kStagePressureAccel, enq + j);
@@ -1200,6 +1282,7 @@ class SceneImpl final : public SceneBase {
       return;
     hipStream_t st = res_->stream;
     PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+    graphs_.clear();  // they point into the arena
     host_arena_.resize(arena_bytes_);
     PRTCL_HIP_CHECK(hipMemcpy(host_arena_.data(), arena_, arena_bytes_, hipMemcpyDeviceToHost));
     if (grid_mem_) {
@@ -1461,7 +1544,8 @@ class SceneImpl final : public SceneBase {
   double clock_ = 0;
   int steps_since_permute_ = 0;
   ScheduleTuner tuner_;
-  Profiler tprof_;  // timing of the tuner's trial steps
+  Profiler tprof_;     // timing of the tuner's trial steps
+  GraphCache graphs_;  // captured IISPH iterations of this scene
   int predicted_iters_ = 2;
   int predicted_cg_ = 4;
   unsigned seq_ = 0;

@@ -46,6 +46,41 @@ struct Profiler {
   void reset();
 };
 
+// Captured hipGraphs of launch sequences, keyed by the bytes of their
+// arguments and replayed with one hipGraphLaunch each. The sequences must not
+// contain memsets (graph memset nodes are avoided on purpose) and must read
+// any per-launch data from device memory, not from captured host values that
+// change between replays.
+class GraphCache {
+ public:
+  GraphCache() = default;
+  GraphCache(const GraphCache &) = delete;
+  GraphCache &operator=(const GraphCache &) = delete;
+  ~GraphCache() { clear(); }
+  // drops every graph (the caller makes sure none is in flight)
+  void clear();
+  // replays the graph of `key`, capturing enqueue() on `st` first if absent
+  void launch(hipStream_t st, const std::string &key, const std::function<void()> &enqueue);
+
+ private:
+  struct Entry {
+    std::string key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+  };
+  std::vector<Entry> entries_;
+  uint64_t clock_ = 0;
+  static constexpr size_t kMaxEntries = 8;
+};
+
+template <class... A>
+std::string graph_key(const A &...a) {
+  std::string k;
+  (k.append(reinterpret_cast<const char *>(&a), sizeof(A)), ...);
+  return k;
+}
+
 // Shared per-context resources a scene borrows.
 struct CtxResources {
   Profiler prof;
@@ -66,6 +101,7 @@ struct CtxResources {
   // schedule of the batched fused passes (FP32, FP64) and its trial timing
   ScheduleTuner batch_tuner[2];
   Profiler batch_prof;
+  GraphCache batch_graphs;
   void ensure_batch(size_t bytes);
   void release();
 };
