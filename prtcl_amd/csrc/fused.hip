@@ -276,7 +276,7 @@ template <class T, bool kWithSource, bool kStage>
 __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrList &L,
                                              vec4_t<T> *vout, unsigned lb) {
   constexpr uint32_t kCap = slab_cap<T>();
-  __shared__ vec4_t<T> sx[kStage ? kCap : 1];
+  __shared__ vec4_t<T> sx[kStage ? kCap + kRunPad : 1];
   __shared__ uint16_t lst[kNbrCap + 1][kBlock];  // row kNbrCap: rejected candidates
   __shared__ BlockRanges sR;
   const unsigned t = threadIdx.x;
@@ -335,27 +335,40 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const uint32_t b = rb[r], e = re[r];
-      // eight candidates per trip, loads issued together (indices clamped to
-      // the run; the clamped duplicates are rejected by the bound check).
-      // Branch-free append: every candidate is written to the next free slot
-      // of the LDS table (row kNbrCap once full) and the count advances only
-      // for an accepted one.
+      // eight candidates per trip from consecutive slots, loads issued
+      // together (the arrays carry kRunPad slots of padding, so a trip may
+      // read past the run: those candidates fail the bound test). Branch-free
+      // append: every candidate is written to the next free slot of the LDS
+      // table and the count advances only for an accepted one; a lane with
+      // fewer than 8 free rows writes to row kNbrCap once full.
       for (uint32_t j = b; j < e; j += 8) {
+        const vec4_t<T> *__restrict__ src = kLds ? sx + (j - base) : S.xm + j;
         vec4_t<T> pk[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t jj = min(j + u, e - 1);
-          pk[u] = kLds ? sx[jj - base] : S.xm[jj];
-        }
+        for (int u = 0; u < 8; ++u)
+          pk[u] = src[u];
         __builtin_amdgcn_sched_barrier(0);
+        const uint32_t lim = e - j;                              // u < lim: inside the run
+        const uint32_t self = static_cast<uint32_t>(i) - j;      // u == self: the particle
+        const uint32_t ent = j - base;                           // entry of candidate 0
+        if (cnt + 8u <= kNbrCap) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t ju = j + u;
-          const bool ok = (ju < e) & (ju != static_cast<uint32_t>(i)) &
-                          (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
-                           S.thr2);
-          lst[cnt < kNbrCap ? cnt : kNbrCap][t] = static_cast<uint16_t>(ju - base);
-          cnt += ok ? 1u : 0u;
+          for (int u = 0; u < 8; ++u) {
+            const bool ok = (static_cast<uint32_t>(u) < lim) & (static_cast<uint32_t>(u) != self) &
+                            (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
+                             S.thr2);
+            lst[cnt][t] = static_cast<uint16_t>(ent + u);
+            cnt += ok ? 1u : 0u;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const bool ok = (static_cast<uint32_t>(u) < lim) & (static_cast<uint32_t>(u) != self) &
+                            (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
+                             S.thr2);
+            lst[cnt < kNbrCap ? cnt : kNbrCap][t] = static_cast<uint16_t>(ent + u);
+            cnt += ok ? 1u : 0u;
+          }
         }
       }
     }

@@ -30,6 +30,9 @@ constexpr int kBlock = 256;
 constexpr int kMaxAxisCellsXY = 2047;  // 11 bits
 constexpr int kMaxAxisCellsZ = 1023;   // 10 bits
 constexpr uint32_t kNbrCap = 48;       // fluid neighbours kept per particle and step
+// slots of padding after the position arrays (xm, xq): candidate trips of 8
+// may read up to 7 slots past the end of a run
+constexpr uint32_t kRunPad = 8;
 
 // per-scene control block (device memory)
 struct Ctl {

@@ -1421,8 +1421,11 @@ class SceneImpl final : public SceneBase {
   static void layout(Arena &A, Ptrs &P, int nf, int nb, bool pcg) {
     const size_t f = static_cast<size_t>(std::max(nf, 1));
     const size_t b = static_cast<size_t>(std::max(nb, 1));
-    for (V4 **q : {&P.xm, &P.v, &P.a, &P.c, &P.G, &P.vd, &P.tmpV4, &P.xq})
+    for (V4 **q : {&P.v, &P.a, &P.c, &P.G, &P.vd, &P.tmpV4})
       A.take(*q, f);
+    // xm and xq swap roles every step; both carry the run padding
+    for (V4 **q : {&P.xm, &P.xq})
+      A.take(*q, f + kRunPad);
     for (T **q : {&P.rho, &P.p, &P.pr, &P.s, &P.AA, &P.Ap, &P.tmpT})
       A.take(*q, f);
     A.take(P.sp, 9 * f);
