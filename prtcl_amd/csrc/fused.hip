@@ -277,7 +277,8 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
                                              vec4_t<T> *vout, unsigned lb) {
   constexpr uint32_t kCap = slab_cap<T>();
   __shared__ vec4_t<T> sx[kStage ? kCap + kRunPad : 1];
-  __shared__ uint16_t lst[kNbrCap + 1][kBlock];  // row kNbrCap: rejected candidates
+  // rows kNbrCap + 1 .. kNbrCap + 8 take the writes of a lane past its cap
+  __shared__ uint16_t lst[kNbrCap + 9][kBlock];
   __shared__ BlockRanges sR;
   const unsigned t = threadIdx.x;
   const int i = static_cast<int>(lb * kBlock + t);
@@ -335,42 +336,44 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const uint32_t b = rb[r], e = re[r];
+      const bool own = (s == 1) & (r == 1);  // the run holding particle i
       // eight candidates per trip from consecutive slots, loads issued
-      // together (the arrays carry kRunPad slots of padding, so a trip may
-      // read past the run: those candidates fail the bound test). Branch-free
-      // append: every candidate is written to the next free slot of the LDS
-      // table and the count advances only for an accepted one; a lane with
-      // fewer than 8 free rows writes to row kNbrCap once full.
-      for (uint32_t j = b; j < e; j += 8) {
+      // together. Branch-free append: every candidate is written to row cnt
+      // of this thread's column of the LDS table and cnt advances only for
+      // an accepted one. The table has 8 rows beyond kNbrCap + 1 and cnt is
+      // clamped to kNbrCap + 1 after each trip, so the writes of a trip stay
+      // in the table and a count above kNbrCap still reads as overflow. Full
+      // trips need no bound test; the last, partial trip of a run reads up
+      // to 7 slots past it (the position arrays carry kRunPad slots of
+      // padding) and rejects them. Only the run of the particle's own cell
+      // can hold the particle itself.
+      auto trip = [&](uint32_t j, auto tail_tag) {
+        constexpr bool kTail = decltype(tail_tag)::value;
         const vec4_t<T> *__restrict__ src = kLds ? sx + (j - base) : S.xm + j;
         vec4_t<T> pk[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           pk[u] = src[u];
         __builtin_amdgcn_sched_barrier(0);
-        const uint32_t lim = e - j;                              // u < lim: inside the run
-        const uint32_t self = static_cast<uint32_t>(i) - j;      // u == self: the particle
-        const uint32_t ent = j - base;                           // entry of candidate 0
-        if (cnt + 8u <= kNbrCap) {
+        const uint32_t lim = e - j;                          // u < lim: inside the run
+        const uint32_t self = static_cast<uint32_t>(i) - j;  // u == self: the particle
+        const uint32_t ent = j - base;                       // entry of candidate 0
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const bool ok = (static_cast<uint32_t>(u) < lim) & (static_cast<uint32_t>(u) != self) &
-                            (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
-                             S.thr2);
-            lst[cnt][t] = static_cast<uint16_t>(ent + u);
-            cnt += ok ? 1u : 0u;
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const bool ok = (static_cast<uint32_t>(u) < lim) & (static_cast<uint32_t>(u) != self) &
-                            (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
-                             S.thr2);
-            lst[cnt < kNbrCap ? cnt : kNbrCap][t] = static_cast<uint16_t>(ent + u);
-            cnt += ok ? 1u : 0u;
-          }
+        for (int u = 0; u < 8; ++u) {
+          const bool ok = (!kTail | (static_cast<uint32_t>(u) < lim)) &
+                          (!own | (static_cast<uint32_t>(u) != self)) &
+                          (sqdist_exact(xi.x - pk[u].x, xi.y - pk[u].y, xi.z - pk[u].z) <
+                           S.thr2);
+          lst[cnt][t] = static_cast<uint16_t>(ent + u);
+          cnt += ok ? 1u : 0u;
         }
-      }
+        cnt = cnt < kNbrCap + 1u ? cnt : kNbrCap + 1u;
+      };
+      uint32_t j = b;
+      for (; j + 8u <= e; j += 8u)
+        trip(j, std::false_type{});
+      if (j < e)
+        trip(j, std::true_type{});
     }
   };
   bool over = !encodable;
