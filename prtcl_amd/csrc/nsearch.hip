@@ -179,23 +179,30 @@ __device__ __forceinline__ void k_cell_keys_body(GridDims g, const vec4_t<T> *x,
     lin = c[0] + static_cast<uint32_t>(g.nx) * (c[1] + static_cast<uint32_t>(g.ny) * c[2]);
   }
   // One atomic per distinct cell of the wavefront (particles arrive in the
-  // previous step's cell order, so a wavefront covers only a few cells); the
-  // lanes of a cell take consecutive ranks after the returned base.
+  // previous step's cell order, so a wavefront covers only a few cells): the
+  // lanes of a cell are grouped by ballots first, then the first lane of
+  // every group adds the group's size in one atomic instruction for all
+  // groups, and the lanes of a group take consecutive ranks after the base.
   const int lane = static_cast<int>(__lane_id());
   unsigned long long pending = __ballot(i < n);
-  uint32_t r = 0;
+  int leader_of = lane;
+  uint32_t in_group = 0, group_size = 0;
   while (pending) {
     const int leader = __ffsll(static_cast<long long>(pending)) - 1;
     const uint32_t key = __shfl(lin, leader, kWave);
     const unsigned long long same = __ballot(lin == key) & pending;
-    uint32_t base = 0;
+    if ((same >> lane) & 1ull) {
+      leader_of = leader;
+      in_group = static_cast<uint32_t>(__popcll(same & ((1ull << lane) - 1ull)));
+    }
     if (lane == leader)
-      base = atomicAdd(&count[key], static_cast<uint32_t>(__popcll(same)));
-    base = __shfl(base, leader, kWave);
-    if ((same >> lane) & 1ull)
-      r = base + static_cast<uint32_t>(__popcll(same & ((1ull << lane) - 1ull)));
+      group_size = static_cast<uint32_t>(__popcll(same));
     pending &= ~same;
   }
+  uint32_t base = 0;
+  if (i < n && leader_of == lane)
+    base = atomicAdd(&count[lin], group_size);
+  const uint32_t r = __shfl(base, leader_of, kWave) + in_group;
   if (i < n)
     rank[i] = r;
 }

@@ -220,13 +220,15 @@ __device__ void begin_solve(Ctl *c, double tol, int min_it, int max_it, unsigned
   publish_progress(c);
 }
 
-__global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2,
+__global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2, int i3,
                       unsigned long long *mirror) {
   switch (op) {
     case kCtlInit:  // the arena is zero-filled at allocation
       c->host_mirror = mirror;
       break;
-    case kCtlBeginSolve:
+    case kCtlBeginSolve:  // i3 >= 0: nprde first, in the same launch
+      if (i3 >= 0)
+        c->nprde = i3;
       begin_solve(c, dv, i0, i1, static_cast<unsigned>(i2));
       break;
     case kCtlSetNprde:
@@ -428,9 +430,9 @@ class SceneImpl final : public SceneBase {
     PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
     return c;
   }
-  void ctl_op(int op, double dv = 0, int i0 = 0, int i1 = 0, int i2 = 0) {
+  void ctl_op(int op, double dv = 0, int i0 = 0, int i1 = 0, int i2 = 0, int i3 = -1) {
     hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, res_->stream, P_.ctl, op, dv, i0, i1, i2,
-                       mirror_dev_);
+                       i3, mirror_dev_);
   }
 
   double get_global(int g) override {
@@ -840,12 +842,14 @@ class SceneImpl final : public SceneBase {
 
   // IISPH iterations until the device reports convergence or the cap. With
   // `tune`, the fused passes are also timed for the schedule tuner.
+  // nprde >= 0 also sets iisph_nprde (one control launch for both)
   int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused,
-                 const FusedSchedule &sc = FusedSchedule{}, bool tune = false) {
+                 const FusedSchedule &sc = FusedSchedule{}, bool tune = false,
+                 int nprde = -1) {
     require_grid();
     hipStream_t st = res_->stream;
     const unsigned seq = ++seq_;
-    ctl_op(kCtlBeginSolve, tol, min_it, max_it, static_cast<int>(seq));
+    ctl_op(kCtlBeginSolve, tol, min_it, max_it, static_cast<int>(seq), nprde);
     const SceneView<T> S = view();
     const NbrList L = nbr_list();
     Profiler &pf = res_->prof;
@@ -990,11 +994,10 @@ class SceneImpl final : public SceneBase {
         SphOps<T>::iisph_setup(st, S);
         pf.end(st, r);
       }
-      ctl_op(kCtlSetNprde, 0, nf_);
       int it = 0;
       double rel = 0;
       int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
-                          desc.iisph_max_iterations, &it, &rel, fused, sc, tune);
+                          desc.iisph_max_iterations, &it, &rel, fused, sc, tune, nf_);
       if (fused)
         tuner_.finish_step();
       if (rc != PRTCL_HIP_OK)
