@@ -91,9 +91,11 @@ __global__ __launch_bounds__(kScanBS) void k_scan_top(uint32_t *tile_sums, uint3
     start[n] = carry;
 }
 
-__device__ __forceinline__ void k_scan_down_body(uint32_t *count, uint32_t n,
-                                                       const uint32_t *tile_sums,
-                                                       uint32_t *start, unsigned bid) {
+// exclusive scan of tile bid after the prefix of the tiles before it;
+// clears the counts and returns the tile's total
+__device__ __forceinline__ uint32_t k_scan_down_body(uint32_t *count, uint32_t n,
+                                                     uint32_t prefix, uint32_t *start,
+                                                     unsigned bid) {
   const size_t base = static_cast<size_t>(bid) * kScanTile +
                       static_cast<size_t>(threadIdx.x) * kScanItems;
   uint32_t vals[kScanItems];
@@ -119,7 +121,7 @@ __device__ __forceinline__ void k_scan_down_body(uint32_t *count, uint32_t n,
   for (int k = 0; k < kScanItems; ++k)
     s += vals[k];
   uint32_t tot;
-  uint32_t ex = block_excl_scan(s, &tot) + tile_sums[bid];
+  uint32_t ex = block_excl_scan(s, &tot) + prefix;
   if (full) {
 #pragma unroll
     for (int k = 0; k < kScanItems / 4; ++k) {
@@ -146,12 +148,25 @@ __device__ __forceinline__ void k_scan_down_body(uint32_t *count, uint32_t n,
       ex += vals[k];
     }
   }
+  return tot;
 }
 
+// one launch for the scan of every tile: each block sums the (unscanned)
+// totals of the tiles before it itself, and the last block writes the total
+// count into start[n]
 __global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t n,
                                                        const uint32_t *tile_sums,
                                                        uint32_t *start) {
-  k_scan_down_body(count, n, tile_sums, start, blockIdx.x);
+  const unsigned bid = blockIdx.x;
+  uint32_t p = 0;
+  for (unsigned t = threadIdx.x; t < bid; t += kScanBS)
+    p += tile_sums[t];
+  uint32_t all;
+  block_excl_scan(p, &all);  // the block total, in every thread
+  p = all;
+  const uint32_t tot = k_scan_down_body(count, n, p, start, bid);
+  if (bid + 1 == gridDim.x && threadIdx.x == 0)
+    start[n] = p + tot;
 }
 
 template <class T>
@@ -349,7 +364,8 @@ __global__ __launch_bounds__(kScanBS) void k_scan_down_b(const BatchEntry<T> *ta
                                                          const uint32_t *tmap) {
   const uint32_t u = tmap[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  k_scan_down_body(E.count, E.S.grid.ncells, E.scan_tmp, E.start, u & kBatchLocalMask);
+  k_scan_down_body(E.count, E.S.grid.ncells, E.scan_tmp[u & kBatchLocalMask], E.start,
+                   u & kBatchLocalMask);
 }
 
 template <class T>
@@ -391,7 +407,6 @@ void scan_counts(hipStream_t st, uint32_t *count, uint32_t *start, uint32_t n,
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanBS), 0, st, count, n, tmp);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBS), 0, st, tmp, ntiles, start, n);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanBS), 0, st, count, n, tmp, start);
 }
 
