@@ -62,7 +62,9 @@ enum prtcl_hip_field {
   PRTCL_HIP_F_VOLUME = 11,                /* "volume" (boundary) */
   PRTCL_HIP_F_WKBB18_VELOCITY_DELTA = 12, /* "wkbb18_velocity_delta" [3] */
   PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL = 13,/* "wkbb18_system_diagonal" [3][3] */
-  PRTCL_HIP_F_COUNT = 14
+  PRTCL_HIP_F_GRADIENT_CORRECTION = 14,   /* "gradient_correction" [3][3] (fluid;
+                                             correction.hpp, wkbb18_gc.hpp) */
+  PRTCL_HIP_F_COUNT = 15
 };
 
 /* Global / uniform scalars read back or set (model.global and group
@@ -94,7 +96,14 @@ enum prtcl_hip_proc {
   PRTCL_HIP_P_IISPH_ITERATION_PRESSURE = 10,
   PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL = 11,
   PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION = 12,
-  PRTCL_HIP_P_COUNT = 13
+  /* correction.hpp:133-217 */
+  PRTCL_HIP_P_CORRECTION_IDENTITY_GRADIENT_CORRECTION = 13,
+  PRTCL_HIP_P_CORRECTION_COMPUTE_GRADIENT_CORRECTION = 14,
+  /* wkbb18_gc.hpp: wkbb18 with the fluid particle's gradient_correction
+   * applied to every kernel gradient of its terms */
+  PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL = 15,
+  PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION = 16,
+  PRTCL_HIP_P_COUNT = 17
 };
 
 /* Scene description: counts, arithmetic type and the model's parameter
@@ -122,7 +131,9 @@ typedef struct prtcl_hip_scene_desc {
   double wkbb18_maximum_error;
   int32_t wkbb18_maximum_iterations;
   int32_t explicit_viscosity;  /* step runs viscosity.accumulate_acceleration */
-  int32_t implicit_viscosity;  /* step runs wkbb18 */
+  int32_t implicit_viscosity;  /* step runs 1: wkbb18; 2: wkbb18_gc, with
+                                  correction.compute_gradient_correction after
+                                  the density (prtcl-lua.bunny.lua:277-281) */
   int32_t permute_every;       /* host-visible permute period; 0 = never */
   int32_t step_mode;           /* prtcl_hip_step kernels: 0 = fused passes over a
                                   per-step neighbour list (default), 1 = one

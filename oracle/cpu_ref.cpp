@@ -93,6 +93,67 @@ inline void invert3(const real *m, real *out) {
       out[r + 3 * c] = cof3(m, c, r) * invdet;
 }
 
+// Moore-Penrose pseudo-inverse of a 3x3 matrix (column-major), invert_pm of
+// eigen.hpp:266-276. Eigen takes it from a complete orthogonal decomposition
+// (not available here; its rank counts pivots above 3 eps of the largest);
+// this restatement uses a one-sided Jacobi SVD (Hestenes) in double: rotate
+// column pairs of B = A (accumulating V) until they are orthogonal, then
+// sigma_k = |b_k| and pinv(A) = sum over sigma_k > 3 eps sigma_max of
+// v_k b_k^T / sigma_k^2. Pinned against numpy.linalg.pinv in the tests.
+inline void pinv3(const real *m, real *out) {
+  double b[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int k = 0; k < 9; ++k)
+    b[k] = static_cast<double>(m[k]);
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    bool rotated = false;
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      double al = 0, be = 0, ga = 0;
+      for (int r = 0; r < 3; ++r) {
+        al += b[r + 3 * p] * b[r + 3 * p];
+        be += b[r + 3 * q] * b[r + 3 * q];
+        ga += b[r + 3 * p] * b[r + 3 * q];
+      }
+      if (!(std::abs(ga) > 1e-17 * std::sqrt(al * be)))
+        continue;
+      rotated = true;
+      const double zeta = (be - al) / (2.0 * ga);
+      const double t = std::copysign(1.0, zeta) / (std::abs(zeta) + std::sqrt(1.0 + zeta * zeta));
+      const double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+      for (int r = 0; r < 3; ++r) {
+        const double bp = b[r + 3 * p], bq = b[r + 3 * q];
+        b[r + 3 * p] = c * bp - sn * bq;
+        b[r + 3 * q] = sn * bp + c * bq;
+        const double vp = v[r + 3 * p], vq = v[r + 3 * q];
+        v[r + 3 * p] = c * vp - sn * vq;
+        v[r + 3 * q] = sn * vp + c * vq;
+      }
+    }
+    if (!rotated)
+      break;
+  }
+  double s2[3], smax2 = 0;
+  for (int k = 0; k < 3; ++k) {
+    s2[k] = b[3 * k] * b[3 * k] + b[3 * k + 1] * b[3 * k + 1] + b[3 * k + 2] * b[3 * k + 2];
+    smax2 = std::max(smax2, s2[k]);
+  }
+  const double thr = 3.0 * static_cast<double>(std::numeric_limits<real>::epsilon());
+  double inv[3];
+  for (int k = 0; k < 3; ++k)
+    inv[k] = (s2[k] > thr * thr * smax2 && s2[k] > 0.0) ? 1.0 / s2[k] : 0.0;
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r)
+      out[r + 3 * c] = static_cast<real>(v[r] * b[c] * inv[0] + v[r + 3] * b[c + 3] * inv[1] +
+                                         v[r + 6] * b[c + 6] * inv[2]);
+}
+
+// L g for a 3x3 L (column-major) with Eigen's fixed-size row order
+// l0 g0 + (l1 g1 + l2 g2)
+inline void matvec3(const real *L, const real *g, real *out) {
+  for (int r = 0; r < 3; ++r)
+    out[r] = L[r] * g[0] + (L[r + 3] * g[1] + L[r + 6] * g[2]);
+}
+
 // ---------------------------------------------------------------------------
 // cubic spline kernel (Monaghan 1992), as cubic_spline_kernel.hpp:43-102 and
 // kernel_facade.hpp:204-229
@@ -419,6 +480,7 @@ struct Scene {
                            "time_of_birth"})
       grp[kFluid].add(nm, 1);
     grp[kFluid].add("wkbb18_system_diagonal", 9);
+    grp[kFluid].add("gradient_correction", 9);
     grp[kBoundary].add("position", 3);
     grp[kBoundary].add("volume", 1);
     for (int g = 0; g < 2; ++g) {
@@ -718,21 +780,61 @@ struct Scene {
       aprde = aprde + chunked_sum<real>(F.n, term);
   }
 
-  // ---- wkbb18.compute_diagonal (wkbb18.hpp:194-270)
-  void wkbb18_compute_diagonal() {
+  // ---- correction.identity_gradient_correction (correction.hpp:133-158)
+  void correction_identity() {
+    auto &F = grp[kFluid];
+    real *L = F.ptr("gradient_correction");
+    for (size_t i = 0; i < F.n; ++i)
+      for (int k = 0; k < 9; ++k)
+        L[9 * i + k] = (k % 4 == 0) ? real{1} : real{0};
+  }
+
+  // ---- correction.compute_gradient_correction (correction.hpp:161-217)
+  void correction_compute() {
+    auto &F = grp[kFluid];
+    real *x = F.ptr("position"), *rho = F.ptr("density"), *m = F.ptr("mass");
+    real *L = F.ptr("gradient_correction");
+    const real h = prm.smoothing_scale;
+    foreach_fluid([&](size_t i, auto &nb) {
+      real M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (size_t j : nb[kFluid]) {  // every fluid group (:196)
+        real d[3], e[3], gw[3];
+        sub3(x + 3 * i, x + 3 * j, d);
+        sub3(x + 3 * j, x + 3 * i, e);
+        kernel_grad(d, h, gw);
+        const real k = m[j] / rho[i];
+        for (int col = 0; col < 3; ++col)
+          for (int row = 0; row < 3; ++row)
+            M[row + 3 * col] += k * (gw[row] * e[col]);
+      }
+      pinv3(M, L + 9 * i);
+    });
+  }
+
+  // ---- wkbb18.compute_diagonal (wkbb18.hpp:194-270); gc: wkbb18_gc, the
+  // gradient is L_i gradW (wkbb18_gc.hpp:236-262)
+  void wkbb18_compute_diagonal(bool gc = false) {
     auto &F = grp[kFluid];
     auto &B = grp[kBoundary];
     real *x = F.ptr("position"), *rho = F.ptr("density"), *m = F.ptr("mass");
     real *sp = F.ptr("wkbb18_system_diagonal");
+    real *Lc = F.ptr("gradient_correction");
     real *bx = B.ptr("position"), *V = B.ptr("volume");
     const real h = prm.smoothing_scale, rho0 = prm.rest_density, dt = prm.time_step;
     const real mu = prm.fluid_viscosity, mub = prm.boundary_viscosity;
     foreach_fluid([&](size_t i, auto &nb) {
       real D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      auto grad = [&](const real *d, real *gw) {
+        kernel_grad(d, h, gw);
+        if (gc) {
+          const real g0[3] = {gw[0], gw[1], gw[2]};
+          matvec3(Lc + 9 * i, g0, gw);
+        }
+      };
       for (size_t j : nb[kFluid]) {  // own group only (:230)
         real d[3], gw[3];
         sub3(x + 3 * i, x + 3 * j, d);
-        kernel_grad(d, h, gw);
+        grad(d, gw);
         real f = ((static_cast<real>(10) * (mu / rho[i])) * (m[j] / rho[j])) /
                  (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
         for (int col = 0; col < 3; ++col)
@@ -742,7 +844,7 @@ struct Scene {
       for (size_t j : nb[kBoundary]) {
         real d[3], gw[3];
         sub3(x + 3 * i, bx + 3 * j, d);
-        kernel_grad(d, h, gw);
+        grad(d, gw);
         real f = ((static_cast<real>(10) * (mub / rho[i])) * ((rho0 * V[j]) / rho[i])) /
                  (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
         for (int col = 0; col < 3; ++col)
@@ -759,10 +861,26 @@ struct Scene {
 
   // ---- wkbb18.accumulate_acceleration (wkbb18.hpp:273-425) with the CG of
   // cg_openmp.hpp:32-236
-  void wkbb18_accumulate_acceleration() {
+  // gc: wkbb18_gc, every term is (s L_i) gradW (wkbb18_gc.hpp:362-390)
+  void wkbb18_accumulate_acceleration(bool gc = false) {
     auto &F = grp[kFluid];
     auto &B = grp[kBoundary];
     const size_t n = F.n;
+    real *Lc = F.ptr("gradient_correction");
+    // l += (s L_i) gradW, or s gradW without the correction
+    auto term = [&](size_t i, real s, const real *gw, real *l) {
+      if (gc) {
+        real sl[9], t3[3];
+        for (int k = 0; k < 9; ++k)
+          sl[k] = s * Lc[9 * i + k];
+        matvec3(sl, gw, t3);
+        for (int k = 0; k < 3; ++k)
+          l[k] += t3[k];
+      } else {
+        for (int k = 0; k < 3; ++k)
+          l[k] += s * gw[k];
+      }
+    };
     if (n == 0) {
       wkbb18_iterations = 0;
       return;
@@ -792,8 +910,7 @@ struct Scene {
           real f = (((static_cast<real>(10) * (mu / rho[i])) * (m[j] / rho[j])) *
                     dot3(dw, d)) /
                    (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
-          for (int k = 0; k < 3; ++k)
-            l[k] += f * gw[k];
+          term(i, f, gw, l);
         }
         for (size_t j : nb[kBoundary]) {
           real d[3], gw[3];
@@ -802,8 +919,7 @@ struct Scene {
           real f = (((static_cast<real>(10) * (mub / rho[i])) * ((rho0 * V[j]) / rho[i])) *
                     dot3(w + 3 * i, d)) /
                    (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
-          for (int k = 0; k < 3; ++k)
-            l[k] += f * gw[k];
+          term(i, f, gw, l);
         }
         for (int k = 0; k < 3; ++k)
           out[3 * i + k] = w[3 * i + k] - dt * l[k];
@@ -844,8 +960,7 @@ struct Scene {
           real f = (((static_cast<real>(10) * (mub / rho[i])) * ((rho0 * V[j]) / rho[i])) *
                     dot3(zero3, d)) /
                    (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
-          for (int k = 0; k < 3; ++k)
-            l[k] += f * gw[k];
+          term(i, f, gw, l);
         }
         for (int k = 0; k < 3; ++k)
           cg_b[3 * i + k] = v[3 * i + k] - dt * l[k];
@@ -959,6 +1074,14 @@ struct Scene {
       wkbb18_compute_diagonal();
     else if (proc == "wkbb18.accumulate_acceleration")
       wkbb18_accumulate_acceleration();
+    else if (proc == "correction.identity_gradient_correction")
+      correction_identity();
+    else if (proc == "correction.compute_gradient_correction")
+      correction_compute();
+    else if (proc == "wkbb18_gc.compute_diagonal")
+      wkbb18_compute_diagonal(true);
+    else if (proc == "wkbb18_gc.accumulate_acceleration")
+      wkbb18_accumulate_acceleration(true);
     else
       return -1;
     return 0;
@@ -978,6 +1101,8 @@ struct Scene {
       }
     }
     compute_density();
+    if (prm.implicit_viscosity == 2)  // prtcl-lua.bunny.lua:279-281
+      correction_compute();
     initialize_acceleration();
     if (prm.explicit_viscosity)
       viscosity_accumulate();
@@ -997,9 +1122,9 @@ struct Scene {
       }
     }
     integrate_velocity_with_hard_fade();
-    if (prm.implicit_viscosity) {
-      wkbb18_compute_diagonal();
-      wkbb18_accumulate_acceleration();
+    if (prm.implicit_viscosity) {  // 1: wkbb18, 2: wkbb18_gc
+      wkbb18_compute_diagonal(prm.implicit_viscosity == 2);
+      wkbb18_accumulate_acceleration(prm.implicit_viscosity == 2);
       integrate_velocity_with_hard_fade();
     }
     integrate_position();
@@ -1190,6 +1315,7 @@ int oracle_morton_compare(const int32_t *lhs, const int32_t *rhs) {
 float oracle_kernel_w(const float *v, float h) { return kernel_w(v, h); }
 void oracle_kernel_grad(const float *v, float h, float *out) { kernel_grad(v, h, out); }
 void oracle_invert3(const float *m, float *out) { invert3(m, out); }
+void oracle_pinv3(const float *m, float *out) { pinv3(m, out); }
 
 int64_t oracle_sorted_order(void *scene, int32_t *group, int64_t *index) {
   auto *sc = static_cast<Scene *>(scene);

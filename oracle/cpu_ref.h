@@ -38,7 +38,8 @@ typedef struct oracle_params {
   int32_t iisph_min_iterations;/* example.lua:419 (2) */
   int32_t iisph_max_iterations;/* example.lua:411 (50; 100 in the configs) */
   int32_t explicit_viscosity;  /* run viscosity.accumulate_acceleration */
-  int32_t implicit_viscosity;  /* run wkbb18 */
+  int32_t implicit_viscosity;  /* 1: run wkbb18; 2: correction after the
+                                  density and wkbb18_gc (prtcl-lua.bunny.lua) */
   int32_t permute_every;       /* example.lua:381 (8); 0 disables */
   int32_t fp64_reductions;     /* 0: FP32 reductions as the reference */
 } oracle_params;
@@ -102,6 +103,8 @@ float oracle_kernel_w(const float *v, float h);
 void oracle_kernel_grad(const float *v, float h, float *out);
 /* 3x3 inverse as used by wkbb18 (column-major storage, like Eigen). */
 void oracle_invert3(const float *m, float *out);
+/* Moore-Penrose pseudo-inverse (invert_pm, eigen.hpp:266-276), column-major. */
+void oracle_pinv3(const float *m, float *out);
 
 /* Sorted order of the grid (group, index) pairs; returns count. */
 int64_t oracle_sorted_order(void *scene, int32_t *group, int64_t *index);

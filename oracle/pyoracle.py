@@ -89,6 +89,7 @@ def lib():
         L.oracle_kernel_w.restype = C.c_float
         L.oracle_kernel_grad.argtypes = [vp, C.c_float, vp]
         L.oracle_invert3.argtypes = [vp, vp]
+        L.oracle_pinv3.argtypes = [vp, vp]
         L.oracle_sorted_order.argtypes = [vp, vp, vp]
         L.oracle_sorted_order.restype = i64
         L.oracle_num_threads.restype = C.c_int
@@ -105,6 +106,7 @@ FLUID_FIELDS = {
     "wkbb18_velocity_delta": 3, "density": 1, "pressure": 1, "mass": 1,
     "iisph_source_term": 1, "iisph_diagonal_element": 1,
     "iisph_right_hand_side": 1, "time_of_birth": 1, "wkbb18_system_diagonal": 9,
+    "gradient_correction": 9,
 }
 BOUNDARY_FIELDS = {"position": 3, "volume": 1}
 
@@ -233,6 +235,14 @@ def invert3(m):
     m = np.ascontiguousarray(m, np.float32).reshape(9)
     out = np.empty(9, np.float32)
     lib().oracle_invert3(_ptr(m), _ptr(out))
+    return out
+
+
+def pinv3(m):
+    """Moore-Penrose pseudo-inverse of a column-major 3x3 (invert_pm)."""
+    m = np.ascontiguousarray(m, np.float32).reshape(9)
+    out = np.empty(9, np.float32)
+    lib().oracle_pinv3(_ptr(m), _ptr(out))
     return out
 
 

@@ -189,6 +189,7 @@ const Mapped kMapped[] = {
     {PRTCL_HIP_FLUID, PRTCL_HIP_F_TIME_OF_BIRTH, "time_of_birth", 1},
     {PRTCL_HIP_FLUID, PRTCL_HIP_F_WKBB18_VELOCITY_DELTA, "wkbb18_velocity_delta", 3},
     {PRTCL_HIP_FLUID, PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL, "wkbb18_system_diagonal", 9},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_GRADIENT_CORRECTION, "gradient_correction", 9},
     {PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_POSITION, "position", 3},
     {PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_VOLUME, "volume", 1},
 };
@@ -252,6 +253,9 @@ prtcl_hip_scene_desc desc_of(Model &m, double radius) {
     read_uniform(f->GetUniform(), "dynamic_viscosity", &d.fluid_viscosity);
     read_uniform(f->GetUniform(), "wkbb18_maximum_error", &d.wkbb18_maximum_error);
     read_uniform(f->GetUniform(), "wkbb18_maximum_iterations", &d.wkbb18_maximum_iterations);
+    // a loaded wkbb18 / wkbb18_gc scheme: the scene carries the PCG vectors
+    if (f->GetVarying().HasField("wkbb18_system_diagonal"))
+      d.implicit_viscosity = f->GetVarying().HasField("gradient_correction") ? 2 : 1;
   }
   if (b)
     read_uniform(b->GetUniform(), "dynamic_viscosity", &d.boundary_viscosity);
@@ -591,6 +595,56 @@ void RegisterSchemes() {
               g.AddUniformFieldImpl<float>("dynamic_viscosity");
               g.AddVaryingFieldImpl<float, 3>("position");
               g.AddVaryingFieldImpl<float>("volume");
+            });
+          });
+    });
+    // wkbb18_gc.hpp:135-190: the wkbb18 bindings plus gradient_correction
+    R.RegisterFactory(name_of("wkbb18_gc"), [] {
+      return new HipScheme(
+          name_of("wkbb18_gc"),
+          {{"compute_diagonal", PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL},
+           {"accumulate_acceleration", PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION}},
+          [](Model &m) {
+            m.AddGlobalFieldImpl<float>("smoothing_scale");
+            m.AddGlobalFieldImpl<float>("time_step");
+            m.AddGlobalFieldImpl<float>("current_time");
+            m.AddGlobalFieldImpl<float>("fade_duration");
+            each_of_type(m, "fluid", [](Group &g) {
+              g.AddUniformFieldImpl<float>("rest_density");
+              g.AddUniformFieldImpl<float>("dynamic_viscosity");
+              g.AddUniformFieldImpl<float>("wkbb18_maximum_error");
+              g.AddUniformFieldImpl<int32_t>("wkbb18_maximum_iterations");
+              g.AddUniformFieldImpl<int32_t>("wkbb18_iterations");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float, 3>("velocity");
+              g.AddVaryingFieldImpl<float, 3>("acceleration");
+              g.AddVaryingFieldImpl<float>("density");
+              g.AddVaryingFieldImpl<float>("mass");
+              g.AddVaryingFieldImpl<float, 3, 3>("gradient_correction");
+              g.AddVaryingFieldImpl<float, 3>("wkbb18_velocity_delta");
+              g.AddVaryingFieldImpl<float, 3, 3>("wkbb18_system_diagonal");
+              g.AddVaryingFieldImpl<float>("time_of_birth");
+            });
+            each_of_type(m, "boundary", [](Group &g) {
+              g.AddUniformFieldImpl<float>("dynamic_viscosity");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float>("volume");
+            });
+          });
+    });
+    // correction.hpp:85-130
+    R.RegisterFactory(name_of("correction"), [] {
+      return new HipScheme(
+          name_of("correction"),
+          {{"identity_gradient_correction", PRTCL_HIP_P_CORRECTION_IDENTITY_GRADIENT_CORRECTION},
+           {"compute_gradient_correction", PRTCL_HIP_P_CORRECTION_COMPUTE_GRADIENT_CORRECTION}},
+          [](Model &m) {
+            m.AddGlobalFieldImpl<float>("smoothing_scale");
+            each_of_type(m, "fluid", [](Group &g) {
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float>("density");
+              g.AddVaryingFieldImpl<float>("mass");
+              g.AddVaryingFieldImpl<float, 3, 3>("gradient_correction");
             });
           });
     });

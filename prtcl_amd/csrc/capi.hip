@@ -89,6 +89,7 @@ const Named kFields[] = {
     {"volume", PRTCL_HIP_F_VOLUME},
     {"wkbb18_velocity_delta", PRTCL_HIP_F_WKBB18_VELOCITY_DELTA},
     {"wkbb18_system_diagonal", PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL},
+    {"gradient_correction", PRTCL_HIP_F_GRADIENT_CORRECTION},
 };
 
 const Named kProcs[] = {
@@ -107,13 +108,19 @@ const Named kProcs[] = {
     {"iisph.iteration_pressure", PRTCL_HIP_P_IISPH_ITERATION_PRESSURE},
     {"wkbb18.compute_diagonal", PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL},
     {"wkbb18.accumulate_acceleration", PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION},
+    {"correction.identity_gradient_correction",
+     PRTCL_HIP_P_CORRECTION_IDENTITY_GRADIENT_CORRECTION},
+    {"correction.compute_gradient_correction", PRTCL_HIP_P_CORRECTION_COMPUTE_GRADIENT_CORRECTION},
+    {"wkbb18_gc.compute_diagonal", PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL},
+    {"wkbb18_gc.accumulate_acceleration", PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION},
 };
 
 bool valid_desc(const prtcl_hip_scene_desc *d) {
   return d && d->struct_size == sizeof(prtcl_hip_scene_desc) && d->n_fluid >= 0 &&
          d->n_boundary >= 0 && d->n_fluid < (int64_t(1) << 30) &&
          d->n_boundary < (int64_t(1) << 30) && (d->real_type == 0 || d->real_type == 1) &&
-         d->smoothing_scale > 0 && d->time_step > 0 && d->iisph_max_iterations >= 0;
+         d->smoothing_scale > 0 && d->time_step > 0 && d->iisph_max_iterations >= 0 &&
+         d->implicit_viscosity >= 0 && d->implicit_viscosity <= 2;
 }
 
 }  // namespace

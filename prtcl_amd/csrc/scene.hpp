@@ -93,6 +93,7 @@ struct SceneView {
   V4 *xm, *v, *a, *c, *G, *vd;
   V4 *xq;  // (x, y, z, m p / rho^2): the one gather of the pressure-acceleration pass
   T *rho, *p, *pr, *s, *AA, *Ap, *sp;
+  T *gc;  // gradient correction L (3x3 column-major per fluid particle)
   int32_t *id;
   uint32_t *fcell;
   V4 *bxV;

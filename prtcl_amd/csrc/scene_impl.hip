@@ -348,6 +348,7 @@ class SceneImpl final : public SceneBase {
         case PRTCL_HIP_F_TIME_OF_BIRTH: return {reinterpret_cast<T *>(P_.v), 4, 3, 1};
         case PRTCL_HIP_F_WKBB18_VELOCITY_DELTA: return {reinterpret_cast<T *>(P_.vd), 4, 0, 3};
         case PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL: return {P_.sp, 9, 0, 9};
+        case PRTCL_HIP_F_GRADIENT_CORRECTION: return {P_.gc, 9, 0, 9};
         default: break;
       }
     } else if (group == PRTCL_HIP_BOUNDARY) {
@@ -583,6 +584,7 @@ class SceneImpl final : public SceneBase {
         for (T **f : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap})
           gather_swap(*f, P_.tmpT, pm, nf_, sizeof(T));
         gather_swap(P_.sp, P_.tmpSp, pm, nf_, 9 * sizeof(T));
+        gather_swap(P_.gc, P_.tmpSp, pm, nf_, 9 * sizeof(T));
       }
     }
     fgrid_dirty_ = false;
@@ -657,6 +659,7 @@ class SceneImpl final : public SceneBase {
     S.AA = P_.AA;
     S.Ap = P_.Ap;
     S.sp = P_.sp;
+    S.gc = P_.gc;
     S.id = P_.id;
     S.fcell = P_.fcell;
     S.bxV = P_.bxV;
@@ -713,16 +716,25 @@ class SceneImpl final : public SceneBase {
         break;
       case PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL:
       case PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION:
+      case PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL:
+      case PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION:
         require_grid();
         run_wkbb18(proc);
+        break;
+      case PRTCL_HIP_P_CORRECTION_IDENTITY_GRADIENT_CORRECTION:
+        Wkbb18Ops<T>::identity_correction(st, view());
+        break;
+      case PRTCL_HIP_P_CORRECTION_COMPUTE_GRADIENT_CORRECTION:
+        require_grid();
+        Wkbb18Ops<T>::correction(st, view(), nullptr);
         break;
       default: throw std::out_of_range("procedure does not exist");
     }
     PRTCL_HIP_CHECK(hipGetLastError());
   }
 
-  // wkbb18.compute_diagonal / wkbb18.accumulate_acceleration; with `fused`
-  // the products walk the step's neighbour list
+  // wkbb18[_gc].compute_diagonal / wkbb18[_gc].accumulate_acceleration;
+  // with `fused` the products walk the step's neighbour list
   void run_wkbb18(int proc, bool fused = false) {
     if (!desc.implicit_viscosity)
       throw std::logic_error("the scene was created without implicit viscosity");
@@ -731,9 +743,12 @@ class SceneImpl final : public SceneBase {
     const NbrList L = nbr_list();
     const NbrList *pl = fused ? &L : nullptr;
     Profiler &pf = res_->prof;
-    if (proc == PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL) {
+    const bool gc = proc == PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL ||
+                    proc == PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION;
+    if (proc == PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL ||
+        proc == PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL) {
       int r = pf.begin(st, kStageWkbb18Diag);
-      Wkbb18Ops<T>::diagonal(st, S, pl);
+      Wkbb18Ops<T>::diagonal(st, S, pl, gc);
       pf.end(st, r);
       return;
     }
@@ -742,13 +757,13 @@ class SceneImpl final : public SceneBase {
     const unsigned seq = ++seq_;
     const int max_k = std::max(0, desc.wkbb18_maximum_iterations);
     int r = pf.begin(st, kStagePcgVector);
-    Wkbb18Ops<T>::setup(st, S, pl, P_.cg_part, desc.wkbb18_maximum_error, max_k, seq);
+    Wkbb18Ops<T>::setup(st, S, pl, gc, P_.cg_part, desc.wkbb18_maximum_error, max_k, seq);
     pf.end(st, r);
     int enq = 0;
     auto enqueue = [&](int n) {
       for (int k = 0; k < n; ++k) {
         int rr = pf.begin(st, kStagePcgSpmv, enq + k);
-        Wkbb18Ops<T>::iteration(st, S, pl, P_.cg_part);
+        Wkbb18Ops<T>::iteration(st, S, pl, gc, P_.cg_part);
         pf.end(st, rr);
       }
       enq += n;
@@ -1011,10 +1026,24 @@ class SceneImpl final : public SceneBase {
         r = pf.begin(st, kStageGravityIntegrateV);
         SphOps<T>::integrate_v(st, S);
         pf.end(st, r);
-        if (desc.implicit_viscosity) {
+        if (desc.implicit_viscosity == 1) {
           // example.lua:427-433: diagonal, PCG, integrate velocity again
           run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL, fused);
           run_wkbb18(PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION, fused);
+          r = pf.begin(st, kStageGravityIntegrateV);
+          SphOps<T>::integrate_v(st, S);
+          pf.end(st, r);
+        } else if (desc.implicit_viscosity == 2) {
+          // prtcl-lua.bunny.lua:279-281, 321-327: the gradient correction of
+          // the step's positions and densities (computed here, right before
+          // its only user; x and rho do not change in between), then
+          // wkbb18_gc
+          r = pf.begin(st, kStageWkbb18Diag);
+          const NbrList L = nbr_list();
+          Wkbb18Ops<T>::correction(st, S, fused ? &L : nullptr);
+          pf.end(st, r);
+          run_wkbb18(PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL, fused);
+          run_wkbb18(PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION, fused);
           r = pf.begin(st, kStageGravityIntegrateV);
           SphOps<T>::integrate_v(st, S);
           pf.end(st, r);
@@ -1401,7 +1430,7 @@ class SceneImpl final : public SceneBase {
  private:
   struct Ptrs {
     V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq;
-    T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp;
+    T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp, *gc;
     int32_t *id, *tmpI;
     uint32_t *fcell, *frank, *fperm, *ninfo;
     uint16_t *nidx;
@@ -1433,6 +1462,7 @@ class SceneImpl final : public SceneBase {
       A.take(*q, f);
     A.take(P.sp, 9 * f);
     A.take(P.tmpSp, 9 * f);
+    A.take(P.gc, 9 * f);
     A.take(P.id, f);
     A.take(P.tmpI, f);
     A.take(P.fcell, f);
@@ -1462,7 +1492,8 @@ class SceneImpl final : public SceneBase {
     for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
                    &P_.btmpV4, &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb})
       f(reinterpret_cast<void **>(q));
-    for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp})
+    for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp,
+                  &P_.gc})
       f(reinterpret_cast<void **>(q));
     for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
       f(reinterpret_cast<void **>(q));
@@ -1503,7 +1534,8 @@ class SceneImpl final : public SceneBase {
                    &P_.btmpV4,
                    &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb})
       Arena::rebase(*q, base);
-    for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp})
+    for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp,
+                  &P_.gc})
       Arena::rebase(*q, base);
     for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
       Arena::rebase(*q, base);

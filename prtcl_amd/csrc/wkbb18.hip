@@ -87,8 +87,144 @@ __device__ __forceinline__ T step_l(T eps, T x) {
   return (eps < x) ? T(1) : T(0);
 }
 
-// ---- wkbb18.compute_diagonal, exact (wkbb18.hpp:194-270)
 template <class T>
+struct RealEps;
+template <>
+struct RealEps<float> {
+  static constexpr double value = 1.1920928955078125e-07;  // 2^-23
+};
+template <>
+struct RealEps<double> {
+  static constexpr double value = 2.220446049250313e-16;  // 2^-52
+};
+
+// Moore-Penrose pseudo-inverse of a 3x3 matrix m (column-major) into out, as
+// invert_pm (eigen.hpp:266-276). The reference takes it from Eigen's complete
+// orthogonal decomposition, whose rank counts the pivots above 3 eps of the
+// largest; here a one-sided Jacobi SVD in FP64 (Hestenes rotations of the
+// column pairs until they are orthogonal to working precision, at most 12
+// sweeps) gives A V = B with orthogonal columns b_k = sigma_k u_k, and
+// pinv(A) = sum over sigma_k > 3 eps sigma_max of v_k b_k^T / sigma_k^2.
+template <class T>
+__device__ void pinv3(const T *m, T *out) {
+  double b[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    b[k] = static_cast<double>(m[k]);
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    bool rotated = false;
+#pragma unroll
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      double al = 0, be = 0, ga = 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        al += b[r + 3 * p] * b[r + 3 * p];
+        be += b[r + 3 * q] * b[r + 3 * q];
+        ga += b[r + 3 * p] * b[r + 3 * q];
+      }
+      if (!(fabs(ga) > 1e-17 * sqrt(al * be)))
+        continue;
+      rotated = true;
+      const double zeta = (be - al) / (2.0 * ga);
+      const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double bp = b[r + 3 * p], bq = b[r + 3 * q];
+        b[r + 3 * p] = c * bp - sn * bq;
+        b[r + 3 * q] = sn * bp + c * bq;
+        const double vp = v[r + 3 * p], vq = v[r + 3 * q];
+        v[r + 3 * p] = c * vp - sn * vq;
+        v[r + 3 * q] = sn * vp + c * vq;
+      }
+    }
+    if (!rotated)
+      break;
+  }
+  double s2[3], smax2 = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s2[k] = b[3 * k] * b[3 * k] + b[3 * k + 1] * b[3 * k + 1] + b[3 * k + 2] * b[3 * k + 2];
+    smax2 = s2[k] > smax2 ? s2[k] : smax2;
+  }
+  const double thr = 3.0 * RealEps<T>::value;
+  const double thr2 = thr * thr * smax2;
+  double inv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    inv[k] = (s2[k] > thr2 && s2[k] > 0.0) ? 1.0 / s2[k] : 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      out[r + 3 * c] = static_cast<T>(v[r] * b[c] * inv[0] + v[r + 3] * b[c + 3] * inv[1] +
+                                      v[r + 6] * b[c + 6] * inv[2]);
+}
+
+// ---- correction.identity_gradient_correction (correction.hpp:133-158)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_corr_identity(SceneView<T> S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  T *L = S.gc + 9 * static_cast<size_t>(i);
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    L[k] = (k % 4 == 0) ? T(1) : T(0);
+}
+
+// ---- correction.compute_gradient_correction (correction.hpp:161-217):
+// L_i = pinv(sum_f (m_j / rho_i) gradW_ij (x_j - x_i)^T), exact walk in the
+// reference's FP32 operation order, or over the step's neighbour list
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_corr(SceneView<T> S, NbrList L) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xi = S.xm[i];
+  const T rhoi = S.rho[i];
+  T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  auto add = [&](const vec4_t<T> &pj, T f, T d0, T d1, T d2) {
+    const T g[3] = {f * d0, f * d1, f * d2};
+    const T e[3] = {pj.x - xi.x, pj.y - xi.y, pj.z - xi.z};
+    const T k = pj.w / rhoi;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        M[r + 3 * c] += k * (g[r] * e[c]);
+  };
+  const uint32_t info = kList ? L.info[i] : kListOverflow;
+  if (kList && !(info & kListOverflow)) {
+    list_visit(L, Spans::of(L.blk[i / kBlock]), static_cast<size_t>(S.nf), i, info,
+               [&](uint32_t j) {
+                 const vec4_t<T> pj = S.xm[j];
+                 const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
+                 add(pj, S.K.gradf(sqdist_exact(d0, d1, d2)), d0, d1, d2);
+               });
+  } else {
+    nbrs_exact(S, xi, S.fcell[i], S.fstart, S.xm,
+               [&](uint32_t, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
+                 add(pj, S.K.gradf(r2), d0, d1, d2);
+               });
+  }
+  pinv3(M, S.gc + 9 * static_cast<size_t>(i));
+}
+
+// L_i g with Eigen's fixed-size product order per row: l0 g0 + (l1 g1 + l2 g2)
+template <class T>
+__device__ __forceinline__ void apply_gc(const T *L, const T g[3], T out[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    out[r] = L[r] * g[0] + (L[r + 3] * g[1] + L[r + 6] * g[2]);
+}
+
+// ---- wkbb18.compute_diagonal, exact (wkbb18.hpp:194-270); with kGC the
+// gradient is L_i gradW (wkbb18_gc.hpp:236-262)
+template <class T, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -97,11 +233,28 @@ __global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
   const vec4_t<T> xi = S.xm[i];
   const T rhoi = S.rho[i];
   const uint32_t cell = S.fcell[i];
+  T Li[9];
+  if (kGC) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      Li[k] = S.gc[9 * static_cast<size_t>(i) + k];
+  }
+  auto grad = [&](T f, T d0, T d1, T d2, T g[3]) {
+    const T g0[3] = {f * d0, f * d1, f * d2};
+    if (kGC) {
+      apply_gc(Li, g0, g);
+    } else {
+      g[0] = g0[0];
+      g[1] = g0[1];
+      g[2] = g0[2];
+    }
+  };
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   nbrs_exact(S, xi, cell, S.fstart, S.xm, [&](uint32_t j, T d0, T d1, T d2, T r2,
                                                 const vec4_t<T> &pj) {
     const T f = S.K.gradf(r2);
-    const T g[3] = {f * d0, f * d1, f * d2};
+    T g[3];
+    grad(f, d0, d1, d2, g);
     const T d[3] = {d0, d1, d2};
     const T k = ((T(10) * (S.mu_f / rhoi)) * (pj.w / S.rho[j])) / (r2 + S.hh001);
 #pragma unroll
@@ -113,7 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
   nbrs_exact(S, xi, cell, S.bstart, S.bxV, [&](uint32_t, T d0, T d1, T d2, T r2,
                                                  const vec4_t<T> &pj) {
     const T f = S.K.gradf(r2);
-    const T g[3] = {f * d0, f * d1, f * d2};
+    T g[3];
+    grad(f, d0, d1, d2, g);
     const T d[3] = {d0, d1, d2};
     const T k = ((T(10) * (S.mu_b / rhoi)) * ((S.rho0 * pj.w) / rhoi)) / (r2 + S.hh001);
 #pragma unroll
@@ -131,8 +285,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
   invert3(M, S.sp + 9 * static_cast<size_t>(i));
 }
 
-// ---- product_s, exact (wkbb18.hpp:342-390): out = w - dt * l(w)
-template <class T>
+// ---- product_s, exact (wkbb18.hpp:342-390): out = w - dt * l(w); with kGC
+// every term is (s L_i) gradW (wkbb18_gc.hpp:362-390)
+template <class T, bool kGC>
 __device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
                                               const vec4_t<T> *__restrict__ w, T out[3]) {
 #pragma clang fp contract(off)
@@ -141,6 +296,30 @@ __device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
   const T rhoi = S.rho[i];
   const uint32_t cell = S.fcell[i];
   T l0 = 0, l1 = 0, l2 = 0;
+  T Li[9];
+  if (kGC) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      Li[k] = S.gc[9 * static_cast<size_t>(i) + k];
+  }
+  auto acc = [&](T s, T f, T d0, T d1, T d2) {
+    if (kGC) {
+      const T g[3] = {f * d0, f * d1, f * d2};
+      T sl[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        sl[k] = s * Li[k];
+      T t[3];
+      apply_gc(sl, g, t);
+      l0 += t[0];
+      l1 += t[1];
+      l2 += t[2];
+    } else {
+      l0 += s * (f * d0);
+      l1 += s * (f * d1);
+      l2 += s * (f * d2);
+    }
+  };
   nbrs_exact(S, xi, cell, S.fstart, S.xm, [&](uint32_t j, T d0, T d1, T d2, T r2,
                                                 const vec4_t<T> &pj) {
     const vec4_t<T> wj = w[j];
@@ -148,9 +327,7 @@ __device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
     const T s = (((T(10) * (S.mu_f / rhoi)) * (pj.w / S.rho[j])) *
                  dot3_exact(wi.x - wj.x, wi.y - wj.y, wi.z - wj.z, d0, d1, d2)) /
                 (r2 + S.hh001);
-    l0 += s * (f * d0);
-    l1 += s * (f * d1);
-    l2 += s * (f * d2);
+    acc(s, f, d0, d1, d2);
   });
   nbrs_exact(S, xi, cell, S.bstart, S.bxV, [&](uint32_t, T d0, T d1, T d2, T r2,
                                                  const vec4_t<T> &pj) {
@@ -158,9 +335,7 @@ __device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
     const T s = (((T(10) * (S.mu_b / rhoi)) * ((S.rho0 * pj.w) / rhoi)) *
                  dot3_exact(wi.x, wi.y, wi.z, d0, d1, d2)) /
                 (r2 + S.hh001);
-    l0 += s * (f * d0);
-    l1 += s * (f * d1);
-    l2 += s * (f * d2);
+    acc(s, f, d0, d1, d2);
   });
   out[0] = wi.x - S.dt * l0;
   out[1] = wi.y - S.dt * l1;
@@ -182,7 +357,9 @@ __device__ __forceinline__ T gradf_fast(const SceneView<T> &S, T r2) {
   return r2 < S.K.eps * S.K.eps ? T(0) : ag3 * (u * u - t * t) * rinv;
 }
 
-template <class T>
+// with kGC: M = I - dt L_i (D + B), and the product applies L_i to the summed
+// vector (equal in real arithmetic to applying it per pair)
+template <class T, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.nf)
@@ -233,12 +410,28 @@ __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L)
   T M[9];
   const T dt = S.dt;
   const T Dt[6] = {D[0] + B[0], D[1] + B[1], D[2] + B[2], D[3] + B[3], D[4] + B[4], D[5] + B[5]};
-  M[0] = T(1) - dt * Dt[0];
-  M[4] = T(1) - dt * Dt[1];
-  M[8] = T(1) - dt * Dt[2];
-  M[1] = M[3] = -dt * Dt[3];
-  M[2] = M[6] = -dt * Dt[4];
-  M[5] = M[7] = -dt * Dt[5];
+  if (kGC) {
+    const T Ds[9] = {Dt[0], Dt[3], Dt[4], Dt[3], Dt[1], Dt[5], Dt[4], Dt[5], Dt[2]};
+    const T *Li = S.gc + 9 * static_cast<size_t>(i);
+    T Lr[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      Lr[k] = Li[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        M[r + 3 * c] = ((r == c) ? T(1) : T(0)) -
+                       dt * (Lr[r] * Ds[3 * c] + Lr[r + 3] * Ds[1 + 3 * c] +
+                             Lr[r + 6] * Ds[2 + 3 * c]);
+  } else {
+    M[0] = T(1) - dt * Dt[0];
+    M[4] = T(1) - dt * Dt[1];
+    M[8] = T(1) - dt * Dt[2];
+    M[1] = M[3] = -dt * Dt[3];
+    M[2] = M[6] = -dt * Dt[4];
+    M[5] = M[7] = -dt * Dt[5];
+  }
   invert3(M, S.sp + 9 * static_cast<size_t>(i));
   // B_i for the list product, (x_i, m_i / rho_i) for its gathers
   S.cb[i] = make4<vec4_t<T>>(B[0], B[1], B[2], T(0));
@@ -246,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L)
   S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w / rhoi);
 }
 
-template <class T>
+template <class T, bool kGC>
 __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrList &L, int i,
                                              const vec4_t<T> *__restrict__ w, T out[3]) {
   const vec4_t<T> xq = S.xq[i];  // (x_i, m_i / rho_i)
@@ -280,9 +473,17 @@ __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrLis
   const T b0 = bd.x * wi.x + bo.x * wi.y + bo.y * wi.z;
   const T b1 = bo.x * wi.x + bd.y * wi.y + bo.z * wi.z;
   const T b2 = bo.y * wi.x + bo.z * wi.y + bd.z * wi.z;
-  out[0] = wi.x - S.dt * (cf * l0 + b0);
-  out[1] = wi.y - S.dt * (cf * l1 + b1);
-  out[2] = wi.z - S.dt * (cf * l2 + b2);
+  if (kGC) {
+    const T *Li = S.gc + 9 * static_cast<size_t>(i);
+    const T u0 = cf * l0 + b0, u1 = cf * l1 + b1, u2 = cf * l2 + b2;
+    out[0] = wi.x - S.dt * (Li[0] * u0 + Li[3] * u1 + Li[6] * u2);
+    out[1] = wi.y - S.dt * (Li[1] * u0 + Li[4] * u1 + Li[7] * u2);
+    out[2] = wi.z - S.dt * (Li[2] * u0 + Li[5] * u1 + Li[8] * u2);
+  } else {
+    out[0] = wi.x - S.dt * (cf * l0 + b0);
+    out[1] = wi.y - S.dt * (cf * l1 + b1);
+    out[2] = wi.z - S.dt * (cf * l2 + b2);
+  }
 }
 
 template <class T>
@@ -317,7 +518,7 @@ __device__ __forceinline__ void store_partials(double *partials, unsigned nblock
 // setup: x = v (guess), r = A x - b with b = v (the rhs of wkbb18.hpp:296-334
 // is v - dt a_b with a_b = 0 because the boundary velocity is zero),
 // y = P r, p = -y; partials of b.b, r.r, r.y
-template <class T, bool kExact>
+template <class T, bool kExact, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, double *part) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -325,9 +526,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, 
     const vec4_t<T> v = S.v[i];
     T ax[3];
     if (kExact)
-      product_exact(S, i, S.v, ax);
+      product_exact<T, kGC>(S, i, S.v, ax);
     else
-      product_list(S, L, i, S.v, ax);
+      product_list<T, kGC>(S, L, i, S.v, ax);
     vec4_t<T> b = v;
     b.w = T(0);
     vec4_t<T> r;
@@ -386,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *p
 }
 
 // q = A p, partial p.q
-template <class T, bool kExact>
+template <class T, bool kExact, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, double *part) {
   if (S.ctl->cg_done)
     return;
@@ -395,9 +596,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, d
   if (i < S.nf) {
     T q[3];
     if (kExact)
-      product_exact(S, i, S.cp, q);
+      product_exact<T, kGC>(S, i, S.cp, q);
     else
-      product_list(S, L, i, S.cp, q);
+      product_list<T, kGC>(S, L, i, S.cp, q);
     const vec4_t<T> qv = make4<vec4_t<T>>(q[0], q[1], q[2], T(0));
     S.cq[i] = qv;
     acc[0] = static_cast<double>(dot3r<T>(S.cp[i], qv));
@@ -505,45 +706,80 @@ __global__ __launch_bounds__(kBlock) void k_cg_apply(SceneView<T> S) {
 }  // namespace
 
 template <class T>
-void Wkbb18Ops<T>::diagonal(hipStream_t st, const SceneView<T> &S, const NbrList *L) {
+void Wkbb18Ops<T>::diagonal(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc) {
   if (S.nf <= 0)
     return;
-  if (L)
-    hipLaunchKernelGGL(k_diag_list<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, *L);
-  else
-    hipLaunchKernelGGL(k_diag_exact<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+  const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
+  if (L) {
+    if (gc)
+      hipLaunchKernelGGL((k_diag_list<T, true>), gr, bl, 0, st, S, *L);
+    else
+      hipLaunchKernelGGL((k_diag_list<T, false>), gr, bl, 0, st, S, *L);
+  } else {
+    if (gc)
+      hipLaunchKernelGGL((k_diag_exact<T, true>), gr, bl, 0, st, S);
+    else
+      hipLaunchKernelGGL((k_diag_exact<T, false>), gr, bl, 0, st, S);
+  }
 }
 
 template <class T>
-void Wkbb18Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+void Wkbb18Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
                          double *part, double tol, int max_k, unsigned seq) {
   const unsigned nb = grid_blocks(S.nf);
   if (S.nf > 0) {
-    if (L)
-      hipLaunchKernelGGL((k_cg_setup<T, false>), dim3(nb), dim3(kBlock), 0, st, S, *L, part);
+    const dim3 gr(nb), bl(kBlock);
+    const NbrList Lv = L ? *L : NbrList{};
+    if (L && gc)
+      hipLaunchKernelGGL((k_cg_setup<T, false, true>), gr, bl, 0, st, S, Lv, part);
+    else if (L)
+      hipLaunchKernelGGL((k_cg_setup<T, false, false>), gr, bl, 0, st, S, Lv, part);
+    else if (gc)
+      hipLaunchKernelGGL((k_cg_setup<T, true, true>), gr, bl, 0, st, S, Lv, part);
     else
-      hipLaunchKernelGGL((k_cg_setup<T, true>), dim3(nb), dim3(kBlock), 0, st, S, NbrList{},
-                         part);
+      hipLaunchKernelGGL((k_cg_setup<T, true, false>), gr, bl, 0, st, S, Lv, part);
   }
   hipLaunchKernelGGL(k_cg_setup_fin, dim3(1), dim3(kBlock), 0, st, S.ctl, part, nb, tol, max_k,
                      seq);
 }
 
 template <class T>
-void Wkbb18Ops<T>::iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+void Wkbb18Ops<T>::iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
                              double *part) {
   if (S.nf <= 0)
     return;
   const unsigned nb = grid_blocks(S.nf);
   const dim3 gr(nb), bl(kBlock);
-  if (L)
-    hipLaunchKernelGGL((k_cg_spmv<T, false>), gr, bl, 0, st, S, *L, part);
+  const NbrList Lv = L ? *L : NbrList{};
+  if (L && gc)
+    hipLaunchKernelGGL((k_cg_spmv<T, false, true>), gr, bl, 0, st, S, Lv, part);
+  else if (L)
+    hipLaunchKernelGGL((k_cg_spmv<T, false, false>), gr, bl, 0, st, S, Lv, part);
+  else if (gc)
+    hipLaunchKernelGGL((k_cg_spmv<T, true, true>), gr, bl, 0, st, S, Lv, part);
   else
-    hipLaunchKernelGGL((k_cg_spmv<T, true>), gr, bl, 0, st, S, NbrList{}, part);
+    hipLaunchKernelGGL((k_cg_spmv<T, true, false>), gr, bl, 0, st, S, Lv, part);
   hipLaunchKernelGGL(k_cg_alpha, dim3(1), bl, 0, st, S.ctl, part, nb);
   hipLaunchKernelGGL(k_cg_update<T>, gr, bl, 0, st, S, part);
   hipLaunchKernelGGL(k_cg_beta, dim3(1), bl, 0, st, S.ctl, part, nb);
   hipLaunchKernelGGL(k_cg_direction<T>, gr, bl, 0, st, S);
+}
+
+template <class T>
+void Wkbb18Ops<T>::correction(hipStream_t st, const SceneView<T> &S, const NbrList *L) {
+  if (S.nf <= 0)
+    return;
+  const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
+  if (L)
+    hipLaunchKernelGGL((k_corr<T, true>), gr, bl, 0, st, S, *L);
+  else
+    hipLaunchKernelGGL((k_corr<T, false>), gr, bl, 0, st, S, NbrList{});
+}
+
+template <class T>
+void Wkbb18Ops<T>::identity_correction(hipStream_t st, const SceneView<T> &S) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_corr_identity<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
 }
 
 template <class T>

@@ -9,16 +9,22 @@ namespace prtcl_amd {
 template <class T>
 struct Wkbb18Ops {
   // sp_i = (I - dt D_i)^-1 (and, with a list, the boundary matrix B_i and the
-  // (x, m / rho) gather array of the list product)
-  static void diagonal(hipStream_t st, const SceneView<T> &S, const NbrList *L);
+  // (x, m / rho) gather array of the list product); gc: wkbb18_gc, every
+  // gradient of particle i's terms multiplied by its gradient correction
+  static void diagonal(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc);
   // x = v, r = A x - b, y = P r, p = -y and the FP64 set-up scalars
-  static void setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, double *part,
-                    double tol, int max_k, unsigned seq);
+  static void setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
+                    double *part, double tol, int max_k, unsigned seq);
   // one CG iteration (five kernels; all exit at once after convergence)
-  static void iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+  static void iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
                         double *part);
   // vd = (x - v) H, a = vd / dt
   static void apply(hipStream_t st, const SceneView<T> &S);
+  // correction.compute_gradient_correction: L_i = pinv(sum_f (m_j / rho_i)
+  // gradW_ij (x_j - x_i)^T), over the grid (L null) or the step's list
+  static void correction(hipStream_t st, const SceneView<T> &S, const NbrList *L);
+  // correction.identity_gradient_correction
+  static void identity_correction(hipStream_t st, const SceneView<T> &S);
 };
 
 }  // namespace prtcl_amd

@@ -25,9 +25,10 @@ FIELDS = {
     "mass": 5, "iisph_helper_c": 6, "iisph_source_term": 7,
     "iisph_diagonal_element": 8, "iisph_right_hand_side": 9, "time_of_birth": 10,
     "volume": 11, "wkbb18_velocity_delta": 12, "wkbb18_system_diagonal": 13,
+    "gradient_correction": 14,
 }
 FIELD_COMPONENTS = {0: 3, 1: 3, 2: 3, 3: 1, 4: 1, 5: 1, 6: 3, 7: 1, 8: 1, 9: 1, 10: 1,
-                    11: 1, 12: 3, 13: 9}
+                    11: 1, 12: 3, 13: 9, 14: 9}
 GLOBALS = {"iisph_aprde": 0, "iisph_nprde": 1, "maximum_speed": 2, "wkbb18_iterations": 3,
            "current_time": 4, "iisph_iterations": 5, "iisph_rel_error": 6}
 PROCS = {
@@ -38,6 +39,9 @@ PROCS = {
     "symplectic_euler.integrate_position": 7, "iisph.setup": 8,
     "iisph.iteration_pressure_acceleration": 9, "iisph.iteration_pressure": 10,
     "wkbb18.compute_diagonal": 11, "wkbb18.accumulate_acceleration": 12,
+    "correction.identity_gradient_correction": 13,
+    "correction.compute_gradient_correction": 14,
+    "wkbb18_gc.compute_diagonal": 15, "wkbb18_gc.accumulate_acceleration": 16,
 }
 
 # exported symbols declared in include/prtcl_hip.h

@@ -67,6 +67,21 @@ def main():
     np.savez_compressed(os.path.join(HERE, "tiny_iisph_wkbb18.npz"), **out)
     print("wrote tiny_iisph_wkbb18.npz", n, "fluid", len(xb), "boundary, cg iters", kcg)
 
+    # gradient correction (numpy.linalg.pinv) and WKBB18 with it (wkbb18_gc)
+    L = T.gradient_correction(rho)
+    Ag = T.wkbb18_matrix(V, rho, mu, mu, L)
+    Dg = np.stack([np.linalg.inv(Ag[3 * i:3 * i + 3, 3 * i:3 * i + 3]) for i in range(n)])
+    Mg = np.zeros_like(Ag)
+    for i in range(n):
+        Mg[3 * i:3 * i + 3, 3 * i:3 * i + 3] = Dg[i]
+    xg = np.linalg.solve(Ag, b)
+    xgcg, kgcg = npref.cg_reference(Ag, b, b.copy(), Mg, 1e-4, 200)
+    gc = dict(xf=xf, xb=xb, v=v, m=m.astype(np.float32), h=h, radius=radius, rho0=rho0,
+              dt=dt, mu=mu, V=V, rho=rho, vstar=vs, L=L, sp=Dg, solution=xg.reshape(n, 3),
+              cg_solution=xgcg.reshape(n, 3), cg_iterations=kgcg)
+    np.savez_compressed(os.path.join(HERE, "tiny_wkbb18_gc.npz"), **gc)
+    print("wrote tiny_wkbb18_gc.npz, cg iters", kgcg)
+
 
 if __name__ == "__main__":
     main()

@@ -118,24 +118,36 @@ class Tiny:
         aprde = (((Ap - s) / rho0) * (pn > 1e-9)).sum()
         return Ap, pn, aprde
 
-    # ---- WKBB18 (wkbb18.prtcl:49-173) as a dense 3N x 3N system
-    def wkbb18_matrix(self, V, rho, mu_f, mu_b):
+    # ---- correction.prtcl:21-38: L_i = pinv(sum_f (m_j / rho_i) gradW_ij (x_j - x_i)^T)
+    def gradient_correction(self, rho, rcond=3 * np.finfo(np.float32).eps):
+        n, m = self.nf, self.m
+        Lf = np.zeros((n, 3, 3))
+        for t in range(len(self.fi)):
+            i, j = self.fi[t], self.fj[t]
+            Lf[i] += (m[j] / rho[i]) * np.outer(self.gf[t], -self.df[t])
+        return np.stack([np.linalg.pinv(Lf[i], rcond=rcond) for i in range(n)])
+
+    # ---- WKBB18 (wkbb18.prtcl:49-173) as a dense 3N x 3N system; with L,
+    # wkbb18_gc (wkbb18_gc.prtcl): every gradient of particle i's terms is L_i gradW
+    def wkbb18_matrix(self, V, rho, mu_f, mu_b, L=None):
         """A with (A x)_i = x_i - dt (sum_f k_ij ((x_i - x_j).x_ij) gradW_ij +
         sum_b k_ib (x_i.x_ib) gradW_ib)"""
         n, h, dt, rho0, m = self.nf, self.h, self.dt, self.rho0, self.m
         A = np.eye(3 * n)
+        gf = self.gf if L is None else np.einsum("tab,tb->ta", L[self.fi], self.gf)
+        gb = self.gb if L is None else np.einsum("tab,tb->ta", L[self.bi], self.gb)
         kf = 10 * (mu_f / rho[self.fi]) * (m[self.fj] / rho[self.fj]) / (
             (self.df ** 2).sum(1) + 0.01 * h * h)
         for t in range(len(self.fi)):
             i, j = self.fi[t], self.fj[t]
-            B = kf[t] * np.outer(self.gf[t], self.df[t])
+            B = kf[t] * np.outer(gf[t], self.df[t])
             A[3 * i:3 * i + 3, 3 * i:3 * i + 3] -= dt * B
             A[3 * i:3 * i + 3, 3 * j:3 * j + 3] += dt * B
         kb = 10 * (mu_b / rho[self.bi]) * (rho0 * V[self.bj] / rho[self.bi]) / (
             (self.db ** 2).sum(1) + 0.01 * h * h)
         for t in range(len(self.bi)):
             i = self.bi[t]
-            A[3 * i:3 * i + 3, 3 * i:3 * i + 3] -= dt * kb[t] * np.outer(self.gb[t], self.db[t])
+            A[3 * i:3 * i + 3, 3 * i:3 * i + 3] -= dt * kb[t] * np.outer(gb[t], self.db[t])
         return A
 
 
