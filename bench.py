@@ -41,6 +41,7 @@ STAGE_BYTES = {
     "nsearch_sort": (96, 0),
     "gravity_integrate_v": (40, 0),           # v, a, t_birth -> v
     "wkbb18_diagonal": (56, 16),              # x, rho, m -> sp
+    "gradient_correction": (56, 0),           # x, rho, m -> L (3x3)
     "pcg_spmv": (44, 16),                     # q = A p and p.q, per CG iteration
     "pcg_vector": (156, 0),                   # x, r, y updates, dots, new p
 }
@@ -69,7 +70,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dambreak_1m",
-                    choices=["dambreak_1m", "dambreak_8k", "viscous_250k", "served_stream"])
+                    choices=["dambreak_1m", "dambreak_8k", "viscous_250k", "viscous_gc_250k",
+                             "served_stream"])
     ap.add_argument("--duration", type=float, default=30.0,
                     help="served stream: length of the arrival trace in seconds")
     ap.add_argument("--rate", type=float, default=200.0,
@@ -104,7 +106,11 @@ def main():
     from prtcl_amd import hip, scenes
 
     sc = {"dambreak_1m": scenes.config1, "dambreak_8k": scenes.config0,
-          "viscous_250k": scenes.config2}[args.workload]()
+          "viscous_250k": scenes.config2, "viscous_gc_250k": scenes.config2}[args.workload]()
+    if args.workload == "viscous_gc_250k":
+        # the same tank with the gradient correction and wkbb18_gc
+        sc.implicit_viscosity = 2
+        sc.name = "viscous_gc_column_250k"
     sc.permute_every = 8
     ctx = hip.Context(0)
     g = hip.HipScene(ctx, sc)
@@ -118,13 +124,15 @@ def main():
     nb_active = int(np.count_nonzero(np.diff(off)))
 
     # timed region: K whole steps, no instrumentation
-    iters, lat = [], []
+    iters, wk_iters, lat = [], [], []
     ctx.synchronize()
     t_start = time.perf_counter()
     prev = t_start
     for _ in range(args.steps):
         g.step(1)
-        iters.append(g.step_info()["iisph_iterations"])
+        info = g.step_info()
+        iters.append(info["iisph_iterations"])
+        wk_iters.append(info["wkbb18_iterations"])
         now = time.perf_counter()
         lat.append(now - prev)
         prev = now
@@ -205,6 +213,7 @@ def main():
             "iisph_max_iterations": sc.iisph_max_iterations, "reductions": "f64",
         },
         "iisph_iterations_mean": round(k_mean, 3),
+        "wkbb18_iterations_mean": round(float(np.mean(wk_iters)), 3),
         "p50_step_ms": round(1e3 * float(np.percentile(lat, 50)), 4),
         "p99_step_ms": round(1e3 * float(np.percentile(lat, 99)), 4),
         "step_roofline_frac": round(step_frac, 5),

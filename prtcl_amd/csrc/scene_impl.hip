@@ -116,7 +116,8 @@ const char *stage_name(int s) {
   static const char *names[kStageCount] = {
       "nsearch_sort",      "density",       "gravity_integrate_v", "viscosity",
       "iisph_setup",       "iisph_pressure_acceleration", "iisph_pressure",
-      "integrate_position", "wkbb18_diagonal", "pcg_spmv", "pcg_vector"};
+      "integrate_position", "wkbb18_diagonal", "pcg_spmv", "pcg_vector",
+      "gradient_correction"};
   return (s >= 0 && s < kStageCount) ? names[s] : "?";
 }
 
@@ -1038,7 +1039,7 @@ class SceneImpl final : public SceneBase {
           // the step's positions and densities (computed here, right before
           // its only user; x and rho do not change in between), then
           // wkbb18_gc
-          r = pf.begin(st, kStageWkbb18Diag);
+          r = pf.begin(st, kStageCorrection);
           const NbrList L = nbr_list();
           Wkbb18Ops<T>::correction(st, S, fused ? &L : nullptr);
           pf.end(st, r);

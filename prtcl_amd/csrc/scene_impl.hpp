@@ -22,7 +22,7 @@ namespace prtcl_amd {
 enum Stage {
   kStageSort = 0, kStageDensity, kStageGravityIntegrateV, kStageViscosity, kStageSetup,
   kStagePressureAccel, kStagePressure, kStageIntegrateX, kStageWkbb18Diag, kStagePcgSpmv,
-  kStagePcgVector, kStageCount
+  kStagePcgVector, kStageCorrection, kStageCount
 };
 const char *stage_name(int s);
 

@@ -105,12 +105,38 @@ struct RealEps<double> {
 // column pairs until they are orthogonal to working precision, at most 12
 // sweeps) gives A V = B with orthogonal columns b_k = sigma_k u_k, and
 // pinv(A) = sum over sigma_k > 3 eps sigma_max of v_k b_k^T / sigma_k^2.
+//
+// Fast path: sigma_min / sigma_max >= |det| / sigma_max^3 >= |det| / |A|_F^3,
+// so |det| > 2 (3 eps) |A|_F^3 proves the matrix full rank under the
+// threshold, and then the pseudo-inverse is the inverse (adjugate / det in
+// FP64). Only the other matrices take the SVD.
 template <class T>
 __device__ void pinv3(const T *m, T *out) {
   double b[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 #pragma unroll
   for (int k = 0; k < 9; ++k)
     b[k] = static_cast<double>(m[k]);
+  {
+    double cof[9], fro2 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        cof[r + 3 * c] = b[r1 + 3 * c1] * b[r2 + 3 * c2] - b[r1 + 3 * c2] * b[r2 + 3 * c1];
+        fro2 += b[r + 3 * c] * b[r + 3 * c];
+      }
+    const double det = b[0] * cof[0] + b[1] * cof[1] + b[2] * cof[2];
+    if (fabs(det) > 6.0 * RealEps<T>::value * fro2 * sqrt(fro2)) {
+      const double id = 1.0 / det;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          out[r + 3 * c] = static_cast<T>(cof[c + 3 * r] * id);  // adj(r, c) = cof(c, r)
+      return;
+    }
+  }
   for (int sweep = 0; sweep < 12; ++sweep) {
     bool rotated = false;
 #pragma unroll
@@ -439,6 +465,27 @@ __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L)
   S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w / rhoi);
 }
 
+template <class T>
+__device__ __forceinline__ T rsqrt_hw(T x);
+template <>
+__device__ __forceinline__ float rsqrt_hw<float>(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+template <>
+__device__ __forceinline__ double rsqrt_hw<double>(double x) {
+  return rsqrt(x);
+}
+template <class T>
+__device__ __forceinline__ T rcp_hw(T x);
+template <>
+__device__ __forceinline__ float rcp_hw<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+template <>
+__device__ __forceinline__ double rcp_hw<double>(double x) {
+  return 1.0 / x;
+}
+
 template <class T, bool kGC>
 __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrList &L, int i,
                                              const vec4_t<T> *__restrict__ w, T out[3]) {
@@ -447,26 +494,38 @@ __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrLis
   const T rhoi = S.rho[i];
   const T cf = T(10) * S.mu_f / rhoi;
   const size_t n = static_cast<size_t>(S.nf);
+  // kernel constants of the closed-form gradient factor
+  const T h = S.K.h, inv_h = T(1) / h;
+  const T ag3 = T(3) * S.K.alpha / (h * h * h * h), eps2 = S.K.eps * S.K.eps;
   T l0 = 0, l1 = 0, l2 = 0;
-  auto fluid = [&](uint32_t j) {
-    const vec4_t<T> pj = S.xq[j];
-    const vec4_t<T> wj = w[j];
-    const T d0 = xq.x - pj.x, d1 = xq.y - pj.y, d2 = xq.z - pj.z;
+  struct XW {
+    vec4_t<T> x, w;  // (x_j, m_j / rho_j) and w_j
+  };
+  auto use = [&](const XW &q) {
+    const T d0 = xq.x - q.x.x, d1 = xq.y - q.x.y, d2 = xq.z - q.x.z;
     const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    const T s = pj.w * ((wi.x - wj.x) * d0 + (wi.y - wj.y) * d1 + (wi.z - wj.z) * d2) *
-                gradf_fast(S, r2) / (r2 + S.hh001);
+    const T rinv = rsqrt_hw(r2);
+    const T qh = r2 * rinv * inv_h;
+    const T t = T(2) - qh;
+    T u = T(2) - T(2) * qh;
+    u = u > T(0) ? u : T(0);
+    const T f = r2 < eps2 ? T(0) : ag3 * (u * u - t * t) * rinv;
+    const T s = q.x.w *
+                ((wi.x - q.w.x) * d0 + (wi.y - q.w.y) * d1 + (wi.z - q.w.z) * d2) * f *
+                rcp_hw(r2 + S.hh001);
     l0 += s * d0;
     l1 += s * d1;
     l2 += s * d2;
   };
   const uint32_t info = L.info[i];
   if (!(info & kListOverflow)) {
-    list_visit(L, Spans::of(L.blk[i / kBlock]), n, i, info, fluid);
+    list_gather<4>(L, Spans::of(L.blk[i / kBlock]), n, i, info,
+                   [&](uint32_t j) { return XW{S.xq[j], w[j]}; }, use);
   } else {
     nbrs_exact(S, xq, S.fcell[i], S.fstart, S.xq,
                [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
                  if (j != static_cast<uint32_t>(i))
-                   fluid(j);
+                   use(XW{S.xq[j], w[j]});
                });
   }
   const vec4_t<T> bd = S.cb[i], bo = S.G[i];

@@ -71,7 +71,7 @@ class Scene:
     explicit_viscosity: bool = False
     fluid_viscosity: float = 0.0
     boundary_viscosity: float = 0.0
-    implicit_viscosity: bool = False
+    implicit_viscosity: int = 0  # 1: wkbb18; 2: correction + wkbb18_gc
     wkbb18_maximum_error: float = 1e-4
     wkbb18_maximum_iterations: int = 200
     permute_every: int = 8
