@@ -64,7 +64,17 @@ enum prtcl_hip_field {
   PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL = 13,/* "wkbb18_system_diagonal" [3][3] */
   PRTCL_HIP_F_GRADIENT_CORRECTION = 14,   /* "gradient_correction" [3][3] (fluid;
                                              correction.hpp, wkbb18_gc.hpp) */
-  PRTCL_HIP_F_COUNT = 15
+  /* pt16.hpp:140-190 (fluid; scenes created with implicit_viscosity = 3) */
+  PRTCL_HIP_F_PT16_NORMALIZER = 15,       /* "pt16_normalizer" (bound, unused) */
+  PRTCL_HIP_F_PT16_DENSITY = 16,          /* "pt16_density" */
+  PRTCL_HIP_F_PT16_SAME_DENSITY = 17,     /* "pt16_same_density" */
+  PRTCL_HIP_F_VORTICITY = 18,             /* "vorticity" [3] */
+  PRTCL_HIP_F_TARGET_VELOCITY_GRADIENT = 19, /* "target_velocity_gradient" [3][3] */
+  PRTCL_HIP_F_PT16_VORTICITY_DIFFUSION_DIAGONAL = 20,    /* bound, unused */
+  PRTCL_HIP_F_PT16_VORTICITY_DIFFUSION_RHS = 21,         /* [3] */
+  PRTCL_HIP_F_PT16_VELOCITY_RECONSTRUCTION_DIAGONAL = 22,/* bound, unused */
+  PRTCL_HIP_F_PT16_VELOCITY_RECONSTRUCTION_RHS = 23,     /* [3] */
+  PRTCL_HIP_F_COUNT = 24
 };
 
 /* Global / uniform scalars read back or set (model.global and group
@@ -78,7 +88,12 @@ enum prtcl_hip_global {
   PRTCL_HIP_G_CURRENT_TIME = 4,      /* "current_time" */
   PRTCL_HIP_G_IISPH_ITERATIONS = 5,  /* iterations of the last IISPH solve */
   PRTCL_HIP_G_IISPH_REL_ERROR = 6,   /* aprde / nprde after the last solve */
-  PRTCL_HIP_G_COUNT = 7
+  /* pt16 fluid uniforms (pt16.hpp:160-175) */
+  PRTCL_HIP_G_PT16_MAXIMUM_ERROR = 7,      /* "pt16_maximum_error" */
+  PRTCL_HIP_G_PT16_MAXIMUM_ITERATIONS = 8, /* "pt16_maximum_iterations" */
+  PRTCL_HIP_G_PT16_ITERATIONS = 9,         /* "pt16_iterations" (last solve) */
+  PRTCL_HIP_G_STRAIN_RATE_VISCOSITY = 10,  /* "strain_rate_viscosity" (xi) */
+  PRTCL_HIP_G_COUNT = 11
 };
 
 /* One id per reference procedure (SchemeBase::RunProcedure names). */
@@ -103,7 +118,12 @@ enum prtcl_hip_proc {
    * applied to every kernel gradient of its terms */
   PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL = 15,
   PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION = 16,
-  PRTCL_HIP_P_COUNT = 17
+  /* pt16.hpp:213-760 (scenes created with implicit_viscosity = 3) */
+  PRTCL_HIP_P_PT16_SETUP = 17,
+  PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION = 18,
+  PRTCL_HIP_P_PT16_VORTICITY_PRESERVATION = 19,
+  PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION = 20,
+  PRTCL_HIP_P_COUNT = 21
 };
 
 /* Scene description: counts, arithmetic type and the model's parameter
@@ -133,11 +153,20 @@ typedef struct prtcl_hip_scene_desc {
   int32_t explicit_viscosity;  /* step runs viscosity.accumulate_acceleration */
   int32_t implicit_viscosity;  /* step runs 1: wkbb18; 2: wkbb18_gc, with
                                   correction.compute_gradient_correction after
-                                  the density (prtcl-lua.bunny.lua:277-281) */
+                                  the density (prtcl-lua.bunny.lua:277-281);
+                                  3: pt16 (example.lua:435-445) */
   int32_t permute_every;       /* host-visible permute period; 0 = never */
   int32_t step_mode;           /* prtcl_hip_step kernels: 0 = fused passes over a
                                   per-step neighbour list (default), 1 = one
                                   kernel per reference procedure */
+  /* pt16 (implicit_viscosity = 3): fluid uniform strain_rate_viscosity, and
+   * the pt16_maximum_error / _iterations the step sets before each solve
+   * (prtcl-lua.bunny.lua:129-155, example.lua:436-443) */
+  double pt16_strain_rate_viscosity;
+  double pt16_vorticity_maximum_error;
+  double pt16_velocity_maximum_error;
+  int32_t pt16_vorticity_maximum_iterations;
+  int32_t pt16_velocity_maximum_iterations;
 } prtcl_hip_scene_desc;
 
 /* Fills a descriptor with the reference script defaults. */

@@ -461,6 +461,11 @@ struct Scene {
   int32_t nprde = 0;
   real max_speed = 0;
   int32_t wkbb18_iterations = 0;
+  // pt16 fluid uniforms (pt16.hpp:160-175)
+  real pt16_maximum_error = static_cast<real>(1e-3);
+  int32_t pt16_maximum_iterations = 500;
+  int32_t pt16_iterations = 0;
+  real strain_rate_viscosity = 0;
   int steps_since_permute = 0;
   std::array<GroupData, 2> grp;
   Grid grid;
@@ -481,6 +486,14 @@ struct Scene {
       grp[kFluid].add(nm, 1);
     grp[kFluid].add("wkbb18_system_diagonal", 9);
     grp[kFluid].add("gradient_correction", 9);
+    for (const char *nm : {"pt16_normalizer", "pt16_density", "pt16_same_density",
+                           "pt16_vorticity_diffusion_diagonal",
+                           "pt16_velocity_reconstruction_diagonal"})
+      grp[kFluid].add(nm, 1);
+    for (const char *nm : {"vorticity", "pt16_vorticity_diffusion_rhs",
+                           "pt16_velocity_reconstruction_rhs"})
+      grp[kFluid].add(nm, 3);
+    grp[kFluid].add("target_velocity_gradient", 9);
     grp[kBoundary].add("position", 3);
     grp[kBoundary].add("volume", 1);
     for (int g = 0; g < 2; ++g) {
@@ -811,6 +824,219 @@ struct Scene {
     });
   }
 
+  // ---- pt16 (pt16.hpp:213-760; pt16.prtcl)
+  // setup: same-group and smoothed densities, the velocity gradient, its
+  // spin / expansion / shear decomposition, vorticity and the target
+  // velocity gradient without the spin
+  void pt16_setup() {
+    auto &F = grp[kFluid];
+    auto &B = grp[kBoundary];
+    real *x = F.ptr("position"), *v = F.ptr("velocity"), *m = F.ptr("mass");
+    real *rho = F.ptr("density"), *same = F.ptr("pt16_same_density");
+    real *prho = F.ptr("pt16_density"), *om = F.ptr("vorticity");
+    real *tvg = F.ptr("target_velocity_gradient");
+    real *bx = B.ptr("position"), *V = B.ptr("volume");
+    const real h = prm.smoothing_scale, rho0 = prm.rest_density, xi = strain_rate_viscosity;
+    foreach_fluid([&](size_t i, auto &nb) {
+      real d[3];
+      same[i] = 0;
+      for (size_t j : nb[kFluid]) {  // own group (:246-255)
+        sub3(x + 3 * i, x + 3 * j, d);
+        same[i] += m[j] * kernel_w(d, h);
+      }
+      prho[i] = 0;
+      for (size_t j : nb[kFluid]) {  // every fluid group (:259-269)
+        sub3(x + 3 * i, x + 3 * j, d);
+        prho[i] += (m[j] / rho0) * kernel_w(d, h);
+      }
+      for (size_t j : nb[kBoundary]) {
+        sub3(x + 3 * i, bx + 3 * j, d);
+        prho[i] += V[j] * kernel_w(d, h);
+      }
+      prho[i] *= rho0;
+    });
+    foreach_fluid([&](size_t i, auto &nb) {
+      real vg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (size_t j : nb[kFluid]) {  // own group (:305-316)
+        real d[3], dv[3], gw[3];
+        sub3(x + 3 * i, x + 3 * j, d);
+        sub3(v + 3 * j, v + 3 * i, dv);
+        kernel_grad(d, h, gw);
+        for (int c = 0; c < 3; ++c)
+          for (int r = 0; r < 3; ++r)
+            vg[r + 3 * c] += m[j] * (dv[r] * gw[c]);
+      }
+      for (int k = 0; k < 9; ++k)
+        vg[k] /= rho[i];
+      const real div = vg[0] + (vg[4] + vg[8]);
+      real R[9], Vm[9], S[9];
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+          R[r + 3 * c] = (vg[r + 3 * c] - vg[c + 3 * r]) / static_cast<real>(2);
+          Vm[r + 3 * c] = (div / static_cast<real>(3)) * ((r == c) ? real{1} : real{0});
+          S[r + 3 * c] = ((vg[r + 3 * c] + vg[c + 3 * r]) / static_cast<real>(2)) - Vm[r + 3 * c];
+        }
+      // vector_from_cross_product_matrix: (R(2,1), R(0,2), R(1,0)) (eigen.hpp:341-346)
+      om[3 * i + 0] = static_cast<real>(2) * R[2 + 3 * 1];
+      om[3 * i + 1] = static_cast<real>(2) * R[0 + 3 * 2];
+      om[3 * i + 2] = static_cast<real>(2) * R[1 + 3 * 0];
+      // unit_step_r(0, a, b) = [0 <= a or 0 <= b] (eigen.hpp:320-328)
+      const real step = (real{0} <= rho[i] - rho0 || real{0} <= -div) ? real{1} : real{0};
+      for (int k = 0; k < 9; ++k)
+        tvg[9 * i + k] = Vm[k] * step + xi * S[k];
+    });
+  }
+
+  // tvg += cross_product_matrix_from_vector(0.5 w) (eigen.hpp:330-339)
+  void pt16_spin() {
+    auto &F = grp[kFluid];
+    real *om = F.ptr("vorticity"), *tvg = F.ptr("target_velocity_gradient");
+    for (size_t i = 0; i < F.n; ++i) {
+      const real a0 = static_cast<real>(0.5) * om[3 * i], a1 = static_cast<real>(0.5) * om[3 * i + 1],
+                 a2 = static_cast<real>(0.5) * om[3 * i + 2];
+      const real C[9] = {0, a2, -a1, -a2, 0, a0, a1, -a0, 0};  // column-major
+      for (int k = 0; k < 9; ++k)
+        tvg[9 * i + k] += C[k];
+    }
+  }
+
+  // CGOpenMP<T> (scalar) solve of cg_openmp.hpp:32-215 over the fluid:
+  // system(out, w), precond(out, w), b, x = guess on entry; returns the loop
+  // counter
+  template <class Sys, class Pre>
+  size_t scalar_cg(size_t n, Sys &&system, Pre &&precond, const std::vector<real> &b,
+                   std::vector<real> &x, real tol, size_t max_k) {
+    std::vector<real> r(n), y(n), p(n), q(n);
+    auto dot = [&](const std::vector<real> &l, const std::vector<real> &rr) -> real {
+      auto term = [&](size_t i) { return l[i] * rr[i]; };
+      if (prm.fp64_reductions)
+        return static_cast<real>(chunked_sum<double>(n, term));
+      return chunked_sum<real>(n, term);
+    };
+    system(r.data(), x.data());
+    for (size_t k = 0; k < n; ++k)
+      r[k] -= b[k];
+    const real b_nsq = dot(b, b);
+    const real thr = std::max(tol * tol * b_nsq, std::numeric_limits<real>::min());
+    precond(y.data(), r.data());
+    for (size_t k = 0; k < n; ++k)
+      p[k] = -y[k];
+    real r_nsq = dot(r, r);
+    real next_r_dot_y = dot(r, y);
+    auto ok = [&](size_t k, real cur) {
+      if (k < 1)
+        return true;
+      if (k > max_k)
+        return false;
+      return !(cur < thr);
+    };
+    size_t iter;
+    for (iter = 0; ok(iter, r_nsq); ++iter) {
+      const real prev_r_dot_y = next_r_dot_y;
+      system(q.data(), p.data());
+      const real p_dot_q = dot(p, q);
+      if (std::abs(p_dot_q) < tol)
+        break;
+      const real alpha = prev_r_dot_y / p_dot_q;
+      for (size_t k = 0; k < n; ++k) {
+        x[k] += alpha * p[k];
+        r[k] += alpha * q[k];
+      }
+      r_nsq = dot(r, r);
+      precond(y.data(), r.data());
+      next_r_dot_y = dot(r, y);
+      if (std::abs(prev_r_dot_y) < tol)
+        break;
+      const real beta = next_r_dot_y / prev_r_dot_y;
+      for (size_t k = 0; k < n; ++k)
+        p[k] = -y[k] + beta * p[k];
+    }
+    return iter;
+  }
+
+  // the two pt16 solves; which 0: vorticity diffusion (diagonal coefficient
+  // pt16_same_density, result into vorticity), 1: velocity reconstruction
+  // (pt16_density, result into velocity); preconditioner
+  // 1 / (pt16_same_density - m W(0)) for both
+  void pt16_solve(int which) {
+    auto &F = grp[kFluid];
+    const size_t n = F.n;
+    real *x = F.ptr("position"), *v = F.ptr("velocity"), *m = F.ptr("mass");
+    real *same = F.ptr("pt16_same_density"), *prho = F.ptr("pt16_density");
+    real *om = F.ptr("vorticity"), *tvg = F.ptr("target_velocity_gradient");
+    real *orhs = F.ptr("pt16_vorticity_diffusion_rhs");
+    real *vrhs = F.ptr("pt16_velocity_reconstruction_rhs"), *tb = F.ptr("time_of_birth");
+    const real h = prm.smoothing_scale, xi = strain_rate_viscosity;
+    const real zero3[3] = {0, 0, 0};
+    const real w0 = kernel_w(zero3, h);
+    if (which == 0) {
+      foreach_fluid([&](size_t i, auto &nb) {  // :395-414
+        real acc[3] = {0, 0, 0};
+        for (size_t j : nb[kFluid]) {
+          real d[3];
+          sub3(x + 3 * i, x + 3 * j, d);
+          const real w = kernel_w(d, h);
+          for (int k = 0; k < 3; ++k)
+            acc[k] += (m[j] * (om[3 * i + k] - om[3 * j + k])) * w;
+        }
+        for (int k = 0; k < 3; ++k)
+          orhs[3 * i + k] = acc[k] * xi;
+      });
+    } else {
+      foreach_fluid([&](size_t i, auto &nb) {  // :588-606
+        real acc[3] = {0, 0, 0};
+        for (size_t j : nb[kFluid]) {
+          real d[3], M[9];
+          sub3(x + 3 * i, x + 3 * j, d);
+          for (int k = 0; k < 9; ++k)
+            M[k] = tvg[9 * i + k] + tvg[9 * j + k];
+          const real mh = m[j] * static_cast<real>(0.5);
+          const real w = kernel_w(d, h);
+          for (int r = 0; r < 3; ++r)
+            acc[r] += (mh * (M[r] * d[0] + (M[r + 3] * d[1] + M[r + 6] * d[2]))) * w;
+        }
+        for (int k = 0; k < 3; ++k)
+          vrhs[3 * i + k] = acc[k];
+      });
+    }
+    real *field = which == 0 ? om : v;
+    const real *rhs = which == 0 ? orhs : vrhs;
+    const real *diag = which == 0 ? same : prho;
+    const real t = current_time, fade = prm.fade_duration;
+    for (int dim = 0; dim < 3; ++dim) {
+      std::vector<real> b(n), xs(n);
+      for (size_t i = 0; i < n; ++i) {
+        b[i] = rhs[3 * i + dim];
+        xs[i] = field[3 * i + dim];
+      }
+      auto system = [&](real *out, const real *w) {
+        foreach_fluid([&](size_t i, auto &nb) {
+          real acc = diag[i] * w[i];
+          for (size_t j : nb[kFluid]) {
+            real d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            acc -= (m[j] * kernel_w(d, h)) * w[j];
+          }
+          out[i] = acc;
+        });
+      };
+      auto precond = [&](real *out, const real *w) {
+        for (size_t i = 0; i < n; ++i)
+          out[i] = w[i] / (same[i] - m[i] * w0);
+      };
+      const size_t it = scalar_cg(n, system, precond, b, xs, pt16_maximum_error,
+                                  static_cast<size_t>(std::max(pt16_maximum_iterations, 0)));
+      for (size_t i = 0; i < n; ++i) {  // apply (:491-502, 700-717)
+        const real age = t - tb[i];
+        field[3 * i + dim] = xs[i] * unit_step_l(real{0}, age - fade) +
+                             field[3 * i + dim] * ((real{0} <= fade - age) ? real{1} : real{0});
+      }
+      pt16_iterations = static_cast<int32_t>(it);
+    }
+    if (which == 0)
+      pt16_spin();
+  }
+
   // ---- wkbb18.compute_diagonal (wkbb18.hpp:194-270); gc: wkbb18_gc, the
   // gradient is L_i gradW (wkbb18_gc.hpp:236-262)
   void wkbb18_compute_diagonal(bool gc = false) {
@@ -1082,6 +1308,14 @@ struct Scene {
       wkbb18_compute_diagonal(true);
     else if (proc == "wkbb18_gc.accumulate_acceleration")
       wkbb18_accumulate_acceleration(true);
+    else if (proc == "pt16.setup")
+      pt16_setup();
+    else if (proc == "pt16.solve_vorticity_diffusion")
+      pt16_solve(0);
+    else if (proc == "pt16.vorticity_preservation")
+      pt16_spin();
+    else if (proc == "pt16.solve_velocity_reconstruction")
+      pt16_solve(1);
     else
       return -1;
     return 0;
@@ -1122,17 +1356,26 @@ struct Scene {
       }
     }
     integrate_velocity_with_hard_fade();
-    if (prm.implicit_viscosity) {  // 1: wkbb18, 2: wkbb18_gc
+    if (prm.implicit_viscosity == 1 || prm.implicit_viscosity == 2) {  // wkbb18 / _gc
       wkbb18_compute_diagonal(prm.implicit_viscosity == 2);
       wkbb18_accumulate_acceleration(prm.implicit_viscosity == 2);
       integrate_velocity_with_hard_fade();
+    } else if (prm.implicit_viscosity == 3) {  // pt16 (example.lua:435-445)
+      pt16_setup();
+      pt16_maximum_error = prm.pt16_vorticity_maximum_error;
+      pt16_maximum_iterations = prm.pt16_vorticity_maximum_iterations;
+      pt16_solve(0);
+      pt16_maximum_error = prm.pt16_velocity_maximum_error;
+      pt16_maximum_iterations = prm.pt16_velocity_maximum_iterations;
+      pt16_solve(1);
     }
     integrate_position();
     clock += static_cast<double>(prm.time_step);
     if (info) {
       info->iisph_iterations = iters;
       info->iisph_relative_error = rel;
-      info->wkbb18_iterations = prm.implicit_viscosity ? wkbb18_iterations : 0;
+      info->wkbb18_iterations = prm.implicit_viscosity == 3 ? pt16_iterations
+                                : prm.implicit_viscosity ? wkbb18_iterations : 0;
       info->maximum_speed = max_speed;
     }
   }
@@ -1186,6 +1429,9 @@ void oracle_set_params(void *scene, const oracle_params *p) {
   auto *sc = static_cast<Scene *>(scene);
   sc->prm = *p;
   sc->grid.radius = p->radius;
+  sc->strain_rate_viscosity = p->strain_rate_viscosity;
+  sc->pt16_maximum_error = p->pt16_vorticity_maximum_error;
+  sc->pt16_maximum_iterations = p->pt16_vorticity_maximum_iterations;
 }
 void oracle_set_time(void *scene, double t) { static_cast<Scene *>(scene)->clock = t; }
 double oracle_get_time(void *scene) { return static_cast<Scene *>(scene)->clock; }
@@ -1249,6 +1495,14 @@ double oracle_get_global(void *scene, const char *name) {
     return sc->wkbb18_iterations;
   if (s == "current_time")
     return sc->current_time;
+  if (s == "pt16_maximum_error")
+    return sc->pt16_maximum_error;
+  if (s == "pt16_maximum_iterations")
+    return sc->pt16_maximum_iterations;
+  if (s == "pt16_iterations")
+    return sc->pt16_iterations;
+  if (s == "strain_rate_viscosity")
+    return sc->strain_rate_viscosity;
   return std::nan("");
 }
 
@@ -1263,6 +1517,12 @@ void oracle_set_global(void *scene, const char *name, double value) {
     sc->max_speed = static_cast<real>(value);
   else if (s == "current_time")
     sc->current_time = static_cast<real>(value);
+  else if (s == "pt16_maximum_error")
+    sc->pt16_maximum_error = static_cast<real>(value);
+  else if (s == "pt16_maximum_iterations")
+    sc->pt16_maximum_iterations = static_cast<int32_t>(value);
+  else if (s == "strain_rate_viscosity")
+    sc->strain_rate_viscosity = static_cast<real>(value);
 }
 
 void oracle_nsearch_update(void *scene) { static_cast<Scene *>(scene)->nsearch_update(); }

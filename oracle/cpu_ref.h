@@ -39,9 +39,18 @@ typedef struct oracle_params {
   int32_t iisph_max_iterations;/* example.lua:411 (50; 100 in the configs) */
   int32_t explicit_viscosity;  /* run viscosity.accumulate_acceleration */
   int32_t implicit_viscosity;  /* 1: run wkbb18; 2: correction after the
-                                  density and wkbb18_gc (prtcl-lua.bunny.lua) */
+                                  density and wkbb18_gc (prtcl-lua.bunny.lua);
+                                  3: pt16 (example.lua:435-445) */
   int32_t permute_every;       /* example.lua:381 (8); 0 disables */
   int32_t fp64_reductions;     /* 0: FP32 reductions as the reference */
+  /* pt16 (implicit_viscosity = 3): fluid uniform strain_rate_viscosity and the
+   * pt16_maximum_error / _iterations the step sets before each of its two
+   * solves (prtcl-lua.bunny.lua:150-155, example.lua:436-443) */
+  float strain_rate_viscosity;
+  float pt16_vorticity_maximum_error;
+  float pt16_velocity_maximum_error;
+  int32_t pt16_vorticity_maximum_iterations;
+  int32_t pt16_velocity_maximum_iterations;
 } oracle_params;
 
 typedef struct oracle_step_info {
@@ -66,7 +75,8 @@ int oracle_get_field(void *scene, const char *group, const char *name,
                      void *out, int64_t count);
 
 /* global reduction results: "iisph_aprde", "iisph_nprde", "maximum_speed",
- * "wkbb18_iterations" */
+ * "wkbb18_iterations", and the pt16 uniforms "pt16_maximum_error",
+ * "pt16_maximum_iterations", "pt16_iterations", "strain_rate_viscosity" */
 double oracle_get_global(void *scene, const char *name);
 void oracle_set_global(void *scene, const char *name, double value);
 

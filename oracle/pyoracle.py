@@ -37,6 +37,11 @@ class Params(C.Structure):
         ("implicit_viscosity", C.c_int32),
         ("permute_every", C.c_int32),
         ("fp64_reductions", C.c_int32),
+        ("strain_rate_viscosity", C.c_float),
+        ("pt16_vorticity_maximum_error", C.c_float),
+        ("pt16_velocity_maximum_error", C.c_float),
+        ("pt16_vorticity_maximum_iterations", C.c_int32),
+        ("pt16_velocity_maximum_iterations", C.c_int32),
     ]
 
 
@@ -106,7 +111,10 @@ FLUID_FIELDS = {
     "wkbb18_velocity_delta": 3, "density": 1, "pressure": 1, "mass": 1,
     "iisph_source_term": 1, "iisph_diagonal_element": 1,
     "iisph_right_hand_side": 1, "time_of_birth": 1, "wkbb18_system_diagonal": 9,
-    "gradient_correction": 9,
+    "gradient_correction": 9, "pt16_normalizer": 1, "pt16_density": 1,
+    "pt16_same_density": 1, "vorticity": 3, "target_velocity_gradient": 9,
+    "pt16_vorticity_diffusion_diagonal": 1, "pt16_vorticity_diffusion_rhs": 3,
+    "pt16_velocity_reconstruction_diagonal": 1, "pt16_velocity_reconstruction_rhs": 3,
 }
 BOUNDARY_FIELDS = {"position": 3, "volume": 1}
 
@@ -138,6 +146,11 @@ class OracleScene:
         p.implicit_viscosity = int(scene.implicit_viscosity)
         p.permute_every = scene.permute_every if permute_every is None else permute_every
         p.fp64_reductions = int(fp64_reductions)
+        p.strain_rate_viscosity = scene.pt16_strain_rate_viscosity
+        p.pt16_vorticity_maximum_error = scene.pt16_vorticity_maximum_error
+        p.pt16_velocity_maximum_error = scene.pt16_velocity_maximum_error
+        p.pt16_vorticity_maximum_iterations = scene.pt16_vorticity_maximum_iterations
+        p.pt16_velocity_maximum_iterations = scene.pt16_velocity_maximum_iterations
         self.params = p
         self.L.oracle_set_params(self.h, C.byref(p))
         self.set("fluid", "position", scene.fluid_x)

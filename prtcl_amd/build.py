@@ -36,6 +36,7 @@ SOURCES = {
     # schedules of a pass share the pair arithmetic and must round alike
     "fused.hip": ["-ffp-contract=on"],
     "wkbb18.hip": ["-ffp-contract=off"],
+    "pt16.hip": ["-ffp-contract=off"],
     "scene_impl.hip": [],
     "serve.hip": [],
     "capi.hip": [],

@@ -190,6 +190,20 @@ const Mapped kMapped[] = {
     {PRTCL_HIP_FLUID, PRTCL_HIP_F_WKBB18_VELOCITY_DELTA, "wkbb18_velocity_delta", 3},
     {PRTCL_HIP_FLUID, PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL, "wkbb18_system_diagonal", 9},
     {PRTCL_HIP_FLUID, PRTCL_HIP_F_GRADIENT_CORRECTION, "gradient_correction", 9},
+    // pt16 (a scene mirrors them when the model has pt16's fields)
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_NORMALIZER, "pt16_normalizer", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_DENSITY, "pt16_density", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_SAME_DENSITY, "pt16_same_density", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_VORTICITY, "vorticity", 3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_TARGET_VELOCITY_GRADIENT, "target_velocity_gradient", 9},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_VORTICITY_DIFFUSION_DIAGONAL,
+     "pt16_vorticity_diffusion_diagonal", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_VORTICITY_DIFFUSION_RHS, "pt16_vorticity_diffusion_rhs",
+     3},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_VELOCITY_RECONSTRUCTION_DIAGONAL,
+     "pt16_velocity_reconstruction_diagonal", 1},
+    {PRTCL_HIP_FLUID, PRTCL_HIP_F_PT16_VELOCITY_RECONSTRUCTION_RHS,
+     "pt16_velocity_reconstruction_rhs", 3},
     {PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_POSITION, "position", 3},
     {PRTCL_HIP_BOUNDARY, PRTCL_HIP_F_VOLUME, "volume", 1},
 };
@@ -256,6 +270,9 @@ prtcl_hip_scene_desc desc_of(Model &m, double radius) {
     // a loaded wkbb18 / wkbb18_gc scheme: the scene carries the PCG vectors
     if (f->GetVarying().HasField("wkbb18_system_diagonal"))
       d.implicit_viscosity = f->GetVarying().HasField("gradient_correction") ? 2 : 1;
+    // a loaded pt16 scheme: the scene carries pt16's fields (and PCG vectors)
+    if (f->GetVarying().HasField("pt16_same_density"))
+      d.implicit_viscosity = 3;
   }
   if (b)
     read_uniform(b->GetUniform(), "dynamic_viscosity", &d.boundary_viscosity);
@@ -315,6 +332,21 @@ void transfer(Model &m, int32_t scene, bool up) {
       check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_IISPH_APRDE, v), "set_global");
     if (read_uniform(gl, "iisph_nprde", &v))
       check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_IISPH_NPRDE, v), "set_global");
+    // pt16's fluid uniforms, set by the host loop between its solves
+    // (example.lua:436-443)
+    Group *f = groups[PRTCL_HIP_FLUID];
+    if (f && f->GetVarying().HasField("pt16_same_density")) {
+      const UniformManager &u = f->GetUniform();
+      if (read_uniform(u, "pt16_maximum_error", &v))
+        check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_PT16_MAXIMUM_ERROR, v),
+              "set_global");
+      if (read_uniform(u, "pt16_maximum_iterations", &v))
+        check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_PT16_MAXIMUM_ITERATIONS, v),
+              "set_global");
+      if (read_uniform(u, "strain_rate_viscosity", &v))
+        check(ctx, prtcl_hip_set_global(ctx, scene, PRTCL_HIP_G_STRAIN_RATE_VISCOSITY, v),
+              "set_global");
+    }
   } else {
     double v = 0;
     check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_IISPH_APRDE, &v), "get_global");
@@ -327,6 +359,11 @@ void transfer(Model &m, int32_t scene, bool up) {
       check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_WKBB18_ITERATIONS, &v),
             "get_global");
       write_uniform(f->GetUniform(), "wkbb18_iterations", v);
+      if (f->GetVarying().HasField("pt16_same_density")) {
+        check(ctx, prtcl_hip_get_global(ctx, scene, PRTCL_HIP_G_PT16_ITERATIONS, &v),
+              "get_global");
+        write_uniform(f->GetUniform(), "pt16_iterations", v);
+      }
     }
   }
 }
@@ -627,6 +664,48 @@ void RegisterSchemes() {
             });
             each_of_type(m, "boundary", [](Group &g) {
               g.AddUniformFieldImpl<float>("dynamic_viscosity");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float>("volume");
+            });
+          });
+    });
+    // pt16.hpp:140-198
+    R.RegisterFactory(name_of("pt16"), [] {
+      return new HipScheme(
+          name_of("pt16"),
+          {{"setup", PRTCL_HIP_P_PT16_SETUP},
+           {"solve_vorticity_diffusion", PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION},
+           {"vorticity_preservation", PRTCL_HIP_P_PT16_VORTICITY_PRESERVATION},
+           {"solve_velocity_reconstruction", PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION}},
+          [](Model &m) {
+            m.AddGlobalFieldImpl<float>("smoothing_scale");
+            m.AddGlobalFieldImpl<float>("time_step");
+            m.AddGlobalFieldImpl<float>("current_time");
+            m.AddGlobalFieldImpl<float>("fade_duration");
+            each_of_type(m, "fluid", [](Group &g) {
+              g.AddUniformFieldImpl<float>("rest_density");
+              g.AddUniformFieldImpl<float>("strain_rate_viscosity");
+              g.AddUniformFieldImpl<float>("pt16_maximum_error");
+              g.AddUniformFieldImpl<int32_t>("pt16_maximum_iterations");
+              g.AddUniformFieldImpl<int32_t>("pt16_iterations");
+              g.AddVaryingFieldImpl<float, 3>("position");
+              g.AddVaryingFieldImpl<float, 3>("velocity");
+              g.AddVaryingFieldImpl<float, 3>("acceleration");
+              g.AddVaryingFieldImpl<float>("density");
+              g.AddVaryingFieldImpl<float>("mass");
+              g.AddVaryingFieldImpl<float, 3, 3>("gradient_correction");
+              g.AddVaryingFieldImpl<float>("time_of_birth");
+              g.AddVaryingFieldImpl<float>("pt16_normalizer");
+              g.AddVaryingFieldImpl<float>("pt16_density");
+              g.AddVaryingFieldImpl<float>("pt16_same_density");
+              g.AddVaryingFieldImpl<float, 3>("vorticity");
+              g.AddVaryingFieldImpl<float, 3, 3>("target_velocity_gradient");
+              g.AddVaryingFieldImpl<float>("pt16_vorticity_diffusion_diagonal");
+              g.AddVaryingFieldImpl<float, 3>("pt16_vorticity_diffusion_rhs");
+              g.AddVaryingFieldImpl<float>("pt16_velocity_reconstruction_diagonal");
+              g.AddVaryingFieldImpl<float, 3>("pt16_velocity_reconstruction_rhs");
+            });
+            each_of_type(m, "boundary", [](Group &g) {
               g.AddVaryingFieldImpl<float, 3>("position");
               g.AddVaryingFieldImpl<float>("volume");
             });

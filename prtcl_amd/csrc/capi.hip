@@ -90,6 +90,16 @@ const Named kFields[] = {
     {"wkbb18_velocity_delta", PRTCL_HIP_F_WKBB18_VELOCITY_DELTA},
     {"wkbb18_system_diagonal", PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL},
     {"gradient_correction", PRTCL_HIP_F_GRADIENT_CORRECTION},
+    {"pt16_normalizer", PRTCL_HIP_F_PT16_NORMALIZER},
+    {"pt16_density", PRTCL_HIP_F_PT16_DENSITY},
+    {"pt16_same_density", PRTCL_HIP_F_PT16_SAME_DENSITY},
+    {"vorticity", PRTCL_HIP_F_VORTICITY},
+    {"target_velocity_gradient", PRTCL_HIP_F_TARGET_VELOCITY_GRADIENT},
+    {"pt16_vorticity_diffusion_diagonal", PRTCL_HIP_F_PT16_VORTICITY_DIFFUSION_DIAGONAL},
+    {"pt16_vorticity_diffusion_rhs", PRTCL_HIP_F_PT16_VORTICITY_DIFFUSION_RHS},
+    {"pt16_velocity_reconstruction_diagonal",
+     PRTCL_HIP_F_PT16_VELOCITY_RECONSTRUCTION_DIAGONAL},
+    {"pt16_velocity_reconstruction_rhs", PRTCL_HIP_F_PT16_VELOCITY_RECONSTRUCTION_RHS},
 };
 
 const Named kProcs[] = {
@@ -113,6 +123,10 @@ const Named kProcs[] = {
     {"correction.compute_gradient_correction", PRTCL_HIP_P_CORRECTION_COMPUTE_GRADIENT_CORRECTION},
     {"wkbb18_gc.compute_diagonal", PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL},
     {"wkbb18_gc.accumulate_acceleration", PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION},
+    {"pt16.setup", PRTCL_HIP_P_PT16_SETUP},
+    {"pt16.solve_vorticity_diffusion", PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION},
+    {"pt16.vorticity_preservation", PRTCL_HIP_P_PT16_VORTICITY_PRESERVATION},
+    {"pt16.solve_velocity_reconstruction", PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION},
 };
 
 bool valid_desc(const prtcl_hip_scene_desc *d) {
@@ -120,7 +134,7 @@ bool valid_desc(const prtcl_hip_scene_desc *d) {
          d->n_boundary >= 0 && d->n_fluid < (int64_t(1) << 30) &&
          d->n_boundary < (int64_t(1) << 30) && (d->real_type == 0 || d->real_type == 1) &&
          d->smoothing_scale > 0 && d->time_step > 0 && d->iisph_max_iterations >= 0 &&
-         d->implicit_viscosity >= 0 && d->implicit_viscosity <= 2;
+         d->implicit_viscosity >= 0 && d->implicit_viscosity <= 3;
 }
 
 }  // namespace
@@ -145,6 +159,11 @@ void prtcl_hip_scene_desc_init(prtcl_hip_scene_desc *d) {
   d->wkbb18_maximum_error = 0.05;
   d->wkbb18_maximum_iterations = 100;
   d->permute_every = 8;
+  d->pt16_strain_rate_viscosity = 0.0;
+  d->pt16_vorticity_maximum_error = 1e-3;
+  d->pt16_velocity_maximum_error = 1e-3;
+  d->pt16_vorticity_maximum_iterations = 500;
+  d->pt16_velocity_maximum_iterations = 500;
 }
 
 int prtcl_hip_ctx_create(int device, uint64_t budget, prtcl_hip_ctx **out) {

@@ -1,0 +1,106 @@
+// FP64 control of the preconditioned CG of cg_openmp.hpp:32-215 on the
+// device, shared by every solver that uses it (WKBB18's 3-vector system, the
+// scalar systems of pt16): per-block partial sums of the dot products, the
+// set-up scalars, alpha with the |p.q| < tol break, beta with the
+// |r.y|_prev < tol break and ok(k, |r|^2), all in one block reading the
+// partials in block order (deterministic), and the progress word the host
+// polls.
+#pragma once
+
+#include "device_common.hpp"
+#include "scene.hpp"
+
+namespace prtcl_amd {
+namespace {
+
+// per-block partial sums of up to three FP64 dot products
+template <int K>
+__device__ __forceinline__ void store_partials(double *partials, unsigned nblocks,
+                                               const double (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double s = block_sum<kBlock>(v[k]);
+    if (threadIdx.x == 0)
+      partials[k * nblocks + blockIdx.x] = s;
+  }
+}
+
+__device__ __forceinline__ double sum_partials(const double *p, unsigned n) {
+  double acc = 0.0;
+  for (unsigned k = threadIdx.x; k < n; k += kBlock)
+    acc += p[k];
+  return block_sum<kBlock>(acc);
+}
+
+__device__ __forceinline__ void publish_cg(Ctl *c) {
+  if (c->host_mirror) {
+    const unsigned long long w = (static_cast<unsigned long long>(c->seq) << 32) |
+                                 (static_cast<unsigned long long>(c->cg_done ? 1u : 0u) << 31) |
+                                 static_cast<unsigned long long>(static_cast<unsigned>(c->cg_iter));
+    __hip_atomic_store(c->host_mirror + 2, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *part,
+                                                         unsigned nb, double tol, int max_k,
+                                                         unsigned seq) {
+  const double bb = sum_partials(part, nb);
+  const double rr = sum_partials(part + nb, nb);
+  const double ry = sum_partials(part + 2 * nb, nb);
+  if (threadIdx.x == 0) {
+    c->cg_tol = tol;
+    c->cg_max_k = max_k;
+    c->cg_b_nsq = bb;
+    // thr = max(tol^2 |b|^2, FLT_MIN)   (cg_openmp.hpp:74)
+    const double thr = tol * tol * bb;
+    c->cg_thr = thr > 1.1754943508222875e-38 ? thr : 1.1754943508222875e-38;
+    c->cg_r_nsq = rr;
+    c->cg_ry = ry;
+    c->cg_iter = 0;
+    c->cg_done = 0;  // ok(0) holds: k < min_k = 1
+    c->seq = seq;
+    publish_cg(c);
+  }
+}
+
+// |p.q| < tol breaks (cg_openmp.hpp:147); alpha = (r.y) / (p.q)
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl *c, const double *part, unsigned nb) {
+  if (c->cg_done)
+    return;
+  const double pq = sum_partials(part, nb);
+  if (threadIdx.x == 0) {
+    c->cg_ry_prev = c->cg_ry;
+    c->cg_pq = pq;
+    if (fabs(pq) < c->cg_tol) {
+      c->cg_done = 1;
+      publish_cg(c);
+    } else {
+      c->cg_alpha = c->cg_ry_prev / pq;
+    }
+  }
+}
+
+// |r.y|_prev < tol breaks (:187); beta; ++k; ok(k, |r|^2) (:100-111)
+__global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, unsigned nb) {
+  if (c->cg_done)
+    return;
+  const double rr = sum_partials(part, nb);
+  const double ry = sum_partials(part + nb, nb);
+  if (threadIdx.x == 0) {
+    c->cg_r_nsq = rr;
+    c->cg_ry = ry;
+    if (fabs(c->cg_ry_prev) < c->cg_tol) {
+      c->cg_done = 1;
+    } else {
+      c->cg_beta = ry / c->cg_ry_prev;
+      const int k = c->cg_iter + 1;
+      c->cg_iter = k;
+      if (k > c->cg_max_k || rr < c->cg_thr)
+        c->cg_done = 1;
+    }
+    publish_cg(c);
+  }
+}
+
+}  // namespace
+}  // namespace prtcl_amd

@@ -1,0 +1,347 @@
+// pt16 vorticity-preserving viscosity (pt16.hpp / pt16.prtcl) on the device.
+// Every pair term walks the 27-cell grid with the reference predicate (the
+// particle itself included, as CopyNeighbors) and is evaluated in the
+// reference's FP32 operation order (this file is built without FMA
+// contraction); matrices are 3x3 column-major, Eigen's fixed-size sums of
+// three terms are x0 + (x1 + x2). The two solves run the CG of
+// cg_openmp.hpp:32-215 once per vector component, with the FP64 device
+// control of cg_ctl.hpp, as the reference's `foreach dimension` loop of
+// scalar CGOpenMP<T> solves (pt16.hpp:420-519, 608-760).
+#include "cg_ctl.hpp"
+#include "exact_walk.hpp"
+#include "pt16.hpp"
+
+namespace prtcl_amd {
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ T comp(const vec4_t<T> &v, int d) {
+  return d == 0 ? v.x : (d == 1 ? v.y : v.z);
+}
+
+template <class T>
+__device__ __forceinline__ void set_comp(vec4_t<T> &v, int d, T x) {
+  if (d == 0)
+    v.x = x;
+  else if (d == 1)
+    v.y = x;
+  else
+    v.z = x;
+}
+
+// ---- pt16.setup
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, T xi) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xv = S.xm[i];
+  const vec4_t<T> vi = S.v[i];
+  const T rhoi = S.rho[i];
+  const uint32_t cell = S.fcell[i];
+  // same-group density and the smoothed density (pt16.prtcl:53-71)
+  T same = 0, prho = 0;
+  nbrs_exact(S, xv, cell, S.fstart, S.xm,
+             [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pj) {
+               same += pj.w * S.K.w(r2);
+             });
+  nbrs_exact(S, xv, cell, S.fstart, S.xm,
+             [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pj) {
+               prho += (pj.w / S.rho0) * S.K.w(r2);
+             });
+  nbrs_exact(S, xv, cell, S.bstart, S.bxV,
+             [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pb) { prho += pb.w * S.K.w(r2); });
+  prho *= S.rho0;
+  // velocity gradient sum_f m_j (v_j - v_i) gradW_ij^T / rho_i (pt16.prtcl:76-88)
+  T vg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  nbrs_exact(S, xv, cell, S.fstart, S.xm,
+             [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
+               const vec4_t<T> vj = S.v[j];
+               const T dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+               const T f = S.K.gradf(r2);
+               const T g[3] = {f * d0, f * d1, f * d2};
+#pragma unroll
+               for (int c = 0; c < 3; ++c)
+#pragma unroll
+                 for (int r = 0; r < 3; ++r)
+                   vg[r + 3 * c] += pj.w * (dv[r] * g[c]);
+             });
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    vg[k] /= rhoi;
+  // spin, expansion and shear rate (pt16.prtcl:90-105)
+  const T div = vg[0] + (vg[4] + vg[8]);
+  T R[9], V[9], Sh[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      R[r + 3 * c] = (vg[r + 3 * c] - vg[c + 3 * r]) / T(2);
+      V[r + 3 * c] = (div / T(3)) * ((r == c) ? T(1) : T(0));
+      Sh[r + 3 * c] = ((vg[r + 3 * c] + vg[c + 3 * r]) / T(2)) - V[r + 3 * c];
+    }
+  // vorticity = 2 (R(2,1), R(0,2), R(1,0)); the target velocity gradient
+  // without the spin: V unit_step_r(0, rho - rho0, -div) + xi S
+  S.om[i] = make4<vec4_t<T>>(T(2) * R[2 + 3 * 1], T(2) * R[0 + 3 * 2], T(2) * R[1 + 3 * 0], T(0));
+  const T step = ((T(0) <= rhoi - S.rho0) || (T(0) <= -div)) ? T(1) : T(0);
+  T *G = S.tvg + 9 * static_cast<size_t>(i);
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    G[k] = V[k] * step + xi * Sh[k];
+  S.psrho[i] = same;
+  S.prho[i] = prho;
+}
+
+// ---- vorticity diffusion rhs
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs(SceneView<T> S, T xi) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xv = S.xm[i];
+  const vec4_t<T> wi = S.om[i];
+  T a0 = 0, a1 = 0, a2 = 0;
+  nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+             [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
+               const vec4_t<T> wj = S.om[j];
+               const T w = S.K.w(r2);
+               a0 += (pj.w * (wi.x - wj.x)) * w;
+               a1 += (pj.w * (wi.y - wj.y)) * w;
+               a2 += (pj.w * (wi.z - wj.z)) * w;
+             });
+  S.orhs[i] = make4<vec4_t<T>>(a0 * xi, a1 * xi, a2 * xi, T(0));
+}
+
+// ---- velocity reconstruction rhs
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xv = S.xm[i];
+  T Gi[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    Gi[k] = S.tvg[9 * static_cast<size_t>(i) + k];
+  T a[3] = {0, 0, 0};
+  nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+             [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
+               const T *Gj = S.tvg + 9 * static_cast<size_t>(j);
+               T M[9];
+#pragma unroll
+               for (int k = 0; k < 9; ++k)
+                 M[k] = Gi[k] + Gj[k];
+               const T mh = pj.w * T(0.5);
+               const T w = S.K.w(r2);
+#pragma unroll
+               for (int r = 0; r < 3; ++r)
+                 a[r] += (mh * (M[r] * d0 + (M[r + 3] * d1 + M[r + 6] * d2))) * w;
+             });
+  S.vrhs[i] = make4<vec4_t<T>>(a[0], a[1], a[2], T(0));
+}
+
+// ---- spin: G += cross_product_matrix(0.5 w) (eigen.hpp:330-339)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_spin(SceneView<T> S) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> w = S.om[i];
+  const T a0 = T(0.5) * w.x, a1 = T(0.5) * w.y, a2 = T(0.5) * w.z;
+  T *G = S.tvg + 9 * static_cast<size_t>(i);
+  // column-major (r, c) at r + 3c: [[0, -a2, a1], [a2, 0, -a0], [-a1, a0, 0]]
+  const T C[9] = {T(0), a2, -a1, -a2, T(0), a0, a1, -a0, T(0)};
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    G[k] += C[k];
+}
+
+// ---- scalar PCG (one component of one system)
+// system diagonal coefficient and preconditioner of the reference:
+//   (A x)_i = c_i x_i - sum_f (m_j W_ij) x_j, c = pt16_same_density (which 0)
+//   or pt16_density (which 1); P y = y / (pt16_same_density_i - m_i W(0))
+template <class T>
+__device__ __forceinline__ T product_s(const SceneView<T> &S, int which, int i, const T *w) {
+#pragma clang fp contract(off)
+  const vec4_t<T> xv = S.xm[i];
+  T out = (which == 0 ? S.psrho[i] : S.prho[i]) * w[i];
+  nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+             [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
+               out -= (pj.w * S.K.w(r2)) * w[j];
+             });
+  return out;
+}
+
+template <class T>
+__device__ __forceinline__ T product_p(const SceneView<T> &S, int i, T r) {
+#pragma clang fp contract(off)
+  return r / (S.psrho[i] - S.xm[i].w * S.K.w(T(0)));
+}
+
+// x = guess, r = A x - b, y = P r, p = -y; partials b.b, r.r, r.y
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, int which, int dim,
+                                                          double *part) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < S.nf) {
+    const vec4_t<T> *guess = which == 0 ? S.om : S.v;
+    const T b = comp<T>(which == 0 ? S.orhs[i] : S.vrhs[i], dim);
+    const T x0 = comp<T>(guess[i], dim);
+    // A x0 over the guess field's component
+    const vec4_t<T> xv = S.xm[i];
+    T ax = (which == 0 ? S.psrho[i] : S.prho[i]) * x0;
+    nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+               [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
+                 ax -= (pj.w * S.K.w(r2)) * comp<T>(guess[j], dim);
+               });
+    const T r = ax - b;
+    const T y = product_p(S, i, r);
+    S.scx[i] = x0;
+    S.scr[i] = r;
+    S.scy[i] = y;
+    S.scp[i] = -y;
+    acc[0] = static_cast<double>(b * b);
+    acc[1] = static_cast<double>(r * r);
+    acc[2] = static_cast<double>(r * y);
+  }
+  store_partials<3>(part, gridDim.x, acc);
+}
+
+// q = A p, partial p.q
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv(SceneView<T> S, int which, double *part) {
+#pragma clang fp contract(off)
+  if (S.ctl->cg_done)
+    return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < S.nf) {
+    const T q = product_s(S, which, i, S.scp);
+    S.scq[i] = q;
+    acc[0] = static_cast<double>(S.scp[i] * q);
+  }
+  store_partials<1>(part, gridDim.x, acc);
+}
+
+// x += alpha p, r += alpha q, y = P r; partials r.r, r.y
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_update(SceneView<T> S, double *part) {
+#pragma clang fp contract(off)
+  if (S.ctl->cg_done)
+    return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < S.nf) {
+    const T alpha = static_cast<T>(S.ctl->cg_alpha);
+    const T x = S.scx[i] + alpha * S.scp[i];
+    const T r = S.scr[i] + alpha * S.scq[i];
+    const T y = product_p(S, i, r);
+    S.scx[i] = x;
+    S.scr[i] = r;
+    S.scy[i] = y;
+    acc[0] = static_cast<double>(r * r);
+    acc[1] = static_cast<double>(r * y);
+  }
+  store_partials<2>(part, gridDim.x, acc);
+}
+
+// p = -y + beta p
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_direction(SceneView<T> S) {
+#pragma clang fp contract(off)
+  if (S.ctl->cg_done)
+    return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const T beta = static_cast<T>(S.ctl->cg_beta);
+  S.scp[i] = -S.scy[i] + beta * S.scp[i];
+}
+
+// apply (pt16.hpp:491-502, 700-717): component dim of the vorticity (which 0)
+// or the velocity (which 1) = x unit_step_l(0, (t - t_birth) - dt_fade) +
+// old unit_step_r(0, dt_fade - (t - t_birth))
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_apply(SceneView<T> S, int which, int dim) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const T tb = S.v[i].w;
+  const T age = S.t - tb;
+  const T on = (T(0) < age - S.fade) ? T(1) : T(0);
+  const T keep = (T(0) <= S.fade - age) ? T(1) : T(0);
+  vec4_t<T> *f = which == 0 ? S.om : S.v;
+  vec4_t<T> u = f[i];
+  set_comp<T>(u, dim, S.scx[i] * on + comp<T>(u, dim) * keep);
+  f[i] = u;
+}
+
+}  // namespace
+
+template <class T>
+void Pt16Ops<T>::setup(hipStream_t st, const SceneView<T> &S, T xi) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pt16_setup<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, xi);
+}
+
+template <class T>
+void Pt16Ops<T>::omega_rhs(hipStream_t st, const SceneView<T> &S, T xi) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pt16_omega_rhs<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S,
+                       xi);
+}
+
+template <class T>
+void Pt16Ops<T>::vrec_rhs(hipStream_t st, const SceneView<T> &S) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pt16_vrec_rhs<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+}
+
+template <class T>
+void Pt16Ops<T>::spin(hipStream_t st, const SceneView<T> &S) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pt16_spin<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_setup(hipStream_t st, const SceneView<T> &S, int which, int dim,
+                          double *part, double tol, int max_k, unsigned seq) {
+  const unsigned nb = grid_blocks(S.nf);
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pt16_cg_setup<T>, dim3(nb), dim3(kBlock), 0, st, S, which, dim, part);
+  hipLaunchKernelGGL(k_cg_setup_fin, dim3(1), dim3(kBlock), 0, st, S.ctl, part, nb, tol, max_k,
+                     seq);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_iteration(hipStream_t st, const SceneView<T> &S, int which, double *part) {
+  if (S.nf <= 0)
+    return;
+  const unsigned nb = grid_blocks(S.nf);
+  const dim3 gr(nb), bl(kBlock);
+  hipLaunchKernelGGL(k_pt16_cg_spmv<T>, gr, bl, 0, st, S, which, part);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(1), bl, 0, st, S.ctl, part, nb);
+  hipLaunchKernelGGL(k_pt16_cg_update<T>, gr, bl, 0, st, S, part);
+  hipLaunchKernelGGL(k_cg_beta, dim3(1), bl, 0, st, S.ctl, part, nb);
+  hipLaunchKernelGGL(k_pt16_cg_direction<T>, gr, bl, 0, st, S);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_apply(hipStream_t st, const SceneView<T> &S, int which, int dim) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pt16_cg_apply<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S,
+                       which, dim);
+}
+
+template struct Pt16Ops<float>;
+template struct Pt16Ops<double>;
+
+}  // namespace prtcl_amd

@@ -1,0 +1,33 @@
+// Launchers of the pt16 vorticity-preserving viscosity kernels (pt16.hip,
+// Peer and Teschner 2016 as pt16.hpp / pt16.prtcl of the reference): the
+// set-up (densities, velocity gradient, vorticity, target velocity
+// gradient), the right-hand sides of the vorticity diffusion and velocity
+// reconstruction systems, the spin term, and the scalar PCG of
+// cg_openmp.hpp:32-215 over one vector component at a time.
+#pragma once
+
+#include "scene.hpp"
+
+namespace prtcl_amd {
+
+template <class T>
+struct Pt16Ops {
+  // pt16.setup (pt16.hpp:213-342); xi = strain_rate_viscosity
+  static void setup(hipStream_t st, const SceneView<T> &S, T xi);
+  // vorticity diffusion rhs: xi sum_f (m_j (w_i - w_j)) W_ij (pt16.hpp:378-418)
+  static void omega_rhs(hipStream_t st, const SceneView<T> &S, T xi);
+  // velocity reconstruction rhs: sum_f ((m_j 0.5) ((G_i + G_j) x_ij)) W_ij
+  // (pt16.hpp:588-606)
+  static void vrec_rhs(hipStream_t st, const SceneView<T> &S);
+  // target velocity gradient += cross_product_matrix(0.5 w) (pt16.hpp:521-535)
+  static void spin(hipStream_t st, const SceneView<T> &S);
+  // which = 0: vorticity diffusion (system diagonal pt16_same_density, guess
+  // and result in vorticity[dim]); which = 1: velocity reconstruction
+  // (diagonal pt16_density, guess and result in velocity[dim])
+  static void cg_setup(hipStream_t st, const SceneView<T> &S, int which, int dim,
+                       double *part, double tol, int max_k, unsigned seq);
+  static void cg_iteration(hipStream_t st, const SceneView<T> &S, int which, double *part);
+  static void cg_apply(hipStream_t st, const SceneView<T> &S, int which, int dim);
+};
+
+}  // namespace prtcl_amd

@@ -94,6 +94,12 @@ struct SceneView {
   V4 *xq;  // (x, y, z, m p / rho^2): the one gather of the pressure-acceleration pass
   T *rho, *p, *pr, *s, *AA, *Ap, *sp;
   T *gc;  // gradient correction L (3x3 column-major per fluid particle)
+  // pt16 (scenes with implicit viscosity 3; empty otherwise): vorticity, the
+  // right-hand sides of its two systems, the target velocity gradient (3x3
+  // column-major), pt16_density, pt16_same_density and the scalar PCG vectors
+  V4 *om, *orhs, *vrhs;
+  T *tvg, *prho, *psrho;
+  T *scx, *scr, *scp, *scq, *scy;
   int32_t *id;
   uint32_t *fcell;
   V4 *bxV;

@@ -16,50 +16,12 @@
 // |r.y|_prev < tol (absolute), stop when |r|^2 < max(tol^2 |b|^2, FLT_MIN),
 // at most max_k + 1 iterations, the returned count is the loop counter.
 #include "wkbb18.hpp"
+#include "cg_ctl.hpp"
+#include "exact_walk.hpp"
 
 namespace prtcl_amd {
 
 namespace {
-
-template <class Fn>
-__device__ __forceinline__ void runs3(const GridDims &g, uint32_t cellc,
-                                      const uint32_t *__restrict__ start, Fn &&fn) {
-  const int cx = static_cast<int>(cellc & 0x7FFu);
-  const int cy = static_cast<int>((cellc >> 11) & 0x7FFu);
-  const int cz = static_cast<int>(cellc >> 22);
-  const int x0 = cx > 0 ? cx - 1 : 0;
-  const int x1 = cx + 1 < g.nx ? cx + 1 : g.nx - 1;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cz + dz;
-    if (z < 0 || z >= g.nz)
-      continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cy + dy;
-      if (y < 0 || y >= g.ny)
-        continue;
-      const uint32_t row = static_cast<uint32_t>(g.nx) *
-                           (static_cast<uint32_t>(y) +
-                            static_cast<uint32_t>(g.ny) * static_cast<uint32_t>(z));
-      fn(start[row + static_cast<uint32_t>(x0)], start[row + static_cast<uint32_t>(x1) + 1u]);
-    }
-  }
-}
-
-// exact neighbour visit (reference predicate), fn(j, d0, d1, d2, r2, pj)
-template <class T, class Fn>
-__device__ __forceinline__ void nbrs_exact(const SceneView<T> &S, const vec4_t<T> &xi,
-                                           uint32_t cellc, const uint32_t *start,
-                                           const vec4_t<T> *__restrict__ pts, Fn &&fn) {
-  runs3(S.grid, cellc, start, [&](uint32_t b, uint32_t e) {
-    for (uint32_t j = b; j < e; ++j) {
-      const vec4_t<T> pj = pts[j];
-      const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-      const T r2 = sqdist_exact(d0, d1, d2);
-      if (r2 < S.thr2)
-        fn(j, d0, d1, d2, r2, pj);
-    }
-  });
-}
 
 template <class T>
 __device__ __forceinline__ T cof3(const T *m, int i, int j) {
@@ -562,18 +524,6 @@ __device__ __forceinline__ vec4_t<T> precond(const T *M, const vec4_t<T> &w) {
   return y;
 }
 
-// per-block partial sums of up to three FP64 dot products
-template <int K>
-__device__ __forceinline__ void store_partials(double *partials, unsigned nblocks,
-                                               const double (&v)[K]) {
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double s = block_sum<kBlock>(v[k]);
-    if (threadIdx.x == 0)
-      partials[k * nblocks + blockIdx.x] = s;
-  }
-}
-
 // setup: x = v (guess), r = A x - b with b = v (the rhs of wkbb18.hpp:296-334
 // is v - dt a_b with a_b = 0 because the boundary velocity is zero),
 // y = P r, p = -y; partials of b.b, r.r, r.y
@@ -607,44 +557,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, 
   store_partials<3>(part, gridDim.x, acc);
 }
 
-__device__ __forceinline__ double sum_partials(const double *p, unsigned n) {
-  double acc = 0.0;
-  for (unsigned k = threadIdx.x; k < n; k += kBlock)
-    acc += p[k];
-  return block_sum<kBlock>(acc);
-}
-
-__device__ __forceinline__ void publish_cg(Ctl *c) {
-  if (c->host_mirror) {
-    const unsigned long long w = (static_cast<unsigned long long>(c->seq) << 32) |
-                                 (static_cast<unsigned long long>(c->cg_done ? 1u : 0u) << 31) |
-                                 static_cast<unsigned long long>(static_cast<unsigned>(c->cg_iter));
-    __hip_atomic_store(c->host_mirror + 2, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *part,
-                                                         unsigned nb, double tol, int max_k,
-                                                         unsigned seq) {
-  const double bb = sum_partials(part, nb);
-  const double rr = sum_partials(part + nb, nb);
-  const double ry = sum_partials(part + 2 * nb, nb);
-  if (threadIdx.x == 0) {
-    c->cg_tol = tol;
-    c->cg_max_k = max_k;
-    c->cg_b_nsq = bb;
-    // thr = max(tol^2 |b|^2, FLT_MIN)   (cg_openmp.hpp:74)
-    const double thr = tol * tol * bb;
-    c->cg_thr = thr > 1.1754943508222875e-38 ? thr : 1.1754943508222875e-38;
-    c->cg_r_nsq = rr;
-    c->cg_ry = ry;
-    c->cg_iter = 0;
-    c->cg_done = 0;  // ok(0) holds: k < min_k = 1
-    c->seq = seq;
-    publish_cg(c);
-  }
-}
-
 // q = A p, partial p.q
 template <class T, bool kExact, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, double *part) {
@@ -663,23 +575,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, d
     acc[0] = static_cast<double>(dot3r<T>(S.cp[i], qv));
   }
   store_partials<1>(part, gridDim.x, acc);
-}
-
-// |p.q| < tol breaks (cg_openmp.hpp:147); alpha = (r.y) / (p.q)
-__global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl *c, const double *part, unsigned nb) {
-  if (c->cg_done)
-    return;
-  const double pq = sum_partials(part, nb);
-  if (threadIdx.x == 0) {
-    c->cg_ry_prev = c->cg_ry;
-    c->cg_pq = pq;
-    if (fabs(pq) < c->cg_tol) {
-      c->cg_done = 1;
-      publish_cg(c);
-    } else {
-      c->cg_alpha = c->cg_ry_prev / pq;
-    }
-  }
 }
 
 // x += alpha p, r += alpha q, y = P r; partials r.r, r.y
@@ -707,28 +602,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(SceneView<T> S, double *pa
     acc[1] = static_cast<double>(dot3r<T>(r, y));
   }
   store_partials<2>(part, gridDim.x, acc);
-}
-
-// |r.y|_prev < tol breaks (:187); beta; ++k; ok(k, |r|^2) (:100-111)
-__global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, unsigned nb) {
-  if (c->cg_done)
-    return;
-  const double rr = sum_partials(part, nb);
-  const double ry = sum_partials(part + nb, nb);
-  if (threadIdx.x == 0) {
-    c->cg_r_nsq = rr;
-    c->cg_ry = ry;
-    if (fabs(c->cg_ry_prev) < c->cg_tol) {
-      c->cg_done = 1;
-    } else {
-      c->cg_beta = ry / c->cg_ry_prev;
-      const int k = c->cg_iter + 1;
-      c->cg_iter = k;
-      if (k > c->cg_max_k || rr < c->cg_thr)
-        c->cg_done = 1;
-    }
-    publish_cg(c);
-  }
 }
 
 // p = -y + beta p

@@ -25,12 +25,18 @@ FIELDS = {
     "mass": 5, "iisph_helper_c": 6, "iisph_source_term": 7,
     "iisph_diagonal_element": 8, "iisph_right_hand_side": 9, "time_of_birth": 10,
     "volume": 11, "wkbb18_velocity_delta": 12, "wkbb18_system_diagonal": 13,
-    "gradient_correction": 14,
+    "gradient_correction": 14, "pt16_normalizer": 15, "pt16_density": 16,
+    "pt16_same_density": 17, "vorticity": 18, "target_velocity_gradient": 19,
+    "pt16_vorticity_diffusion_diagonal": 20, "pt16_vorticity_diffusion_rhs": 21,
+    "pt16_velocity_reconstruction_diagonal": 22, "pt16_velocity_reconstruction_rhs": 23,
 }
 FIELD_COMPONENTS = {0: 3, 1: 3, 2: 3, 3: 1, 4: 1, 5: 1, 6: 3, 7: 1, 8: 1, 9: 1, 10: 1,
-                    11: 1, 12: 3, 13: 9, 14: 9}
+                    11: 1, 12: 3, 13: 9, 14: 9, 15: 1, 16: 1, 17: 1, 18: 3, 19: 9, 20: 1,
+                    21: 3, 22: 1, 23: 3}
 GLOBALS = {"iisph_aprde": 0, "iisph_nprde": 1, "maximum_speed": 2, "wkbb18_iterations": 3,
-           "current_time": 4, "iisph_iterations": 5, "iisph_rel_error": 6}
+           "current_time": 4, "iisph_iterations": 5, "iisph_rel_error": 6,
+           "pt16_maximum_error": 7, "pt16_maximum_iterations": 8, "pt16_iterations": 9,
+           "strain_rate_viscosity": 10}
 PROCS = {
     "neighborhood.update": 0, "neighborhood.permute": 1, "aiast12.compute_volume": 2,
     "density.compute_density": 3, "gravity.initialize_acceleration": 4,
@@ -42,6 +48,8 @@ PROCS = {
     "correction.identity_gradient_correction": 13,
     "correction.compute_gradient_correction": 14,
     "wkbb18_gc.compute_diagonal": 15, "wkbb18_gc.accumulate_acceleration": 16,
+    "pt16.setup": 17, "pt16.solve_vorticity_diffusion": 18, "pt16.vorticity_preservation": 19,
+    "pt16.solve_velocity_reconstruction": 20,
 }
 
 # exported symbols declared in include/prtcl_hip.h
@@ -85,6 +93,11 @@ class SceneDesc(C.Structure):
         ("implicit_viscosity", C.c_int32),
         ("permute_every", C.c_int32),
         ("step_mode", C.c_int32),
+        ("pt16_strain_rate_viscosity", C.c_double),
+        ("pt16_vorticity_maximum_error", C.c_double),
+        ("pt16_velocity_maximum_error", C.c_double),
+        ("pt16_vorticity_maximum_iterations", C.c_int32),
+        ("pt16_velocity_maximum_iterations", C.c_int32),
     ]
 
 
@@ -176,6 +189,11 @@ def desc_for(scene, real_type: int = 0, step_mode: int = 0) -> SceneDesc:
     d.explicit_viscosity = int(scene.explicit_viscosity)
     d.implicit_viscosity = int(scene.implicit_viscosity)
     d.permute_every = scene.permute_every
+    d.pt16_strain_rate_viscosity = float(np.float32(scene.pt16_strain_rate_viscosity))
+    d.pt16_vorticity_maximum_error = float(np.float32(scene.pt16_vorticity_maximum_error))
+    d.pt16_velocity_maximum_error = float(np.float32(scene.pt16_velocity_maximum_error))
+    d.pt16_vorticity_maximum_iterations = int(scene.pt16_vorticity_maximum_iterations)
+    d.pt16_velocity_maximum_iterations = int(scene.pt16_velocity_maximum_iterations)
     return d
 
 

@@ -71,7 +71,13 @@ class Scene:
     explicit_viscosity: bool = False
     fluid_viscosity: float = 0.0
     boundary_viscosity: float = 0.0
-    implicit_viscosity: int = 0  # 1: wkbb18; 2: correction + wkbb18_gc
+    implicit_viscosity: int = 0  # 1: wkbb18; 2: correction + wkbb18_gc; 3: pt16
+    # pt16 settings of prtcl-lua.bunny.lua:150-155
+    pt16_strain_rate_viscosity: float = 0.9
+    pt16_vorticity_maximum_error: float = 0.1
+    pt16_vorticity_maximum_iterations: int = 500
+    pt16_velocity_maximum_error: float = 0.1
+    pt16_velocity_maximum_iterations: int = 500
     wkbb18_maximum_error: float = 1e-4
     wkbb18_maximum_iterations: int = 200
     permute_every: int = 8

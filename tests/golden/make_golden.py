@@ -82,6 +82,35 @@ def main():
     np.savez_compressed(os.path.join(HERE, "tiny_wkbb18_gc.npz"), **gc)
     print("wrote tiny_wkbb18_gc.npz, cg iters", kgcg)
 
+    # pt16 on the same state (velocity v*, strain_rate_viscosity 0.9,
+    # tolerance 1e-4): setup, vorticity diffusion (dense solve and the CG of
+    # cg_openmp.hpp per component), spin, velocity reconstruction
+    xi = 0.9
+    same, prho, om, tvg = T.pt16_setup(V, rho, vs, xi)
+    pre = np.diag(1.0 / (same - m * npref.w(0.0, h)))
+    Ao, Av = T.pt16_matrix(same), T.pt16_matrix(prho)
+    orhs = T.pt16_omega_rhs(om, xi)
+    om_sol, om_cg, om_k = np.zeros((n, 3)), np.zeros((n, 3)), []
+    for d in range(3):
+        om_sol[:, d] = np.linalg.solve(Ao, orhs[:, d])
+        xcg_d, k_d = npref.cg_reference(Ao, orhs[:, d], om[:, d].copy(), pre, 1e-4, 500)
+        om_cg[:, d] = xcg_d
+        om_k.append(k_d)
+    tvg2 = tvg + T.cross_matrix(0.5 * om_cg)
+    vrhs = T.pt16_vrec_rhs(tvg2)
+    v_sol, v_cg, v_k = np.zeros((n, 3)), np.zeros((n, 3)), []
+    for d in range(3):
+        v_sol[:, d] = np.linalg.solve(Av, vrhs[:, d])
+        xcg_d, k_d = npref.cg_reference(Av, vrhs[:, d], vs[:, d].copy(), pre, 1e-4, 500)
+        v_cg[:, d] = xcg_d
+        v_k.append(k_d)
+    pt = dict(xf=xf, xb=xb, v=v, m=m.astype(np.float32), h=h, radius=radius, rho0=rho0, dt=dt,
+              xi=xi, V=V, rho=rho, vstar=vs, same=same, prho=prho, om=om, tvg=tvg,
+              orhs=orhs, om_solution=om_sol, om_cg=om_cg, om_iterations=np.array(om_k),
+              tvg_spin=tvg2, vrhs=vrhs, v_solution=v_sol, v_cg=v_cg, v_iterations=np.array(v_k))
+    np.savez_compressed(os.path.join(HERE, "tiny_pt16.npz"), **pt)
+    print("wrote tiny_pt16.npz, cg iters", om_k, v_k)
+
 
 if __name__ == "__main__":
     main()

@@ -118,6 +118,51 @@ class Tiny:
         aprde = (((Ap - s) / rho0) * (pn > 1e-9)).sum()
         return Ap, pn, aprde
 
+    # ---- pt16.prtcl (Peer & Teschner 2016): setup and its two scalar systems
+    def pt16_setup(self, V, rho, v, xi):
+        """pt16.prtcl:48-109: same-group and smoothed densities, velocity
+        gradient, vorticity and target velocity gradient (without spin)"""
+        m, rho0, h = self.m, self.rho0, self.h
+        wf = w(np.linalg.norm(self.df, axis=1), h)
+        wb = w(np.linalg.norm(self.db, axis=1), h)
+        same = self._sum1(self.fi, m[self.fj] * wf)
+        prho = (self._sum1(self.fi, m[self.fj] / rho0 * wf) + self._sum1(self.bi, V[self.bj] * wb)) * rho0
+        vg = np.zeros((self.nf, 3, 3))
+        np.add.at(vg, self.fi, m[self.fj][:, None, None] *
+                  np.einsum("ta,tb->tab", v[self.fj] - v[self.fi], self.gf))
+        vg /= rho[:, None, None]
+        div = np.trace(vg, axis1=1, axis2=2)
+        R = (vg - vg.transpose(0, 2, 1)) / 2
+        Vm = (div / 3)[:, None, None] * np.eye(3)
+        S = (vg + vg.transpose(0, 2, 1)) / 2 - Vm
+        om = 2 * np.stack([R[:, 2, 1], R[:, 0, 2], R[:, 1, 0]], 1)
+        step = ((rho - rho0 >= 0) | (-div >= 0)).astype(float)
+        tvg = Vm * step[:, None, None] + xi * S
+        return same, prho, om, tvg
+
+    def pt16_matrix(self, diag):
+        """(A x)_i = diag_i x_i - sum_f m_j W_ij x_j over fluid pairs (self included)"""
+        A = np.diag(np.asarray(diag, np.float64))
+        wf = w(np.linalg.norm(self.df, axis=1), self.h)
+        np.add.at(A, (self.fi, self.fj), -self.m[self.fj] * wf)
+        return A
+
+    def pt16_omega_rhs(self, om, xi):
+        wf = w(np.linalg.norm(self.df, axis=1), self.h)
+        return xi * self._sum3(self.fi, (self.m[self.fj] * wf)[:, None] * (om[self.fi] - om[self.fj]))
+
+    def pt16_vrec_rhs(self, tvg):
+        wf = w(np.linalg.norm(self.df, axis=1), self.h)
+        M = tvg[self.fi] + tvg[self.fj]
+        return self._sum3(self.fi, (0.5 * self.m[self.fj] * wf)[:, None] *
+                          np.einsum("tab,tb->ta", M, self.df))
+
+    @staticmethod
+    def cross_matrix(a):
+        z = np.zeros(len(a))
+        return np.stack([np.stack([z, -a[:, 2], a[:, 1]], 1), np.stack([a[:, 2], z, -a[:, 0]], 1),
+                         np.stack([-a[:, 1], a[:, 0], z], 1)], 1)
+
     # ---- correction.prtcl:21-38: L_i = pinv(sum_f (m_j / rho_i) gradW_ij (x_j - x_i)^T)
     def gradient_correction(self, rho, rcond=3 * np.finfo(np.float32).eps):
         n, m = self.nf, self.m
