@@ -15,19 +15,17 @@ namespace prtcl_amd {
 
 namespace {
 
+// component d (0, 1, 2) of the real4 at p, addressed arithmetically (base +
+// 4 d): a select over the vector type's x / y / z accessors inside the
+// neighbour loop was compiled without the address of the d = 2 case
 template <class T>
-__device__ __forceinline__ T comp(const vec4_t<T> &v, int d) {
-  return d == 0 ? v.x : (d == 1 ? v.y : v.z);
+__device__ __forceinline__ T comp(const vec4_t<T> *p, int d) {
+  return reinterpret_cast<const T *>(p)[d];
 }
 
 template <class T>
-__device__ __forceinline__ void set_comp(vec4_t<T> &v, int d, T x) {
-  if (d == 0)
-    v.x = x;
-  else if (d == 1)
-    v.y = x;
-  else
-    v.z = x;
+__device__ __forceinline__ T *comp_ptr(vec4_t<T> *p, int d) {
+  return reinterpret_cast<T *>(p) + d;
 }
 
 // ---- pt16.setup
@@ -192,14 +190,14 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, int wh
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < S.nf) {
     const vec4_t<T> *guess = which == 0 ? S.om : S.v;
-    const T b = comp<T>(which == 0 ? S.orhs[i] : S.vrhs[i], dim);
-    const T x0 = comp<T>(guess[i], dim);
+    const T b = comp<T>((which == 0 ? S.orhs : S.vrhs) + i, dim);
+    const T x0 = comp<T>(guess + i, dim);
     // A x0 over the guess field's component
     const vec4_t<T> xv = S.xm[i];
     T ax = (which == 0 ? S.psrho[i] : S.prho[i]) * x0;
     nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
                [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
-                 ax -= (pj.w * S.K.w(r2)) * comp<T>(guess[j], dim);
+                 ax -= (pj.w * S.K.w(r2)) * comp<T>(guess + j, dim);
                });
     const T r = ax - b;
     const T y = product_p(S, i, r);
@@ -278,10 +276,8 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_apply(SceneView<T> S, int wh
   const T age = S.t - tb;
   const T on = (T(0) < age - S.fade) ? T(1) : T(0);
   const T keep = (T(0) <= S.fade - age) ? T(1) : T(0);
-  vec4_t<T> *f = which == 0 ? S.om : S.v;
-  vec4_t<T> u = f[i];
-  set_comp<T>(u, dim, S.scx[i] * on + comp<T>(u, dim) * keep);
-  f[i] = u;
+  T *f = comp_ptr<T>((which == 0 ? S.om : S.v) + i, dim);
+  *f = S.scx[i] * on + *f * keep;
 }
 
 }  // namespace

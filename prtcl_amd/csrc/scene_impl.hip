@@ -15,12 +15,6 @@
 #include "pt16.hpp"
 
 namespace prtcl_amd {
-// pt16's two solves (scalar CG) on the device: switched off until the fault
-// of their first run is explained (pt16.setup and the CPU restatement stay)
-constexpr bool kPt16SolvesEnabled = false;
-}  // namespace prtcl_amd
-
-namespace prtcl_amd {
 
 void *CtxResources::ensure_staging(size_t bytes) {
   if (bytes > staging_bytes) {
@@ -908,16 +902,6 @@ class SceneImpl final : public SceneBase {
       }
     };
     int r = pf.begin(st, kStageWkbb18Diag);
-    switch (proc) {
-      case PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION:
-      case PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION:
-        if (!kPt16SolvesEnabled)
-          throw std::logic_error(
-              "pt16 solves are switched off on the device: the first run of their "
-              "scalar CG set-up faulted and the cause is not identified");
-        break;
-      default: break;
-    }
     switch (proc) {
       case PRTCL_HIP_P_PT16_SETUP:
         Pt16Ops<T>::setup(st, S, xi);

@@ -1,8 +1,8 @@
 """GPU parity of pt16 (pt16.hpp / pt16.prtcl) through the C ABI: the set-up
-against the FP64 golden fixture. The two solves (scalar CG per component)
-are switched off on the device: their first GPU run faulted in the CG set-up
-and the cause is not identified, so their tests are skipped and the library
-refuses them with an error (checked here before any kernel is launched)."""
+and both scalar CG solves against the FP64 golden fixture, steps against the
+oracle in both step modes, and eviction. The solves run every component
+(x, y, z), the vorticity system first: a solve over the last component is the
+case a component select compiled without its address would fault on."""
 
 import numpy as np
 import pytest
@@ -11,10 +11,6 @@ from prtcl_amd import scenes
 from test_oracle_pt16 import golden, mats, rel, scene_from_golden
 
 pytestmark = pytest.mark.gpu
-
-
-SOLVES_OFF = ("pt16's solves are switched off on the device: the first GPU run of their "
-              "scalar CG set-up faulted and the cause is not identified")
 
 
 def _golden_setup(hip_ctx, g):
@@ -37,22 +33,9 @@ def test_pt16_setup_vs_golden(hip_ctx):
     assert rel(mats(s.download("fluid", "target_velocity_gradient")), g["tvg"]) < 1e-4
 
 
-def test_pt16_solves_are_refused(hip_ctx):
-    """the library raises before launching any kernel of a solve"""
-    from prtcl_amd import hip
-    g = golden()
-    print("case: golden tiny scene, pt16 solves refused", flush=True)
-    s = _golden_setup(hip_ctx, g)
-    for proc in ("pt16.solve_vorticity_diffusion", "pt16.solve_velocity_reconstruction"):
-        with pytest.raises(hip.HipError, match="switched off"):
-            s.run(proc)
-    # the scene stays usable
-    assert rel(s.download("fluid", "vorticity"), g["om"]) < 1e-4
-
-
-@pytest.mark.skip(reason=SOLVES_OFF)
 def test_pt16_solves_vs_golden(hip_ctx):
     g = golden()
+    print("case: golden tiny scene, pt16 solves", flush=True)
     s = _golden_setup(hip_ctx, g)
     s.set_global("pt16_maximum_error", 1e-4)
     s.set_global("pt16_maximum_iterations", 500)
@@ -79,7 +62,6 @@ def pt16_block(nx=8, ny=6, nz=8):
                           pt16_vorticity_maximum_error=1e-4, pt16_velocity_maximum_error=1e-4)
 
 
-@pytest.mark.skip(reason=SOLVES_OFF)
 @pytest.mark.parametrize("step_mode", [1, 0], ids=["procedures", "fused"])
 def test_pt16_steps_vs_oracle(hip_ctx, oracle_lib, step_mode):
     from prtcl_amd import hip
@@ -101,7 +83,6 @@ def test_pt16_steps_vs_oracle(hip_ctx, oracle_lib, step_mode):
         assert err < tol, (field, err)
 
 
-@pytest.mark.skip(reason=SOLVES_OFF)
 def test_pt16_scene_evict_restore(hip_ctx):
     from prtcl_amd import hip
     print("case: pt16 block, evict and restore", flush=True)
