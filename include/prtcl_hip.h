@@ -193,6 +193,10 @@ int prtcl_hip_scene_create(prtcl_hip_ctx *ctx, const prtcl_hip_scene_desc *desc,
 int prtcl_hip_scene_destroy(prtcl_hip_ctx *ctx, int32_t scene);
 /* Device bytes a scene with this description would occupy. */
 uint64_t prtcl_hip_scene_footprint(const prtcl_hip_scene_desc *desc);
+/* The description a scene was created with (a loaded scene: as completed
+ * from its archive). Replaces Group::GetItemCount (group.hpp:59) and reads of
+ * the model's parameter fields. */
+int prtcl_hip_scene_describe(prtcl_hip_ctx *ctx, int32_t scene, prtcl_hip_scene_desc *out);
 
 /* ---- field transfer ----------------------------------------------------
  * Replaces reading/writing VaryingFieldSpan items (varying_field.hpp:25-58)
@@ -255,6 +259,41 @@ int prtcl_hip_copy_neighbors(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
  * pinned by group.test.cpp:110-120). perm has n entries. */
 int prtcl_hip_last_permutation(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
                                uint32_t *perm);
+
+/* ---- model archives: checkpoint and resume --------------------------------
+ * Replaces Model::Save / Model::Load over NativeBinaryArchiveWriter /
+ * NativeBinaryArchiveReader (model.cpp:54-75, util/archive.cpp:4-83), which
+ * the Lua binding exposes as model:save_native_binary(path) and
+ * prtcl.data.model.load_native_binary(path) (prtcl-lua/module_data.cpp:83-95).
+ *
+ * save writes the scene in that format, readable by the reference's
+ * Model::Load and share/python/prtcl_archive.py: the global fields of the
+ * schemes (smoothing_scale, time_step, fade_duration, iisph_relaxation,
+ * gravity, current_time, maximum_speed, iisph_aprde, iisph_nprde), a group
+ * "b" of type "boundary" and a group "f" of type "fluid" (tag "dynamic")
+ * with every varying field the scene holds, in host order, and their
+ * uniform fields (rest_density, dynamic_viscosity, the wkbb18 / pt16 solver
+ * uniforms of a viscous scene). Three extra fields carry the device state
+ * of a bit-identical continuation: global "device_clock" (f64),
+ * "device_steps_since_permute" (s32) and "device_slot" (s32 per particle)
+ * in each group. Works on an evicted scene too. bytes_written may be NULL.
+ *
+ * load creates a scene from an archive: particle counts, real type and the
+ * model's parameters come from the archive, the loop settings (IISPH
+ * tolerance and iterations, explicit viscosity, permute period, step mode,
+ * radius, pt16 per-solve settings) from `settings` (NULL: the defaults of
+ * prtcl_hip_scene_desc_init), the implicit viscosity scheme from the fluid
+ * fields present (wkbb18_system_diagonal: wkbb18, with gradient_correction:
+ * wkbb18_gc, pt16_same_density: pt16), else from `settings`. A scene holds
+ * one group of type "fluid" and at most one of type "boundary"; varying
+ * fields of other names are skipped, fields the scene holds but the archive
+ * lacks stay zero (as new fields of the reference). Errors:
+ * PRTCL_HIP_INVALID_ARGUMENT for a malformed or truncated archive or a field
+ * of another type or item count, PRTCL_HIP_ADMISSION_DENIED over budget. */
+int prtcl_hip_scene_save_archive(prtcl_hip_ctx *ctx, int32_t scene, const char *path,
+                                 uint64_t *bytes_written);
+int prtcl_hip_scene_load_archive(prtcl_hip_ctx *ctx, const char *path,
+                                 const prtcl_hip_scene_desc *settings, int32_t *scene_id);
 
 /* ---- residency and serving ----------------------------------------------
  * No reference counterpart (the reference runs one scene per process from

@@ -39,6 +39,7 @@ SOURCES = {
     "pt16.hip": ["-ffp-contract=off"],
     "scene_impl.hip": [],
     "serve.hip": [],
+    "checkpoint.hip": [],
     "capi.hip": [],
 }
 

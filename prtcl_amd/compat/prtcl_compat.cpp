@@ -8,7 +8,153 @@
 
 namespace prtcl {
 
+// ---- archives ----------------------------------------------------------------
+void NativeBinaryArchiveWriter::Bytes(const void *p, size_t n) {
+  os_->write(static_cast<const char *>(p), static_cast<std::streamsize>(n));
+  if (!*os_)
+    throw std::runtime_error("archive: write failed");
+}
+void NativeBinaryArchiveWriter::SaveSize(size_t value) { Bytes(&value, sizeof(value)); }
+void NativeBinaryArchiveWriter::SaveString(std::string_view value) {
+  SaveSize(value.size());
+  Bytes(value.data(), value.size());
+}
+void NativeBinaryArchiveWriter::SaveValues(size_t n, bool const *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveWriter::SaveValues(size_t n, int32_t const *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveWriter::SaveValues(size_t n, int64_t const *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveWriter::SaveValues(size_t n, float const *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveWriter::SaveValues(size_t n, double const *v) { Bytes(v, n * sizeof(*v)); }
+
+void NativeBinaryArchiveReader::Bytes(void *p, size_t n) {
+  is_->read(static_cast<char *>(p), static_cast<std::streamsize>(n));
+  if (static_cast<size_t>(is_->gcount()) != n)
+    throw std::runtime_error("archive: unexpected end of data");
+}
+size_t NativeBinaryArchiveReader::LoadSize() {
+  size_t v = 0;
+  Bytes(&v, sizeof(v));
+  return v;
+}
+std::string NativeBinaryArchiveReader::LoadString() {
+  const size_t n = LoadSize();
+  std::string out(n, '\0');
+  Bytes(out.data(), n);
+  return out;
+}
+void NativeBinaryArchiveReader::LoadValues(size_t n, bool *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveReader::LoadValues(size_t n, int32_t *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveReader::LoadValues(size_t n, int64_t *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveReader::LoadValues(size_t n, float *v) { Bytes(v, n * sizeof(*v)); }
+void NativeBinaryArchiveReader::LoadValues(size_t n, double *v) { Bytes(v, n * sizeof(*v)); }
+
+// ---- field types ---------------------------------------------------------------
+std::string_view CTypeName(CType c) {
+  switch (c) {
+    case CType::kBoolean: return "b";
+    case CType::kSInt32: return "s32";
+    case CType::kSInt64: return "s64";
+    case CType::kFloat32: return "f32";
+    case CType::kFloat64: return "f64";
+    default: return "INVALID";
+  }
+}
+
+CType CTypeFromString(std::string_view s) {
+  if (s == "b")
+    return CType::kBoolean;
+  if (s == "s32")
+    return CType::kSInt32;
+  if (s == "s64")
+    return CType::kSInt64;
+  if (s == "f32")
+    return CType::kFloat32;
+  if (s == "f64")
+    return CType::kFloat64;
+  return CType::kInvalid;
+}
+
+size_t CTypeBytes(CType c) {
+  switch (c) {
+    case CType::kBoolean: return sizeof(bool);
+    case CType::kSInt32: return 4;
+    case CType::kSInt64: return 8;
+    case CType::kFloat32: return 4;
+    case CType::kFloat64: return 8;
+    default: return 0;
+  }
+}
+
+size_t TensorType::Extent() const {
+  size_t n = 1;
+  for (size_t e : shape)
+    n *= e;
+  return n;
+}
+
+// component type name, then rank and extents (tensor_type.cpp:7-15,
+// shape.cpp:9-20)
+void TensorType::Save(ArchiveWriter &archive) const {
+  archive.SaveString(CTypeName(ctype));
+  archive.SaveSize(shape.size());
+  for (size_t e : shape)
+    archive.SaveSize(e);
+}
+
+void TensorType::Load(ArchiveReader &archive) {
+  ctype = CTypeFromString(archive.LoadString());
+  if (ctype == CType::kInvalid)
+    throw std::runtime_error("archive: invalid component type");
+  shape.resize(archive.LoadSize());
+  for (size_t &e : shape)
+    e = archive.LoadSize();
+}
+
 // ---- fields ----------------------------------------------------------------
+void FieldStore::SaveComponents(ArchiveWriter &archive, size_t count) const {
+  const void *d = bytes_.data();
+  switch (type_.ctype) {
+    case CType::kBoolean: archive.SaveValues(count, static_cast<const bool *>(d)); break;
+    case CType::kSInt32: archive.SaveValues(count, static_cast<const int32_t *>(d)); break;
+    case CType::kSInt64: archive.SaveValues(count, static_cast<const int64_t *>(d)); break;
+    case CType::kFloat32: archive.SaveValues(count, static_cast<const float *>(d)); break;
+    case CType::kFloat64: archive.SaveValues(count, static_cast<const double *>(d)); break;
+    default: throw std::logic_error("field of invalid type");
+  }
+}
+
+void FieldStore::LoadComponents(ArchiveReader &archive, size_t count) {
+  void *d = bytes_.data();
+  switch (type_.ctype) {
+    case CType::kBoolean: archive.LoadValues(count, static_cast<bool *>(d)); break;
+    case CType::kSInt32: archive.LoadValues(count, static_cast<int32_t *>(d)); break;
+    case CType::kSInt64: archive.LoadValues(count, static_cast<int64_t *>(d)); break;
+    case CType::kFloat32: archive.LoadValues(count, static_cast<float *>(d)); break;
+    case CType::kFloat64: archive.LoadValues(count, static_cast<double *>(d)); break;
+    default: throw std::logic_error("field of invalid type");
+  }
+}
+
+void FieldStore::SaveVarying(ArchiveWriter &archive) const {
+  const size_t n = Items();
+  archive.SaveSize(n);
+  if (n > 0)
+    SaveComponents(archive, n * extent_);
+}
+
+void FieldStore::LoadVarying(ArchiveReader &archive) {
+  const size_t n = archive.LoadSize();
+  Resize(n);
+  if (n > 0)
+    LoadComponents(archive, n * extent_);
+}
+
+void FieldStore::SaveUniform(ArchiveWriter &archive) const { SaveComponents(archive, extent_); }
+
+void FieldStore::LoadUniform(ArchiveReader &archive) {
+  Resize(1);
+  LoadComponents(archive, extent_);
+}
+
 void FieldStore::Permute(const size_t *perm, size_t n) {
   const size_t item = elem_ * extent_;
   if (n != Items())
@@ -27,17 +173,38 @@ FieldStore *UniformManager::Find(std::string_view name) const {
   return it == fields_.end() ? nullptr : it->second.get();
 }
 
-FieldStore *UniformManager::Add(std::string_view name, const std::type_info &t, size_t elem,
-                                size_t extent) {
-  auto &slot = fields_[std::string(name)];
-  if (slot) {
-    if (!slot->Same(t, extent))
+FieldStore *UniformManager::Add(std::string_view name, const TensorType &type) {
+  if (type.ctype == CType::kInvalid)
+    throw std::invalid_argument("field of invalid type");
+  auto it = fields_.find(std::string(name));
+  if (it != fields_.end()) {
+    if (!it->second->Same(type))
       throw FieldOfDifferentTypeAlreadyExistsError{};
-    return slot.get();
+    return it->second.get();
   }
-  slot = std::make_unique<FieldStore>(t, elem, extent);
+  auto &slot = fields_[std::string(name)];
+  slot = std::make_unique<FieldStore>(type);
   slot->Resize(1);
   return slot.get();
+}
+
+void UniformManager::Save(ArchiveWriter &archive) const {
+  archive.SaveSize(fields_.size());
+  for (const auto &[name, field] : fields_) {
+    archive.SaveString(name);
+    field->GetType().Save(archive);
+    field->SaveUniform(archive);
+  }
+}
+
+void UniformManager::Load(ArchiveReader &archive) {
+  const size_t n = archive.LoadSize();
+  for (size_t k = 0; k < n; ++k) {
+    const std::string name = archive.LoadString();
+    TensorType type;
+    type.Load(archive);
+    Add(name, type)->LoadUniform(archive);
+  }
 }
 
 std::vector<std::string> UniformManager::GetFieldNames() const {
@@ -52,17 +219,41 @@ FieldStore *VaryingManager::Find(std::string_view name) const {
   return it == fields_.end() ? nullptr : it->second.get();
 }
 
-FieldStore *VaryingManager::Add(std::string_view name, const std::type_info &t, size_t elem,
-                                size_t extent) {
-  auto &slot = fields_[std::string(name)];
-  if (slot) {
-    if (!slot->Same(t, extent))
+FieldStore *VaryingManager::Add(std::string_view name, const TensorType &type) {
+  if (type.ctype == CType::kInvalid)
+    throw std::invalid_argument("field of invalid type");
+  auto it = fields_.find(std::string(name));
+  if (it != fields_.end()) {
+    if (!it->second->Same(type))
       throw FieldOfDifferentTypeAlreadyExistsError{};
-    return slot.get();
+    return it->second.get();
   }
-  slot = std::make_unique<FieldStore>(t, elem, extent);
+  auto &slot = fields_[std::string(name)];
+  slot = std::make_unique<FieldStore>(type);
   slot->Resize(count_);
   return slot.get();
+}
+
+void VaryingManager::Save(ArchiveWriter &archive) const {
+  archive.SaveSize(count_);
+  archive.SaveSize(fields_.size());
+  for (const auto &[name, field] : fields_) {
+    archive.SaveString(name);
+    field->GetType().Save(archive);
+    field->SaveVarying(archive);
+  }
+}
+
+void VaryingManager::Load(ArchiveReader &archive) {
+  count_ = archive.LoadSize();
+  dirty_ = true;
+  const size_t n = archive.LoadSize();
+  for (size_t k = 0; k < n; ++k) {
+    const std::string name = archive.LoadString();
+    TensorType type;
+    type.Load(archive);
+    Add(name, type)->LoadVarying(archive);
+  }
 }
 
 void VaryingManager::ResizeItems(size_t n) {
@@ -87,8 +278,44 @@ std::vector<std::string> VaryingManager::GetFieldNames() const {
   return v;
 }
 
-// ---- model -----------------------------------------------------------------
+// ---- groups and model ------------------------------------------------------
+void Group::Save(ArchiveWriter &archive) const {
+  archive.SaveSize(tags_.size());
+  for (const std::string &tag : tags_)
+    archive.SaveString(tag);
+  varying_.Save(archive);
+  uniform_.Save(archive);
+}
+
+void Group::Load(ArchiveReader &archive) {
+  const size_t n = archive.LoadSize();
+  for (size_t k = 0; k < n; ++k)
+    tags_.insert(archive.LoadString());
+  varying_.Load(archive);
+  uniform_.Load(archive);
+}
+
 Model::~Model() { hip::Forget(this); }
+
+void Model::Save(ArchiveWriter &archive) const {
+  global_.Save(archive);
+  archive.SaveSize(groups_.size());
+  for (const auto &[name, group] : groups_) {
+    archive.SaveString(name);
+    archive.SaveString(group->GetGroupType());
+    group->Save(archive);
+  }
+}
+
+void Model::Load(ArchiveReader &archive) {
+  global_.Load(archive);
+  const size_t n = archive.LoadSize();
+  for (size_t k = 0; k < n; ++k) {
+    const std::string name = archive.LoadString();
+    const std::string type = archive.LoadString();
+    AddGroup(name, type).Load(archive);
+  }
+}
 
 Group &Model::AddGroup(std::string_view name, std::string_view type) {
   if (name.empty())
@@ -224,11 +451,14 @@ bool read_uniform(const UniformManager &u, const char *name, T *out, size_t k = 
   FieldStore *f = u.Find(name);
   if (!f)
     return false;
-  if (f->Same(typeid(float), f->Extent()) && k < f->Extent())
+  if (k >= f->Extent())
+    return false;
+  const CType c = f->GetType().ctype;
+  if (c == CType::kFloat32)
     *out = static_cast<T>(static_cast<const float *>(f->Data())[k]);
-  else if (f->Same(typeid(double), f->Extent()) && k < f->Extent())
+  else if (c == CType::kFloat64)
     *out = static_cast<T>(static_cast<const double *>(f->Data())[k]);
-  else if (f->Same(typeid(int32_t), f->Extent()) && k < f->Extent())
+  else if (c == CType::kSInt32)
     *out = static_cast<T>(static_cast<const int32_t *>(f->Data())[k]);
   else
     return false;
@@ -240,11 +470,12 @@ void write_uniform(const UniformManager &u, const char *name, T v) {
   FieldStore *f = u.Find(name);
   if (!f || f->Extent() != 1)
     return;
-  if (f->Same(typeid(float), 1))
+  const CType c = f->GetType().ctype;
+  if (c == CType::kFloat32)
     *static_cast<float *>(f->Data()) = static_cast<float>(v);
-  else if (f->Same(typeid(double), 1))
+  else if (c == CType::kFloat64)
     *static_cast<double *>(f->Data()) = static_cast<double>(v);
-  else if (f->Same(typeid(int32_t), 1))
+  else if (c == CType::kSInt32)
     *static_cast<int32_t *>(f->Data()) = static_cast<int32_t>(v);
 }
 
@@ -312,7 +543,7 @@ void transfer(Model &m, int32_t scene, bool up) {
     if (!g || g->GetItemCount() == 0)
       continue;
     FieldStore *fs = g->GetVarying().Find(mf.name);
-    if (!fs || !fs->Same(typeid(float), mf.extent))
+    if (!fs || !fs->Is(CType::kFloat32, mf.extent))
       continue;
     const size_t count = g->GetItemCount() * mf.extent;
     if (up)

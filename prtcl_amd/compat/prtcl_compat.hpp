@@ -1,11 +1,16 @@
 // Source-compatible subset of the reference's C++ host interface (Model,
-// Group, field spans, Neighborhood, SchemeBase, SchemeRegistry), with GPU
-// schemes that run every procedure through the C ABI of include/prtcl_hip.h.
+// Group, field spans, archives, Neighborhood, SchemeBase, SchemeRegistry),
+// with GPU schemes that run every procedure through the C ABI of
+// include/prtcl_hip.h.
 //
 // Mirrors: src/prtcl/data/model.hpp:24-120 (Model), data/group.hpp:28-150
 // (Group), data/varying_field.hpp:25-58 and uniform_field.hpp:21-53 (spans),
-// util/neighborhood.hpp:23-42 (Neighborhood), schemes/scheme_base.hpp:18-114
-// (SchemeBase, SchemeRegistry, SchemeRegistration). Differences: spans index
+// data/component_type.cpp, data/shape.cpp and data/tensor_type.cpp (field
+// types), util/archive.hpp:13-73 (archives) with the Save/Load members of
+// model.cpp:54-75, group.cpp:31-59, varying_manager.cpp:21-45 and
+// uniform_manager.cpp:22-41, util/neighborhood.hpp:23-42 (Neighborhood),
+// schemes/scheme_base.hpp:18-114 (SchemeBase, SchemeRegistry,
+// SchemeRegistration). Differences: spans index
 // to a pointer to the item's components (no Eigen maps); range accessors
 // return std::vector copies of the names; errors are the reference's string
 // literals where it throws them ("procedure does not exist", "scheme is not
@@ -22,14 +27,16 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <istream>
 #include <limits>
 #include <map>
 #include <memory>
+#include <ostream>
 #include <set>
 #include <stdexcept>
 #include <string>
 #include <string_view>
-#include <typeinfo>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/prtcl_hip.h"
@@ -51,14 +58,118 @@ struct GroupOfDifferentTypeAlreadyExists : std::logic_error {
 
 enum class GroupIndex : size_t { kNoIndex = std::numeric_limits<size_t>::max() };
 
+// ---- archives (util/archive.hpp:13-73) ------------------------------------
+class ArchiveWriter {
+ public:
+  virtual ~ArchiveWriter() = default;
+  virtual void SaveString(std::string_view value) = 0;
+  virtual void SaveSize(size_t value) = 0;
+  virtual void SaveValues(size_t count, bool const *values) = 0;
+  virtual void SaveValues(size_t count, int32_t const *values) = 0;
+  virtual void SaveValues(size_t count, int64_t const *values) = 0;
+  virtual void SaveValues(size_t count, float const *values) = 0;
+  virtual void SaveValues(size_t count, double const *values) = 0;
+};
+
+class ArchiveReader {
+ public:
+  virtual ~ArchiveReader() = default;
+  virtual std::string LoadString() = 0;
+  virtual size_t LoadSize() = 0;
+  virtual void LoadValues(size_t count, bool *values) = 0;
+  virtual void LoadValues(size_t count, int32_t *values) = 0;
+  virtual void LoadValues(size_t count, int64_t *values) = 0;
+  virtual void LoadValues(size_t count, float *values) = 0;
+  virtual void LoadValues(size_t count, double *values) = 0;
+};
+
+// The reference's native binary format (archive.cpp:4-83): a size is the
+// 8 bytes of a size_t, a string its size then its bytes, values their
+// native bytes one after the other; the machine's byte order throughout.
+// A read past the end of the stream throws std::runtime_error.
+class NativeBinaryArchiveWriter final : public ArchiveWriter {
+ public:
+  explicit NativeBinaryArchiveWriter(std::ostream &ostream) : os_(&ostream) {}
+  void SaveSize(size_t value) override;
+  void SaveString(std::string_view value) override;
+  void SaveValues(size_t count, bool const *values) override;
+  void SaveValues(size_t count, int32_t const *values) override;
+  void SaveValues(size_t count, int64_t const *values) override;
+  void SaveValues(size_t count, float const *values) override;
+  void SaveValues(size_t count, double const *values) override;
+
+ private:
+  void Bytes(const void *p, size_t n);
+  std::ostream *os_;
+};
+
+class NativeBinaryArchiveReader final : public ArchiveReader {
+ public:
+  explicit NativeBinaryArchiveReader(std::istream &istream) : is_(&istream) {}
+  size_t LoadSize() override;
+  std::string LoadString() override;
+  void LoadValues(size_t count, bool *values) override;
+  void LoadValues(size_t count, int32_t *values) override;
+  void LoadValues(size_t count, int64_t *values) override;
+  void LoadValues(size_t count, float *values) override;
+  void LoadValues(size_t count, double *values) override;
+
+ private:
+  void Bytes(void *p, size_t n);
+  std::istream *is_;
+};
+
+// ---- field types (component_type.cpp:8-38, shape.cpp, tensor_type.cpp) ---
+enum class CType { kInvalid, kBoolean, kSInt32, kSInt64, kFloat32, kFloat64 };
+
+template <class T>
+constexpr CType CTypeOf() {
+  if constexpr (std::is_same_v<T, bool>)
+    return CType::kBoolean;
+  else if constexpr (std::is_same_v<T, int32_t>)
+    return CType::kSInt32;
+  else if constexpr (std::is_same_v<T, int64_t>)
+    return CType::kSInt64;
+  else if constexpr (std::is_same_v<T, float>)
+    return CType::kFloat32;
+  else {
+    static_assert(std::is_same_v<T, double>, "fields hold b, s32, s64, f32 or f64 components");
+    return CType::kFloat64;
+  }
+}
+
+// "b", "s32", "s64", "f32", "f64" ("INVALID" otherwise)
+std::string_view CTypeName(CType c);
+// inverse of CTypeName; kInvalid for any other string
+CType CTypeFromString(std::string_view s);
+size_t CTypeBytes(CType c);
+
+// component type and item shape; a scalar has rank 0, a 3-vector shape {3},
+// a 3x3 matrix {3, 3} (items stored column-major, as Eigen)
+struct TensorType {
+  CType ctype = CType::kInvalid;
+  std::vector<size_t> shape;
+  size_t Extent() const;
+  bool operator==(const TensorType &o) const { return ctype == o.ctype && shape == o.shape; }
+  bool operator!=(const TensorType &o) const { return !(*this == o); }
+  void Save(ArchiveWriter &archive) const;
+  void Load(ArchiveReader &archive);
+};
+
+template <class T, size_t... N>
+TensorType TensorTypeOf() {
+  return TensorType{CTypeOf<T>(), {N...}};
+}
+
 // ---- field storage -------------------------------------------------------
 class FieldStore {
  public:
-  FieldStore(const std::type_info &t, size_t elem, size_t extent)
-      : type_(&t), elem_(elem), extent_(extent) {}
-  bool Same(const std::type_info &t, size_t extent) const {
-    return *type_ == t && extent_ == extent;
-  }
+  explicit FieldStore(TensorType type)
+      : type_(std::move(type)), elem_(CTypeBytes(type_.ctype)), extent_(type_.Extent()) {}
+  const TensorType &GetType() const { return type_; }
+  bool Same(const TensorType &t) const { return type_ == t; }
+  // component type and number of components, whatever the shape
+  bool Is(CType c, size_t extent) const { return type_.ctype == c && extent_ == extent; }
   size_t Extent() const { return extent_; }
   size_t ElementBytes() const { return elem_; }
   void *Data() { return bytes_.data(); }
@@ -68,9 +179,17 @@ class FieldStore {
   void Resize(size_t n) { bytes_.resize(n * elem_ * extent_, 0); }
   // gather: new item i = old item perm[i] (varying_manager.hpp:86-102)
   void Permute(const size_t *perm, size_t n);
+  // item count then the components of every item (varying_field.hpp:224-250)
+  void SaveVarying(ArchiveWriter &archive) const;
+  void LoadVarying(ArchiveReader &archive);
+  // the components of the one item (uniform_field.hpp Save/Load)
+  void SaveUniform(ArchiveWriter &archive) const;
+  void LoadUniform(ArchiveReader &archive);
 
  private:
-  const std::type_info *type_;
+  void SaveComponents(ArchiveWriter &archive, size_t count) const;
+  void LoadComponents(ArchiveReader &archive, size_t count);
+  TensorType type_;
   size_t elem_, extent_;
   std::vector<unsigned char> bytes_;
 };
@@ -108,15 +227,28 @@ class UniformManager {
  public:
   template <class T, size_t... N>
   UniformFieldSpan<T, N...> AddFieldImpl(std::string_view name) {
-    return UniformFieldSpan<T, N...>(Add(name, typeid(T), sizeof(T), (size_t(1) * ... * N)));
+    return UniformFieldSpan<T, N...>(Add(name, TensorTypeOf<T, N...>()));
   }
+  template <class T, size_t... N>
+  UniformFieldSpan<T, N...> FieldSpan(std::string_view name) const {
+    FieldStore *f = Find(name);
+    if (!f || !f->Same(TensorTypeOf<T, N...>()))
+      throw std::out_of_range("field does not exist");
+    return UniformFieldSpan<T, N...>(f);
+  }
+  // adds a field of a type known at run time (loading)
+  FieldStore *AddField(std::string_view name, const TensorType &type) { return Add(name, type); }
+  size_t GetFieldCount() const { return fields_.size(); }
   bool HasField(std::string_view name) const { return fields_.count(std::string(name)) != 0; }
   FieldStore *Find(std::string_view name) const;
   void RemoveField(std::string_view name) { fields_.erase(std::string(name)); }
   std::vector<std::string> GetFieldNames() const;
+  // field count, then per field (ascending names) its name, type and value
+  void Save(ArchiveWriter &archive) const;
+  void Load(ArchiveReader &archive);
 
  private:
-  FieldStore *Add(std::string_view name, const std::type_info &t, size_t elem, size_t extent);
+  FieldStore *Add(std::string_view name, const TensorType &type);
   std::map<std::string, std::unique_ptr<FieldStore>> fields_;
 };
 
@@ -124,15 +256,17 @@ class VaryingManager {
  public:
   template <class T, size_t... N>
   VaryingFieldSpan<T, N...> AddFieldImpl(std::string_view name) {
-    return VaryingFieldSpan<T, N...>(Add(name, typeid(T), sizeof(T), (size_t(1) * ... * N)));
+    return VaryingFieldSpan<T, N...>(Add(name, TensorTypeOf<T, N...>()));
   }
   template <class T, size_t... N>
   VaryingFieldSpan<T, N...> FieldSpan(std::string_view name) const {
     FieldStore *f = Find(name);
-    if (!f || !f->Same(typeid(T), (size_t(1) * ... * N)))
+    if (!f || !f->Same(TensorTypeOf<T, N...>()))
       throw std::out_of_range("field does not exist");
     return VaryingFieldSpan<T, N...>(f);
   }
+  FieldStore *AddField(std::string_view name, const TensorType &type) { return Add(name, type); }
+  size_t GetFieldCount() const { return fields_.size(); }
   bool HasField(std::string_view name) const { return fields_.count(std::string(name)) != 0; }
   FieldStore *Find(std::string_view name) const;
   void RemoveField(std::string_view name) { fields_.erase(std::string(name)); }
@@ -142,9 +276,16 @@ class VaryingManager {
   bool IsDirty() const { return dirty_; }
   void SetDirty(bool v) { dirty_ = v; }
   std::vector<std::string> GetFieldNames() const;
+  // item count, field count, then per field (ascending names) its name,
+  // type and values; loading sets the item count, adds the stored fields
+  // (or checks the type of existing ones) and resizes each stored field to
+  // its stored count (fields not in the archive keep their items, as the
+  // reference's VaryingManager::Load)
+  void Save(ArchiveWriter &archive) const;
+  void Load(ArchiveReader &archive);
 
  private:
-  FieldStore *Add(std::string_view name, const std::type_info &t, size_t elem, size_t extent);
+  FieldStore *Add(std::string_view name, const TensorType &type);
   std::map<std::string, std::unique_ptr<FieldStore>> fields_;
   size_t count_ = 0;
   bool dirty_ = true;
@@ -189,6 +330,9 @@ class Group {
   void Permute(const size_t *perm, size_t n) { varying_.PermuteItems(perm, n); }
   bool IsDirty() const { return varying_.IsDirty(); }
   void SetDirty(bool v) { varying_.SetDirty(v); }
+  // tags, varying fields, uniform fields (name and type are the model's)
+  void Save(ArchiveWriter &archive) const;
+  void Load(ArchiveReader &archive);
 
  private:
   Model *model_;
@@ -224,6 +368,12 @@ class Model {
     return global_.AddFieldImpl<T, N...>(name);
   }
   void RemoveGlobalField(std::string_view name) { global_.RemoveField(name); }
+
+  // global fields, then per group (ascending names) its name, type and
+  // contents; loading adds the stored groups (or reuses a group of the same
+  // name and type) and loads into them
+  void Save(ArchiveWriter &archive) const;
+  void Load(ArchiveReader &archive);
 
  private:
   void Reindex();

@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "checkpoint.hpp"
 #include "scene_impl.hpp"
 #include "serve.hpp"
 
@@ -128,6 +129,17 @@ const Named kProcs[] = {
     {"pt16.vorticity_preservation", PRTCL_HIP_P_PT16_VORTICITY_PRESERVATION},
     {"pt16.solve_velocity_reconstruction", PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION},
 };
+
+}  // namespace
+
+const char *prtcl_amd::field_name(int id) {
+  for (const auto &f : kFields)
+    if (f.id == id)
+      return f.name;
+  throw std::out_of_range("field does not exist");
+}
+
+namespace {
 
 bool valid_desc(const prtcl_hip_scene_desc *d) {
   return d && d->struct_size == sizeof(prtcl_hip_scene_desc) && d->n_fluid >= 0 &&
@@ -407,6 +419,59 @@ prtcl_amd::Server::Hooks hooks(prtcl_hip_ctx *ctx) {
   return h;
 }
 }  // namespace
+
+int prtcl_hip_scene_describe(prtcl_hip_ctx *ctx, int32_t scene, prtcl_hip_scene_desc *out) {
+  return guarded(ctx, [&]() -> int {
+    if (!out)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "out is NULL");
+    *out = find(ctx, scene)->desc;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_scene_save_archive(prtcl_hip_ctx *ctx, int32_t scene, const char *path,
+                                 uint64_t *bytes_written) {
+  return guarded(ctx, [&]() -> int {
+    if (!path)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "path is NULL");
+    const uint64_t n = prtcl_amd::save_scene_archive(*find(ctx, scene), ctx->res, path);
+    if (bytes_written)
+      *bytes_written = n;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_scene_load_archive(prtcl_hip_ctx *ctx, const char *path,
+                                 const prtcl_hip_scene_desc *settings, int32_t *scene_id) {
+  return guarded(ctx, [&]() -> int {
+    if (!path || !scene_id)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "path or scene_id is NULL");
+    prtcl_hip_scene_desc d;
+    if (settings) {
+      if (settings->struct_size != sizeof(prtcl_hip_scene_desc))
+        return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid scene description");
+      d = *settings;
+    } else {
+      prtcl_hip_scene_desc_init(&d);
+    }
+    auto parsed = prtcl_amd::parse_scene_archive(path, d);
+    if (!valid_desc(&d))
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT,
+                  "the archive's parameters give an invalid scene description");
+    int32_t id = -1;
+    const int rc = prtcl_hip_scene_create(ctx, &d, &id);
+    if (rc != PRTCL_HIP_OK)
+      return rc;
+    try {
+      prtcl_amd::load_scene_archive(*find(ctx, id), *parsed);
+    } catch (...) {
+      prtcl_hip_scene_destroy(ctx, id);
+      throw;
+    }
+    *scene_id = id;
+    return PRTCL_HIP_OK;
+  });
+}
 
 int prtcl_hip_scene_evict(prtcl_hip_ctx *ctx, int32_t scene) {
   return guarded(ctx, [&]() -> int {

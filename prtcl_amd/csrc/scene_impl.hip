@@ -448,6 +448,86 @@ class SceneImpl final : public SceneBase {
     PRTCL_HIP_CHECK(hipStreamSynchronize(st));
   }
 
+  int field_components(int group, int field) override {
+    ensure_resident();
+    if (group != PRTCL_HIP_FLUID && group != PRTCL_HIP_BOUNDARY)
+      return 0;
+    // the WKBB18 arrays exist in every scene (scratch of other passes), but
+    // only the viscous schemes bind them as fields
+    const int iv = desc.implicit_viscosity;
+    if ((field == PRTCL_HIP_F_WKBB18_VELOCITY_DELTA ||
+         field == PRTCL_HIP_F_WKBB18_SYSTEM_DIAGONAL) && iv != 1 && iv != 2)
+      return 0;
+    if (field == PRTCL_HIP_F_GRADIENT_CORRECTION && iv != 2)
+      return 0;
+    const Loc L = loc(group, field);
+    return L.base ? L.ncomp : 0;
+  }
+
+  void download_async(int group, int field, void *pinned, size_t count) override {
+    ensure_resident();
+    Loc L = loc(group, field);
+    const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
+    if (!L.base)
+      throw std::invalid_argument("field does not exist for this group");
+    if (count != static_cast<size_t>(n) * L.ncomp)
+      throw std::invalid_argument("count does not match the group size");
+    if (n == 0)
+      return;
+    hipStream_t st = res_->stream;
+    // the staging buffer is reused in stream order: this kernel runs after
+    // the previous copy out of it has finished
+    if (res_->staging_bytes < sizeof(T) * count)
+      throw std::logic_error("staging buffer smaller than the field");
+    const int32_t *ids = group == PRTCL_HIP_FLUID ? P_.id : P_.bid;
+    hipLaunchKernelGGL(k_download<T>, dim3(grid_blocks(n)), dim3(kBlock), 0, st,
+                       static_cast<T *>(res_->staging), L.ncomp, L.base, L.stride, L.off, ids, n);
+    PRTCL_HIP_CHECK(hipGetLastError());
+    PRTCL_HIP_CHECK(hipMemcpyAsync(pinned, res_->staging, sizeof(T) * count,
+                                   hipMemcpyDeviceToHost, st));
+  }
+
+  void slots(int group, int32_t *slot_of_host) override {
+    ensure_resident();
+    const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
+    std::vector<int32_t> id(n);
+    if (n > 0) {
+      PRTCL_HIP_CHECK(hipMemcpyAsync(id.data(), group == PRTCL_HIP_FLUID ? P_.id : P_.bid,
+                                     4 * static_cast<size_t>(n), hipMemcpyDeviceToHost,
+                                     res_->stream));
+      PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
+    }
+    for (int s = 0; s < n; ++s)
+      slot_of_host[id[s]] = s;
+  }
+
+  void set_slots(int group, const int32_t *slot_of_host) override {
+    ensure_resident();
+    const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
+    std::vector<int32_t> id(n, -1);
+    for (int h = 0; h < n; ++h) {
+      const int32_t s = slot_of_host[h];
+      if (s < 0 || s >= n || id[s] >= 0)
+        throw std::invalid_argument("slots are not a permutation of the particles");
+      id[s] = h;
+    }
+    if (n > 0) {
+      PRTCL_HIP_CHECK(hipMemcpyAsync(group == PRTCL_HIP_FLUID ? P_.id : P_.bid, id.data(),
+                                     4 * static_cast<size_t>(n), hipMemcpyHostToDevice,
+                                     res_->stream));
+      PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
+    }
+    if (group == PRTCL_HIP_FLUID)
+      fgrid_dirty_ = true;
+    else
+      bgrid_dirty_ = true;
+  }
+
+  double exact_clock() const override { return clock_; }
+  void set_exact_clock(double t) override { clock_ = t; }
+  int steps_since_permute() const override { return steps_since_permute_; }
+  void set_steps_since_permute(int k) override { steps_since_permute_ = k; }
+
   // ------------------------------------------------------------------ globals
   Ctl read_ctl() {
     Ctl c;

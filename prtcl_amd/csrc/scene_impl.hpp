@@ -139,6 +139,26 @@ class SceneBase {
   virtual void evict() = 0;
   virtual void ensure_resident() = 0;
   virtual uint64_t resident_bytes_needed() const = 0;
+
+  // checkpoint support. Components of a field of a group, 0 when the scene
+  // does not hold it.
+  virtual int field_components(int group, int field) = 0;
+  // download() without the synchronisation: the values land in `pinned`
+  // (page-locked host memory) once the context stream reaches this point
+  virtual void download_async(int group, int field, void *pinned, size_t count) = 0;
+  // slot of every particle of a group, in host order
+  virtual void slots(int group, int32_t *slot_of_host) = 0;
+  // puts host particle h in slot slot_of_host[h] (a permutation of 0..n-1);
+  // fields uploaded afterwards land in those slots, and the first neighbour
+  // sort breaks ties in that order, so a scene saved with its slots is
+  // continued bit-identically
+  virtual void set_slots(int group, const int32_t *slot_of_host) = 0;
+  // the step clock (current_time is this value rounded to the real type)
+  // and the steps since the last host-visible permute
+  virtual double exact_clock() const = 0;
+  virtual void set_exact_clock(double t) = 0;
+  virtual int steps_since_permute() const = 0;
+  virtual void set_steps_since_permute(int k) = 0;
 };
 
 SceneBase *make_scene(const prtcl_hip_scene_desc &desc, CtxResources *res);

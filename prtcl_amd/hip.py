@@ -57,6 +57,7 @@ EXPORTS = [
     "prtcl_hip_scene_desc_init", "prtcl_hip_ctx_create", "prtcl_hip_ctx_destroy",
     "prtcl_hip_last_error", "prtcl_hip_ctx_bytes_in_use", "prtcl_hip_synchronize",
     "prtcl_hip_scene_create", "prtcl_hip_scene_destroy", "prtcl_hip_scene_footprint",
+    "prtcl_hip_scene_describe", "prtcl_hip_scene_save_archive", "prtcl_hip_scene_load_archive",
     "prtcl_hip_upload_field", "prtcl_hip_download_field", "prtcl_hip_field_id",
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
     "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
@@ -143,6 +144,10 @@ def lib():
                                             C.POINTER(C.c_double)]
         L.prtcl_hip_step.argtypes = [vp, C.POINTER(i32), i32, i32]
         L.prtcl_hip_set_schedule.argtypes = [vp, i32, i32]
+        L.prtcl_hip_scene_describe.argtypes = [vp, i32, C.POINTER(SceneDesc)]
+        L.prtcl_hip_scene_save_archive.argtypes = [vp, i32, C.c_char_p, C.POINTER(C.c_uint64)]
+        L.prtcl_hip_scene_load_archive.argtypes = [vp, C.c_char_p, C.POINTER(SceneDesc),
+                                                   C.POINTER(i32)]
         L.prtcl_hip_scene_evict.argtypes = [vp, i32]
         L.prtcl_hip_scene_resident.argtypes = [vp, i32, C.POINTER(i32)]
         L.prtcl_hip_serve_make_room.argtypes = [vp, C.c_uint64]
@@ -314,6 +319,43 @@ class HipScene:
             self.upload("fluid", "mass", scene.fluid_m)
             self.upload("fluid", "time_of_birth", scene.time_of_birth)
             self.upload("boundary", "position", scene.boundary_x)
+
+    @classmethod
+    def load_archive(cls, ctx: Context, path, settings=None, step_mode: int = 0):
+        """A scene created from a model archive (prtcl_hip_scene_load_archive).
+        settings: a prtcl_amd.scenes.Scene whose loop settings apply (IISPH
+        tolerance and iterations, viscosity options, permute period), or None
+        for the defaults; counts, real type and the model's parameters come
+        from the archive."""
+        self = cls.__new__(cls)
+        self.ctx, self.L = ctx, ctx.L
+        self.scene = settings
+        if settings is not None:
+            desc = desc_for(settings, 0, step_mode)
+        else:
+            desc = SceneDesc()
+            self.L.prtcl_hip_scene_desc_init(C.byref(desc))
+            desc.step_mode = step_mode
+        sid = C.c_int32()
+        ctx.check(self.L.prtcl_hip_scene_load_archive(ctx.h, os.fsencode(path), C.byref(desc),
+                                                      C.byref(sid)), "scene_load_archive")
+        self.id = sid.value
+        d = SceneDesc()
+        ctx.check(self.L.prtcl_hip_scene_describe(ctx.h, self.id, C.byref(d)), "scene_describe")
+        self.desc = d
+        self.real_type = d.real_type
+        self.dtype = np.float64 if d.real_type == 1 else np.float32
+        self.nf, self.nb = d.n_fluid, d.n_boundary
+        return self
+
+    def save_archive(self, path) -> int:
+        """Writes the scene as a model archive (prtcl_hip_scene_save_archive);
+        returns the bytes written."""
+        n = C.c_uint64()
+        self.ctx.check(self.L.prtcl_hip_scene_save_archive(self.ctx.h, self.id,
+                                                            os.fsencode(path), C.byref(n)),
+                       "scene_save_archive")
+        return n.value
 
     def destroy(self):
         if self.id is not None:

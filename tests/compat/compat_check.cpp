@@ -3,6 +3,12 @@
 //   compat_check cpu             -- model / group / registry semantics, no GPU
 //   compat_check gpu FILE STEPS  -- the example.lua step loop through the GPU
 //                                   schemes vs prtcl_hip_step on the same scene
+//   compat_check archive DIR     -- the reference's archive round trips
+//                                   (model / group / varying_manager /
+//                                   varying_field / uniform_field .test.cpp),
+//                                   DIR/mixed.bin written from a model built
+//                                   here, DIR/from_oracle.bin loaded and saved
+//                                   again as DIR/resaved.bin
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -11,6 +17,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -323,12 +330,227 @@ static int gpu_check(const char *path, int steps) {
   return failures ? 1 : 0;
 }
 
+// ---- archives ------------------------------------------------------------
+// the model of tests/test_compat.py::mixed_model
+static void build_mixed(Model &m) {
+  auto g = m.AddGlobalFieldImpl<float, 3>("gravity");
+  g[0] = 0.0f;
+  g[1] = 0.0f;
+  g[2] = -9.81f;
+  *m.AddGlobalFieldImpl<int32_t>("iisph_nprde") = 7;
+  *m.AddGlobalFieldImpl<float>("smoothing_scale") = 0.025f;
+  *m.AddGlobalFieldImpl<int64_t>("big") = -5;
+  *m.AddGlobalFieldImpl<bool>("flag") = true;
+  *m.AddGlobalFieldImpl<double>("z_f64") = 1.5;
+  Group &f = m.AddGroup("f", "fluid");
+  f.AddTag("visible");
+  f.AddTag("dynamic");
+  f.Resize(4);
+  auto x = f.AddVaryingFieldImpl<float, 3>("position");
+  auto sd = f.AddVaryingFieldImpl<float, 3, 3>("wkbb18_system_diagonal");
+  auto id = f.AddVaryingFieldImpl<int64_t>("id");
+  auto alive = f.AddVaryingFieldImpl<bool>("alive");
+  auto mass = f.AddVaryingFieldImpl<double>("mass");
+  for (int i = 0; i < 4; ++i) {
+    for (int k = 0; k < 3; ++k)
+      x[i][k] = static_cast<float>(10 * k + i);
+    for (int k = 0; k < 9; ++k)
+      sd[i][k] = static_cast<float>(9 * i + k);
+    id[i][0] = 100 + i;
+    alive[i][0] = i % 2 == 0;
+    mass[i][0] = 0.5 * i;
+  }
+  *f.AddUniformFieldImpl<float>("rest_density") = 1000.0f;
+  *f.AddUniformFieldImpl<int32_t>("wkbb18_maximum_iterations") = 100;
+  Group &b = m.AddGroup("b", "boundary");
+  b.AddVaryingFieldImpl<float, 3>("position");
+  m.AddGroup("a_empty", "other");
+}
+
+static int archive_checks(const std::string &dir) {
+  // model.test.cpp:70-103
+  {
+    std::string data;
+    {
+      std::ostringstream os;
+      NativeBinaryArchiveWriter ar{os};
+      Model model;
+      model.AddGroup("a", "a_type");
+      model.AddGroup("b", "b_type");
+      model.AddGlobalFieldImpl<bool, 1, 2>("bx1x2");
+      model.Save(ar);
+      data = os.str();
+    }
+    std::istringstream is{data};
+    NativeBinaryArchiveReader ar{is};
+    Model model;
+    model.Load(ar);
+    CHECK(model.TryGetGroup("a") && model.TryGetGroup("a")->GetGroupType() == "a_type");
+    CHECK(model.TryGetGroup("b") && model.TryGetGroup("b")->GetGroupType() == "b_type");
+    CHECK(model.GetGlobal().GetFieldCount() == 1);
+    CHECK((model.GetGlobal().FieldSpan<bool, 1, 2>("bx1x2").data() != nullptr));
+    CHECK(throws<std::out_of_range>([&] { model.GetGlobal().FieldSpan<bool, 2, 1>("bx1x2"); }));
+  }
+  // varying_manager.test.cpp:87-157 (entry (r, c) of an R x C item at r + R c)
+  {
+    std::string data;
+    {
+      std::ostringstream os;
+      NativeBinaryArchiveWriter ar{os};
+      VaryingManager varying;
+      varying.ResizeItems(5);
+      auto b = varying.AddFieldImpl<bool, 1, 2>("bx1x2");
+      auto s = varying.AddFieldImpl<int32_t, 3, 1>("s32x3x1");
+      auto d = varying.AddFieldImpl<double, 2, 3>("f64x2x3");
+      const bool bv[5] = {true, false, true, false, true};
+      const int sv[5] = {12, 34, 56, 78, 90};
+      for (int i = 0; i < 5; ++i) {
+        b[i][0 + 1 * 1] = bv[i];
+        s[i][2 + 3 * 0] = sv[i];
+        d[i][1 + 2 * 1] = sv[i];
+      }
+      CHECK(varying.GetFieldCount() == 3 && varying.GetItemCount() == 5);
+      varying.Save(ar);
+      data = os.str();
+    }
+    std::istringstream is{data};
+    NativeBinaryArchiveReader ar{is};
+    VaryingManager varying;
+    varying.Load(ar);
+    CHECK(varying.GetFieldCount() == 3 && varying.GetItemCount() == 5);
+    auto b = varying.FieldSpan<bool, 1, 2>("bx1x2");
+    auto s = varying.FieldSpan<int32_t, 3, 1>("s32x3x1");
+    auto d = varying.FieldSpan<double, 2, 3>("f64x2x3");
+    const int sv[5] = {12, 34, 56, 78, 90};
+    for (int i = 0; i < 5; ++i) {
+      CHECK(b[i][1] == (i % 2 == 0));
+      CHECK(s[i][2] == sv[i]);
+      CHECK(d[i][3] == static_cast<double>(sv[i]));
+    }
+  }
+  // group.test.cpp:128-160: loads into a group of another name and type
+  {
+    std::string data;
+    {
+      std::ostringstream os;
+      NativeBinaryArchiveWriter ar{os};
+      Model model;
+      auto &group = model.AddGroup("g", "t");
+      group.AddVaryingFieldImpl<float, 1, 2>("a");
+      group.AddUniformFieldImpl<bool, 1, 2>("b");
+      group.Resize(102);
+      group.Save(ar);
+      data = os.str();
+    }
+    std::istringstream is{data};
+    NativeBinaryArchiveReader ar{is};
+    Model model;
+    auto &group = model.AddGroup("h", "s");
+    group.Load(ar);
+    CHECK(group.GetItemCount() == 102);
+    CHECK((group.GetVarying().FieldSpan<float, 1, 2>("a").GetSize() == 102));
+    CHECK((group.GetUniform().FieldSpan<bool, 1, 2>("b").data() != nullptr));
+  }
+  // varying_field.test.cpp:10-66 and uniform_field.test.cpp:8-31, on the
+  // field storage
+  {
+    std::ostringstream os;
+    NativeBinaryArchiveWriter ar{os};
+    FieldStore e(TensorTypeOf<bool>());
+    e.SaveVarying(ar);
+    FieldStore v(TensorTypeOf<int64_t, 3, 2>());
+    v.Resize(3);
+    static_cast<int64_t *>(v.Data())[0 * 6 + 0] = 123;
+    static_cast<int64_t *>(v.Data())[1 * 6 + 1 + 3 * 1] = 456;
+    static_cast<int64_t *>(v.Data())[2 * 6 + 2] = 789;
+    v.SaveVarying(ar);
+    FieldStore u(TensorTypeOf<int64_t>());
+    u.Resize(1);
+    *static_cast<int64_t *>(u.Data()) = 1234;
+    u.SaveUniform(ar);
+    std::istringstream is{os.str()};
+    NativeBinaryArchiveReader rd{is};
+    FieldStore e2(TensorTypeOf<bool>());
+    e2.LoadVarying(rd);
+    CHECK(e2.Items() == 0);
+    FieldStore v2(TensorTypeOf<int64_t, 3, 2>());
+    v2.LoadVarying(rd);
+    CHECK(v2.Items() == 3);
+    CHECK(static_cast<int64_t *>(v2.Data())[0] == 123);
+    CHECK(static_cast<int64_t *>(v2.Data())[6 + 4] == 456);
+    CHECK(static_cast<int64_t *>(v2.Data())[12 + 2] == 789);
+    FieldStore u2(TensorTypeOf<int64_t>());
+    u2.LoadUniform(rd);
+    CHECK(*static_cast<int64_t *>(u2.Data()) == 1234);
+  }
+  // errors: truncated data, a field of another type, an unknown type name
+  {
+    std::ostringstream os;
+    NativeBinaryArchiveWriter ar{os};
+    Model m;
+    build_mixed(m);
+    m.Save(ar);
+    const std::string full = os.str();
+    for (size_t cut : {size_t(0), size_t(7), full.size() / 2, full.size() - 1}) {
+      std::istringstream is{full.substr(0, cut)};
+      NativeBinaryArchiveReader rd{is};
+      Model m2;
+      CHECK(throws<std::runtime_error>([&] { m2.Load(rd); }));
+    }
+    Model m3;
+    m3.AddGroup("f", "fluid").AddVaryingFieldImpl<double, 3>("position");
+    std::istringstream is{full};
+    NativeBinaryArchiveReader rd{is};
+    CHECK(throws<FieldOfDifferentTypeAlreadyExistsError>([&] { m3.Load(rd); }));
+    Model m4;
+    m4.AddGroup("f", "boundary");
+    std::istringstream is4{full};
+    NativeBinaryArchiveReader rd4{is4};
+    CHECK(throws<GroupOfDifferentTypeAlreadyExists>([&] { m4.Load(rd4); }));
+    CHECK(CTypeFromString("f16") == CType::kInvalid && CTypeFromString("s32") == CType::kSInt32);
+  }
+  // files for the byte comparison with the restatement
+  {
+    Model m;
+    build_mixed(m);
+    std::ofstream out(dir + "/mixed.bin", std::ios::binary);
+    NativeBinaryArchiveWriter ar{out};
+    m.Save(ar);
+  }
+  {
+    std::ifstream in(dir + "/from_oracle.bin", std::ios::binary);
+    if (!in) {
+      std::printf("FAIL missing %s/from_oracle.bin\n", dir.c_str());
+      return 1;
+    }
+    NativeBinaryArchiveReader rd{in};
+    Model m;
+    m.Load(rd);
+    Group *f = m.TryGetGroup("f");
+    CHECK(f && f->HasTag("dynamic") && f->GetItemCount() == 4);
+    if (f) {
+      auto x = f->GetVarying().FieldSpan<float, 3>("position");
+      CHECK(x[3][2] == 23.0f);
+      auto sd = f->GetVarying().FieldSpan<float, 3, 3>("wkbb18_system_diagonal");
+      CHECK(sd[2][5] == 23.0f);
+    }
+    std::ofstream out(dir + "/resaved.bin", std::ios::binary);
+    NativeBinaryArchiveWriter ar{out};
+    m.Save(ar);
+  }
+  if (failures == 0)
+    std::printf("OK\n");
+  return failures == 0 ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
   try {
     if (argc >= 2 && std::string(argv[1]) == "cpu")
       return cpu_checks();
     if (argc >= 4 && std::string(argv[1]) == "gpu")
       return gpu_check(argv[2], std::atoi(argv[3]));
+    if (argc >= 3 && std::string(argv[1]) == "archive")
+      return archive_checks(argv[2]);
   } catch (const std::exception &e) {
     std::printf("FAIL exception: %s\n", e.what());
     return 1;
@@ -336,6 +558,6 @@ int main(int argc, char **argv) {
     std::printf("FAIL exception: %s\n", s);
     return 1;
   }
-  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS\n");
+  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS | archive DIR\n");
   return 2;
 }

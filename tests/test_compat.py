@@ -32,6 +32,52 @@ def test_compat_model_and_registry(tmp_path):
     assert r.stdout.strip().endswith("OK")
 
 
+def mixed_model():
+    """the model compat_check.cpp's build_mixed() builds"""
+    from oracle import archive as A
+    i = np.arange(4)
+    return {
+        "globals": {"gravity": A.field("f32", [3], [0, 0, -9.81]),
+                    "iisph_nprde": A.field("s32", [], [7]),
+                    "smoothing_scale": A.field("f32", [], [0.025]),
+                    "big": A.field("s64", [], [-5]),
+                    "flag": A.field("b", [], [True]),
+                    "z_f64": A.field("f64", [], [1.5])},
+        "groups": {
+            "f": {"type": "fluid", "tags": ["visible", "dynamic"], "items": 4,
+                  "varying": {
+                      "position": A.field("f32", [3], np.stack([i, 10 + i, 20 + i], 1)),
+                      "wkbb18_system_diagonal": A.field("f32", [3, 3],
+                                                        np.arange(36).reshape(4, 9)),
+                      "id": A.field("s64", [], 100 + i),
+                      "alive": A.field("b", [], i % 2 == 0),
+                      "mass": A.field("f64", [], 0.5 * i)},
+                  "uniform": {"rest_density": A.field("f32", [], [1000]),
+                              "wkbb18_maximum_iterations": A.field("s32", [], [100])}},
+            "b": {"type": "boundary", "tags": [], "items": 0,
+                  "varying": {"position": A.field("f32", [3], np.zeros((0, 3)))}},
+            "a_empty": {"type": "other", "tags": [], "items": 0},
+        },
+    }
+
+
+def test_compat_archive_matches_restatement(tmp_path):
+    """Model::Save / Load of the C++ interface: the reference's round-trip
+    tests, and byte equality with the restatement in both directions."""
+    from oracle import archive as A
+    want = A.write_model(mixed_model())
+    (tmp_path / "from_oracle.bin").write_bytes(want)
+    exe = build_check(tmp_path)
+    r = subprocess.run([exe, "archive", str(tmp_path)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
+    assert (tmp_path / "mixed.bin").read_bytes() == want
+    assert (tmp_path / "resaved.bin").read_bytes() == want
+    m = A.read_model(want)
+    assert m["groups"]["f"]["varying"]["wkbb18_system_diagonal"]["data"][2, 5] == 23
+
+
 @pytest.mark.gpu
 def test_compat_step_loop_matches_c_abi(tmp_path):
     from prtcl_amd import scenes
