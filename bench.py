@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dambreak_1m",
                     choices=["dambreak_1m", "dambreak_8k", "viscous_250k", "viscous_gc_250k",
-                             "served_stream"])
+                             "served_stream", "checkpoint_1m"])
     ap.add_argument("--duration", type=float, default=30.0,
                     help="served stream: length of the arrival trace in seconds")
     ap.add_argument("--rate", type=float, default=200.0,
@@ -102,6 +102,9 @@ def main():
         return
     if args.workload == "served_stream":
         bench_serve(args)
+        return
+    if args.workload == "checkpoint_1m":
+        bench_checkpoint(args)
         return
     from prtcl_amd import hip, scenes
 
@@ -229,6 +232,83 @@ def main():
         g.destroy()
         out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
 
+    print(json.dumps(out), flush=True)
+
+
+def bench_checkpoint(args):
+    """Checkpoint and resume of the 1M dam break (model archive of the
+    reference's native binary format): archive bytes per second written by
+    prtcl_hip_scene_save_archive to a file and to /dev/null (the device gather
+    to host order and the copies alone), and read back by
+    prtcl_hip_scene_load_archive into a new scene."""
+    import tempfile
+    from prtcl_amd import hip, scenes
+    sc = scenes.config1()
+    sc.permute_every = 8
+    ctx = hip.Context(0)
+    g = hip.HipScene(ctx, sc)
+    g.prepare()
+    for _ in range(max(1, args.warmup)):
+        g.step(1)
+    ctx.synchronize()
+    reps = max(1, min(args.steps, 10))
+    d = tempfile.mkdtemp(prefix="prtcl_ckpt_")
+    path = os.path.join(d, "scene.bin")
+    nbytes = g.save_archive(path)
+    g.save_archive(os.devnull)
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    t_file = timed(lambda: g.save_archive(path))
+    t_null = timed(lambda: g.save_archive(os.devnull))
+
+    def load():
+        h = hip.HipScene.load_archive(ctx, path, settings=sc)
+        ctx.synchronize()
+        h.destroy()
+
+    t_load = timed(load)
+    out = {
+        "metric": "checkpoint write bytes/s of the 1M dam-break scene (model archive)",
+        "value": round(nbytes / t_file, 1),
+        "unit": "bytes/s",
+        "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_file, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "checkpoint_" + sc.name, "fluid_particles": sc.n_fluid,
+                   "boundary_particles": sc.n_boundary, "archive_bytes": nbytes,
+                   "file": path},
+        "save_ms": round(1e3 * t_file, 3),
+        "save_to_devnull_ms": round(1e3 * t_null, 3),
+        "save_to_devnull_bytes_per_s": round(nbytes / t_null, 1),
+        "load_ms": round(1e3 * t_load, 3),
+        "load_bytes_per_s": round(nbytes / t_load, 1),
+    }
+    if not args.no_cpu_baseline:
+        # the restatement writing the same fields from host arrays
+        from oracle import archive as A
+        m = A.read_model(open(path, "rb").read())
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0) or n == 0:
+            with open(path + ".cpu", "wb") as fh:
+                fh.write(A.write_model(m))
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+        out["cpu_baseline"] = {"value": round(nbytes / dt, 1), "unit": "bytes/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{n} writes of the same archive from host arrays "
+                                         "by oracle/archive.py"}
+        os.remove(path + ".cpu")
+    os.remove(path)
+    os.rmdir(d)
     print(json.dumps(out), flush=True)
 
 
