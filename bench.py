@@ -240,7 +240,8 @@ def bench_checkpoint(args):
     reference's native binary format): archive bytes per second written by
     prtcl_hip_scene_save_archive to a file and to /dev/null (the device gather
     to host order and the copies alone), and read back by
-    prtcl_hip_scene_load_archive into a new scene."""
+    prtcl_hip_scene_load_archive into a new scene; and the fluid group's VTK
+    text (prtcl_hip_scene_save_vtk)."""
     import tempfile
     from prtcl_amd import hip, scenes
     sc = scenes.config1()
@@ -274,6 +275,11 @@ def bench_checkpoint(args):
         h.destroy()
 
     t_load = timed(load)
+    # VTK text of the fluid group (formatted on the device)
+    vpath = os.path.join(d, "f.vtk")
+    vbytes = g.save_vtk("fluid", vpath)
+    t_vtk = timed(lambda: g.save_vtk("fluid", vpath))
+    t_vtk_null = timed(lambda: g.save_vtk("fluid", os.devnull))
     out = {
         "metric": "checkpoint write bytes/s of the 1M dam-break scene (model archive)",
         "value": round(nbytes / t_file, 1),
@@ -290,6 +296,10 @@ def bench_checkpoint(args):
         "save_to_devnull_bytes_per_s": round(nbytes / t_null, 1),
         "load_ms": round(1e3 * t_load, 3),
         "load_bytes_per_s": round(nbytes / t_load, 1),
+        "vtk_fluid_bytes": vbytes,
+        "vtk_fluid_ms": round(1e3 * t_vtk, 3),
+        "vtk_fluid_to_devnull_ms": round(1e3 * t_vtk_null, 3),
+        "vtk_fluid_particles_per_s": round(sc.n_fluid / t_vtk, 1),
     }
     if not args.no_cpu_baseline:
         # the restatement writing the same fields from host arrays
@@ -307,6 +317,19 @@ def bench_checkpoint(args):
                                "sample": f"{n} writes of the same archive from host arrays "
                                          "by oracle/archive.py"}
         os.remove(path + ".cpu")
+        # the restatement of SaveVTK on a bounded sample of the fluid group
+        from oracle import vtk as V
+        k = 20000
+        f = {name: g.download("fluid", name)[:k] for name in
+             ("position", "velocity", "acceleration", "mass", "density", "pressure",
+              "time_of_birth")}
+        t0 = time.perf_counter()
+        V.write_vtk("f", "fluid", f)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline_vtk"] = {"value": round(k / dt, 1), "unit": "particles/s",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"the first {k} fluid particles by oracle/vtk.py"}
+    os.remove(vpath)
     os.remove(path)
     os.rmdir(d)
     print(json.dumps(out), flush=True)

@@ -295,6 +295,19 @@ int prtcl_hip_scene_save_archive(prtcl_hip_ctx *ctx, int32_t scene, const char *
 int prtcl_hip_scene_load_archive(prtcl_hip_ctx *ctx, const char *path,
                                  const prtcl_hip_scene_desc *settings, int32_t *scene_id);
 
+/* ---- VTK output -----------------------------------------------------------
+ * Replaces SaveVTK(std::ostream&, Group const&) (util/save_vtk.cpp:57-129),
+ * bound to Lua as group:save_vtk(path) (prtcl-lua/module_data.cpp:126-131):
+ * writes one group of the scene (PRTCL_HIP_FLUID as "f fluid",
+ * PRTCL_HIP_BOUNDARY as "b boundary") as the same ASCII legacy-VTK POLYDATA
+ * text, byte for byte: POINTS from position, then POINT_DATA velocity and
+ * acceleration (VECTORS), _index (the host index), mass, volume, density,
+ * pressure, time_of_birth (SCALARS) for the fields the group holds, every
+ * real as "%.6f". The numbers are formatted on the device. bytes_written
+ * may be NULL. */
+int prtcl_hip_scene_save_vtk(prtcl_hip_ctx *ctx, int32_t scene, int32_t group, const char *path,
+                             uint64_t *bytes_written);
+
 /* ---- residency and serving ----------------------------------------------
  * No reference counterpart (the reference runs one scene per process from
  * its Lua loop). Device state of a scene can be evicted to host memory and is

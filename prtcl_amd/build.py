@@ -40,6 +40,7 @@ SOURCES = {
     "scene_impl.hip": [],
     "serve.hip": [],
     "checkpoint.hip": [],
+    "vtk.hip": [],
     "capi.hip": [],
 }
 

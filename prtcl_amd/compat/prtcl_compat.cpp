@@ -372,6 +372,78 @@ void Model::SetDirty(bool v) {
     kv.second->SetDirty(v);
 }
 
+// ---- VTK output ----------------------------------------------------------------
+namespace {
+
+// the items of a float / double field of 1, 2 or 3 components, one line
+// each, components separated by a space (save_vtk.cpp:11-53)
+template <class T>
+void vtk_lines(std::ostream &o, const FieldStore &f) {
+  const T *d = static_cast<const T *>(f.Data());
+  const size_t nc = f.Extent();
+  for (size_t i = 0; i < f.Items(); ++i) {
+    for (size_t c = 0; c < nc; ++c) {
+      if (c)
+        o << ' ';
+      o << std::fixed << d[i * nc + c];
+    }
+    o << '\n';
+  }
+}
+
+void vtk_field(std::ostream &o, const VaryingManager &v, const char *name) {
+  const FieldStore *f = v.Find(name);
+  if (!f)
+    throw std::out_of_range("field does not exist");
+  const TensorType &t = f->GetType();
+  const bool shape_ok = t.shape.empty() || (t.shape.size() == 1 && t.shape[0] >= 1 &&
+                                            t.shape[0] <= 3);
+  if (shape_ok && t.ctype == CType::kFloat32)
+    return vtk_lines<float>(o, *f);
+  if (shape_ok && t.ctype == CType::kFloat64)
+    return vtk_lines<double>(o, *f);
+  throw std::logic_error("not implemented");
+}
+
+}  // namespace
+
+void SaveVTK(std::ostream &o, const Group &group) {
+  const VaryingManager &v = group.GetVarying();
+  const size_t n = group.GetItemCount();
+  o << "# vtk DataFile Version 2.0\n";
+  o << group.GetGroupName() << " " << group.GetGroupType() << "\n";
+  o << "ASCII\n";
+  o << "DATASET POLYDATA\n";
+  const auto flags = o.flags();
+  o << "POINTS " << n << " float\n";
+  vtk_field(o, v, "position");
+  o << "POINT_DATA " << n << "\n";
+  auto vectors = [&](const char *name) {
+    if (v.HasField(name)) {
+      o << "VECTORS " << name << " float\n";
+      vtk_field(o, v, name);
+    }
+  };
+  auto scalars = [&](const char *name) {
+    if (v.HasField(name)) {
+      o << "SCALARS " << name << " float 1\n";
+      o << "LOOKUP_TABLE default\n";
+      vtk_field(o, v, name);
+    }
+  };
+  vectors("velocity");
+  vectors("acceleration");
+  o << "SCALARS _index float 1\n";
+  o << "LOOKUP_TABLE default\n";
+  for (size_t i = 0; i < n; ++i)
+    o << std::fixed << i << "\n";
+  for (const char *name : {"mass", "volume", "density", "pressure", "time_of_birth",
+                           "implicit_function"})
+    scalars(name);
+  vectors("implicit_function_gradient");
+  o.flags(flags);
+}
+
 // ---- schemes registry --------------------------------------------------------
 SchemeRegistry &GetSchemeRegistry() {
   static SchemeRegistry registry;

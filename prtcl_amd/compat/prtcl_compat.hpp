@@ -382,6 +382,16 @@ class Model {
   std::vector<Group *> by_index_;
 };
 
+// ---- VTK output (util/save_vtk.hpp:9) --------------------------------------
+// The group as ASCII legacy-VTK POLYDATA: POINTS from "position", then
+// POINT_DATA velocity, acceleration (VECTORS), _index (the item index), mass,
+// volume, density, pressure, time_of_birth, implicit_function (SCALARS) and
+// implicit_function_gradient (VECTORS) for the fields present, reals printed
+// with std::fixed. A present field that is not a float or double scalar or
+// 1-, 2- or 3-vector throws std::logic_error ("not implemented", as the
+// reference's NotImplementedError).
+void SaveVTK(std::ostream &o, const Group &group);
+
 // ---- neighbourhood ---------------------------------------------------------
 class Neighborhood {
  public:

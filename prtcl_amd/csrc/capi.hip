@@ -11,6 +11,7 @@
 #include "checkpoint.hpp"
 #include "scene_impl.hpp"
 #include "serve.hpp"
+#include "vtk.hpp"
 
 using prtcl_amd::CtxResources;
 using prtcl_amd::HipError;
@@ -469,6 +470,18 @@ int prtcl_hip_scene_load_archive(prtcl_hip_ctx *ctx, const char *path,
       throw;
     }
     *scene_id = id;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_scene_save_vtk(prtcl_hip_ctx *ctx, int32_t scene, int32_t group, const char *path,
+                             uint64_t *bytes_written) {
+  return guarded(ctx, [&]() -> int {
+    if (!path)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "path is NULL");
+    const uint64_t n = prtcl_amd::save_scene_vtk(*find(ctx, scene), ctx->res, group, path);
+    if (bytes_written)
+      *bytes_written = n;
     return PRTCL_HIP_OK;
   });
 }

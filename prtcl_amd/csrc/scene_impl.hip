@@ -487,6 +487,22 @@ class SceneImpl final : public SceneBase {
                                    hipMemcpyDeviceToHost, st));
   }
 
+  void gather_device(int group, int field, void *dev, size_t count) override {
+    ensure_resident();
+    Loc L = loc(group, field);
+    const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
+    if (!L.base)
+      throw std::invalid_argument("field does not exist for this group");
+    if (count != static_cast<size_t>(n) * L.ncomp)
+      throw std::invalid_argument("count does not match the group size");
+    if (n == 0)
+      return;
+    const int32_t *ids = group == PRTCL_HIP_FLUID ? P_.id : P_.bid;
+    hipLaunchKernelGGL(k_download<T>, dim3(grid_blocks(n)), dim3(kBlock), 0, res_->stream,
+                       static_cast<T *>(dev), L.ncomp, L.base, L.stride, L.off, ids, n);
+    PRTCL_HIP_CHECK(hipGetLastError());
+  }
+
   void slots(int group, int32_t *slot_of_host) override {
     ensure_resident();
     const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;

@@ -146,6 +146,9 @@ class SceneBase {
   // download() without the synchronisation: the values land in `pinned`
   // (page-locked host memory) once the context stream reaches this point
   virtual void download_async(int group, int field, void *pinned, size_t count) = 0;
+  // the field in host order into device memory `dev` (count values of the
+  // real type), enqueued on the context stream
+  virtual void gather_device(int group, int field, void *dev, size_t count) = 0;
   // slot of every particle of a group, in host order
   virtual void slots(int group, int32_t *slot_of_host) = 0;
   // puts host particle h in slot slot_of_host[h] (a permutation of 0..n-1);

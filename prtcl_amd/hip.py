@@ -58,6 +58,7 @@ EXPORTS = [
     "prtcl_hip_last_error", "prtcl_hip_ctx_bytes_in_use", "prtcl_hip_synchronize",
     "prtcl_hip_scene_create", "prtcl_hip_scene_destroy", "prtcl_hip_scene_footprint",
     "prtcl_hip_scene_describe", "prtcl_hip_scene_save_archive", "prtcl_hip_scene_load_archive",
+    "prtcl_hip_scene_save_vtk",
     "prtcl_hip_upload_field", "prtcl_hip_download_field", "prtcl_hip_field_id",
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
     "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
@@ -148,6 +149,7 @@ def lib():
         L.prtcl_hip_scene_save_archive.argtypes = [vp, i32, C.c_char_p, C.POINTER(C.c_uint64)]
         L.prtcl_hip_scene_load_archive.argtypes = [vp, C.c_char_p, C.POINTER(SceneDesc),
                                                    C.POINTER(i32)]
+        L.prtcl_hip_scene_save_vtk.argtypes = [vp, i32, i32, C.c_char_p, C.POINTER(C.c_uint64)]
         L.prtcl_hip_scene_evict.argtypes = [vp, i32]
         L.prtcl_hip_scene_resident.argtypes = [vp, i32, C.POINTER(i32)]
         L.prtcl_hip_serve_make_room.argtypes = [vp, C.c_uint64]
@@ -355,6 +357,15 @@ class HipScene:
         self.ctx.check(self.L.prtcl_hip_scene_save_archive(self.ctx.h, self.id,
                                                             os.fsencode(path), C.byref(n)),
                        "scene_save_archive")
+        return n.value
+
+    def save_vtk(self, group, path) -> int:
+        """Writes one group as legacy VTK text (prtcl_hip_scene_save_vtk);
+        returns the bytes written."""
+        n = C.c_uint64()
+        self.ctx.check(self.L.prtcl_hip_scene_save_vtk(self.ctx.h, self.id, self._group(group),
+                                                        os.fsencode(path), C.byref(n)),
+                       "scene_save_vtk")
         return n.value
 
     def destroy(self):

@@ -3,6 +3,11 @@
 //   compat_check cpu             -- model / group / registry semantics, no GPU
 //   compat_check gpu FILE STEPS  -- the example.lua step loop through the GPU
 //                                   schemes vs prtcl_hip_step on the same scene
+//   compat_check vtk IN OUT      -- SaveVTK of a group read from IN (int64
+//                                   real type 0/1, int64 n, then position,
+//                                   velocity, acceleration (3n each), mass,
+//                                   density, pressure, time_of_birth (n each)),
+//                                   written to OUT
 //   compat_check archive DIR     -- the reference's archive round trips
 //                                   (model / group / varying_manager /
 //                                   varying_field / uniform_field .test.cpp),
@@ -543,6 +548,48 @@ static int archive_checks(const std::string &dir) {
   return failures == 0 ? 0 : 1;
 }
 
+// ---- VTK -----------------------------------------------------------------
+template <class T>
+static int vtk_check_t(std::ifstream &in, int64_t n, const std::string &out_path) {
+  Model m;
+  Group &f = m.AddGroup("f", "fluid");
+  f.Resize(static_cast<size_t>(n));
+  const struct {
+    const char *name;
+    int comps;
+  } fields[] = {{"position", 3}, {"velocity", 3}, {"acceleration", 3}, {"mass", 1},
+                {"density", 1},  {"pressure", 1}, {"time_of_birth", 1}};
+  for (const auto &fd : fields) {
+    T *d = fd.comps == 3 ? f.AddVaryingFieldImpl<T, 3>(fd.name).data()
+                         : f.AddVaryingFieldImpl<T>(fd.name).data();
+    in.read(reinterpret_cast<char *>(d), static_cast<std::streamsize>(sizeof(T) * n * fd.comps));
+    if (!in)
+      return 3;
+  }
+  std::ofstream out(out_path, std::ios::binary);
+  SaveVTK(out, f);
+  // a field of another shape is refused, as the reference's NotImplementedError
+  Model m2;
+  Group &g = m2.AddGroup("g", "t");
+  g.Resize(2);
+  g.AddVaryingFieldImpl<T, 3, 3>("position");
+  std::ostringstream sink;
+  CHECK(throws<std::logic_error>([&] { SaveVTK(sink, g); }));
+  if (failures == 0)
+    std::printf("OK\n");
+  return failures == 0 ? 0 : 1;
+}
+
+static int vtk_check(const std::string &in_path, const std::string &out_path) {
+  std::ifstream in(in_path, std::ios::binary);
+  int64_t hdr[2];
+  in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+  if (!in)
+    return 3;
+  return hdr[0] == 1 ? vtk_check_t<double>(in, hdr[1], out_path)
+                     : vtk_check_t<float>(in, hdr[1], out_path);
+}
+
 int main(int argc, char **argv) {
   try {
     if (argc >= 2 && std::string(argv[1]) == "cpu")
@@ -551,6 +598,8 @@ int main(int argc, char **argv) {
       return gpu_check(argv[2], std::atoi(argv[3]));
     if (argc >= 3 && std::string(argv[1]) == "archive")
       return archive_checks(argv[2]);
+    if (argc >= 4 && std::string(argv[1]) == "vtk")
+      return vtk_check(argv[2], argv[3]);
   } catch (const std::exception &e) {
     std::printf("FAIL exception: %s\n", e.what());
     return 1;
@@ -558,6 +607,6 @@ int main(int argc, char **argv) {
     std::printf("FAIL exception: %s\n", s);
     return 1;
   }
-  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS | archive DIR\n");
+  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS | archive DIR | vtk IN OUT\n");
   return 2;
 }
