@@ -503,6 +503,7 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
   S.p[i] = T(0);
   S.pr[i] = T(0);
   S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, T(0));
+  S.xa[S.xa_x(static_cast<size_t>(i))] = xi;
   if (kWithSource) {
     const T src = (rho0 - rho) -
                   S.dt * rho0 * irho * (dsum + rho0 * (vs0 * gb0 + vs1 * gb1 + vs2 * gb2));
@@ -663,7 +664,8 @@ __device__ __forceinline__ void k_accel_list_body(const SceneView<T> &S, const N
     grid_walk(S, i, xi, [&](uint32_t j) { pair(S.xq[j]); });
   const vec4_t<T> ai = make4<vec4_t<T>>(pi * ci.x - K.ag3 * a0, pi * ci.y - K.ag3 * a1,
                                         pi * ci.z - K.ag3 * a2, T(0));
-  S.a[i] = ai;
+  // the pressure pass gathers it from here; the integration copies it to a
+  S.xa[S.xa_a(static_cast<size_t>(i))] = ai;
 }
 
 template <class T, bool kStage>
@@ -692,7 +694,8 @@ __device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, cons
   const vec4_t<T> xi = valid ? S.xm[i] : zero4;
   const uint32_t info = valid ? L.info[i] : kListOverflow;
   // own fields loaded ahead of the walk so their latency overlaps it
-  const vec4_t<T> ai = valid ? S.a[i] : zero4, gs = valid ? S.G[i] : zero4;
+  const vec4_t<T> ai = valid ? S.xa[S.xa_a(static_cast<size_t>(i))] : zero4;
+  const vec4_t<T> gs = valid ? S.G[i] : zero4;
   const T rhoi = valid ? S.rho[i] : T(1), AA = valid ? S.AA[i] : T(0);
   const T si = valid ? S.s[i] : T(0), p_old = valid ? S.p[i] : T(0);
   T sum = 0;
@@ -705,14 +708,17 @@ __device__ __forceinline__ void k_pressure_list_body(const SceneView<T> &S, cons
       L, R, static_cast<size_t>(S.nf), i, info,
       [&](int s) {
         stage_span(R, s, S.xm, sx);
-        stage_span(R, s, S.a, sa);
+        stage_span_fn(R, s, [&](size_t g) { return S.xa[S.xa_a(g)]; }, sa);
       },
-      [&](uint32_t u) { return XA{sx[u], sa[u]}; }, [&](uint32_t j) { return XA{S.xm[j], S.a[j]}; },
+      [&](uint32_t u) { return XA{sx[u], sa[u]}; },
+      [&](uint32_t j) { return XA{S.xa[S.xa_x(j)], S.xa[S.xa_a(j)]}; },
       pair);
   double term = 0.0;
   if (valid) {
     if (info & kListOverflow)
-      grid_walk(S, i, xi, [&](uint32_t j) { pair(XA{S.xm[j], S.a[j]}); });
+      grid_walk(S, i, xi, [&](uint32_t j) {
+        pair(XA{S.xa[S.xa_x(j)], S.xa[S.xa_a(j)]});
+      });
     const T Ap = S.dt * S.dt * S.rho0 / rhoi *
                  ((ai.x * gs.x + ai.y * gs.y + ai.z * gs.z) - K.ag3 * sum);
     const T rcp = (fabs(AA) > T(1e-9)) ? T(1) / AA : T(0);
@@ -783,7 +789,8 @@ __device__ __forceinline__ void k_integrate_body(SceneView<T> S, unsigned lb) {
   bool bad = false;
   if (i < S.nf) {
     vec4_t<T> v = S.v[i];
-    const vec4_t<T> a = S.a[i];
+    const vec4_t<T> a = S.xa[S.xa_a(static_cast<size_t>(i))];
+    S.a[i] = a;  // the acceleration field after the step, as the reference's
     const T H = fade_step(S, v.w);
     v.x += S.dt * a.x * H;
     v.y += S.dt * a.y * H;
@@ -923,6 +930,20 @@ void FusedOps<T>::pressure(hipStream_t st, const SceneView<T> &S, const NbrList 
   else
     hipLaunchKernelGGL((k_pressure_list<T, false>), dim3(nb), dim3(kBlock), 0, st, S, L);
   hipLaunchKernelGGL(k_pressure_finalize, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials, nb);
+}
+
+// a = the acceleration of the last pressure-acceleration pass
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_copy_accel(SceneView<T> S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < S.nf)
+    S.a[i] = S.xa[S.xa_a(static_cast<size_t>(i))];
+}
+
+template <class T>
+void FusedOps<T>::copy_accel(hipStream_t st, const SceneView<T> &S) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_copy_accel<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
 }
 
 template <class T>

@@ -89,9 +89,13 @@ struct FusedOps {
   // density, neighbour list, IISPH c / AA / s, integrated velocity into vtmp
   static void setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
                     bool explicit_viscosity, vec4_t<T> *vtmp, bool lds);
+  // the pressure acceleration goes to the second half of S.xa only
   static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
   static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
+  // v += dt a H, x += dt v, with a from S.xa, also stored to S.a
   static void integrate(hipStream_t st, const SceneView<T> &S);
+  // S.a from S.xa (before a procedure kernel reads the acceleration)
+  static void copy_accel(hipStream_t st, const SceneView<T> &S);
   // the same passes over every scene of a batch (pmap: block map)
   static void setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
                           unsigned npblocks, bool lds);

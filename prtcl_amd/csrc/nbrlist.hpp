@@ -72,6 +72,15 @@ __device__ __forceinline__ void stage_span(const Spans &R, int s, const V *__res
     dst[u] = p[u];
 }
 
+// the same with the value of global slot g given by at(g)
+template <class V, class At>
+__device__ __forceinline__ void stage_span_fn(const Spans &R, int s, At &&at, V *dst) {
+  const size_t base = R.base(s);
+  const uint32_t len = R.len(s);
+  for (uint32_t u = threadIdx.x; u < len; u += kBlock)
+    dst[u] = at(base + u);
+}
+
 constexpr uint32_t kChunk = 8;  // list entries per 16-byte chunk
 
 // the 8 entries of chunk g of particle i

@@ -92,6 +92,19 @@ struct SceneView {
   const uint32_t *bstart;
   V4 *xm, *v, *a, *c, *G, *vd;
   V4 *xq;  // (x, y, z, m p / rho^2): the one gather of the pressure-acceleration pass
+  // (x, y, z, m) and (a, 0) records of the pressure pass's gathers, in
+  // groups of kXaGroup particles (their x-records, then their a-records), so
+  // both records of a particle share one cache line: xa_x() / xa_a() give
+  // the index of a record. One particle per group (x and a side by side,
+  // 32 bytes for FP32) measured faster than 4 per group (128-byte groups,
+  // whole 64-byte segments per pass): the pressure pass gains more from the
+  // adjacent pair than the acceleration pass loses on half-segment writes.
+  V4 *xa;
+  static constexpr size_t kXaGroup = 1;
+  __host__ __device__ static size_t xa_x(size_t j) {
+    return (j / kXaGroup) * (2 * kXaGroup) + j % kXaGroup;
+  }
+  __host__ __device__ static size_t xa_a(size_t j) { return xa_x(j) + kXaGroup; }
   T *rho, *p, *pr, *s, *AA, *Ap, *sp;
   T *gc;  // gradient correction L (3x3 column-major per fluid particle)
   // pt16 (scenes with implicit viscosity 3; empty otherwise): vorticity, the

@@ -787,6 +787,7 @@ class SceneImpl final : public SceneBase {
     S.G = P_.G;
     S.vd = P_.vd;
     S.xq = P_.xq;
+    S.xa = P_.xa;
     S.rho = P_.rho;
     S.p = P_.p;
     S.pr = P_.pr;
@@ -1250,6 +1251,8 @@ class SceneImpl final : public SceneBase {
         FusedOps<T>::integrate(st, S);
         pf.end(st, r);
       } else {
+        if (fused)
+          FusedOps<T>::copy_accel(st, S);
         r = pf.begin(st, kStageGravityIntegrateV);
         SphOps<T>::integrate_v(st, S);
         pf.end(st, r);
@@ -1667,7 +1670,7 @@ class SceneImpl final : public SceneBase {
 
  private:
   struct Ptrs {
-    V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq;
+    V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq, *xa;
     T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp, *gc;
     V4 *om, *orhs, *vrhs;
     T *tvg, *prho, *psrho, *pone, *odiag, *vdiag, *scx, *scr, *scp, *scq, *scy;
@@ -1698,6 +1701,7 @@ class SceneImpl final : public SceneBase {
     // xm and xq swap roles every step; both carry the run padding
     for (V4 **q : {&P.xm, &P.xq})
       A.take(*q, f + kRunPad);
+    A.take(P.xa, SceneView<T>::xa_x(f - 1) + 1 + SceneView<T>::kXaGroup);
     for (T **q : {&P.rho, &P.p, &P.pr, &P.s, &P.AA, &P.Ap, &P.tmpT})
       A.take(*q, f);
     A.take(P.sp, 9 * f);
@@ -1736,7 +1740,7 @@ class SceneImpl final : public SceneBase {
   // every device pointer into the arena (resp. the grid memory)
   template <class F>
   void each_arena_ptr(F &&f) {
-    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
+    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.xa, &P_.bxV,
                    &P_.btmpV4, &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb, &P_.om,
                    &P_.orhs, &P_.vrhs})
       f(reinterpret_cast<void **>(q));
@@ -1779,7 +1783,7 @@ class SceneImpl final : public SceneBase {
     bytes = A.off;
     arena_bytes_ = A.off;
     char *base = static_cast<char *>(arena_);
-    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.bxV,
+    for (V4 **q : {&P_.xm, &P_.v, &P_.a, &P_.c, &P_.G, &P_.vd, &P_.tmpV4, &P_.xq, &P_.xa, &P_.bxV,
                    &P_.btmpV4,
                    &P_.cgx, &P_.cgr, &P_.cgp, &P_.cgq, &P_.cgy, &P_.cgb, &P_.om, &P_.orhs,
                    &P_.vrhs})
