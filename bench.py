@@ -280,6 +280,10 @@ def bench_checkpoint(args):
     vbytes = g.save_vtk("fluid", vpath)
     t_vtk = timed(lambda: g.save_vtk("fluid", vpath))
     t_vtk_null = timed(lambda: g.save_vtk("fluid", os.devnull))
+    # sphere tracing of the fluid at the scripts' 800 x 600 sensor
+    cam_args = ((2.5, -3.0, 1.6), (-1.9, 3.5, -1.2), (0, 0, 1), 1.0, 800, 600)
+    img, par = g.trace(hip.camera(*cam_args), "fluid")
+    t_trace = timed(lambda: g.trace(hip.camera(*cam_args), "fluid"))
     out = {
         "metric": "checkpoint write bytes/s of the 1M dam-break scene (model archive)",
         "value": round(nbytes / t_file, 1),
@@ -300,6 +304,9 @@ def bench_checkpoint(args):
         "vtk_fluid_ms": round(1e3 * t_vtk, 3),
         "vtk_fluid_to_devnull_ms": round(1e3 * t_vtk_null, 3),
         "vtk_fluid_particles_per_s": round(sc.n_fluid / t_vtk, 1),
+        "trace_800x600_ms": round(1e3 * t_trace, 3),
+        "trace_surface_pixels": int(np.count_nonzero(img != 0.3)),
+        "trace_pixels_per_s": round(800 * 600 / t_trace, 1),
     }
     if not args.no_cpu_baseline:
         # the restatement writing the same fields from host arrays
@@ -329,6 +336,18 @@ def bench_checkpoint(args):
         out["cpu_baseline_vtk"] = {"value": round(k / dt, 1), "unit": "particles/s",
                                    "cores": 1, "kind": "port",
                                    "sample": f"the first {k} fluid particles by oracle/vtk.py"}
+        # the tracer restatement on a 16 x 12 sensor with the same field of view
+        from oracle import pyoracle as P
+        xs = g.download("fluid", "position").astype(np.float64)
+        h = float(np.float32(sc.smoothing_scale))
+        small = list(cam_args[:4]) + [16, 12]
+        t0 = time.perf_counter()
+        P.sphere_trace(xs, h, P.camera(*small))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline_trace"] = {"value": round(16 * 12 / dt, 1), "unit": "pixels/s",
+                                     "cores": 1, "kind": "port",
+                                     "sample": "a 16 x 12 sensor (same camera and scene) "
+                                               "by oracle/trace_ref.cpp"}
     os.remove(vpath)
     os.remove(path)
     os.rmdir(d)

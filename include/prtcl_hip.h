@@ -308,6 +308,28 @@ int prtcl_hip_scene_load_archive(prtcl_hip_ctx *ctx, const char *path,
 int prtcl_hip_scene_save_vtk(prtcl_hip_ctx *ctx, int32_t scene, int32_t group, const char *path,
                              uint64_t *bytes_written);
 
+/* ---- sphere tracing --------------------------------------------------------
+ * Replaces SphereTracer<PinholeCamera>::Trace(model) (util/sphere_tracer.hpp:
+ * 46-268), bound to Lua as prtcl.util.sphere_tracer.new(camera):trace(model)
+ * (prtcl-lua/module_util.cpp:162-175), for one group of a scene (the
+ * tracer's "visible" group). The camera is the reference's PinholeCamera
+ * (geometry/pinhole_camera.hpp:19-29). Writes width * height doubles to
+ * image (and, if not NULL, the ray parameter of each pixel to parameter),
+ * row iy at [iy * width + ix]: 0 where the march ran out of max_steps, 0.3
+ * where the ray left the scene, else -dot(ray direction, surface normal).
+ * threshold is SphereTracer::SetThreshold (0.5 by default), max_steps
+ * SetMaxSteps (300). Double precision on the device. */
+typedef struct prtcl_hip_camera {
+  double origin[3];
+  double principal[3];
+  double up[3];
+  double focal_length;
+  int64_t width, height; /* sensor, pixels */
+} prtcl_hip_camera;
+int prtcl_hip_scene_trace(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                          const prtcl_hip_camera *camera, double threshold, int32_t max_steps,
+                          double *image, double *parameter);
+
 /* ---- residency and serving ----------------------------------------------
  * No reference counterpart (the reference runs one scene per process from
  * its Lua loop). Device state of a scene can be evicted to host memory and is

@@ -121,6 +121,23 @@ int64_t oracle_sorted_order(void *scene, int32_t *group, int64_t *index);
 
 int oracle_num_threads(void);
 
+/* Sphere tracer (trace_ref.cpp): PinholeCamera fields (pinhole_camera.hpp:19-29). */
+typedef struct oracle_camera {
+  double origin[3];
+  double principal[3];
+  double up[3];
+  double focal_length;
+  int64_t width, height;
+} oracle_camera;
+/* 1/(4 pi) of the cubic spline kernel in 3 dimensions, as the reference rounds it. */
+double oracle_spline_alpha3(void);
+/* SphereTracer::Trace over the n positions x (double [n][3]) of the visible
+ * particles: image and the ray parameter (may be NULL) per pixel, row iy at
+ * [iy * width + ix]. Returns 0, or 1 on invalid arguments. */
+int oracle_sphere_trace(const double *x, int64_t n, double smoothing_scale,
+                        const oracle_camera *cam, double threshold, int64_t max_steps,
+                        double *image, double *parameter);
+
 #ifdef __cplusplus
 }
 #endif

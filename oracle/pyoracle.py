@@ -54,6 +54,36 @@ class StepInfo(C.Structure):
     ]
 
 
+class Camera(C.Structure):
+    """PinholeCamera (pinhole_camera.hpp:19-29)."""
+    _fields_ = [("origin", C.c_double * 3), ("principal", C.c_double * 3),
+                ("up", C.c_double * 3), ("focal_length", C.c_double),
+                ("width", C.c_int64), ("height", C.c_int64)]
+
+
+def camera(origin, principal, up, focal_length, width, height) -> Camera:
+    c = Camera()
+    for k in range(3):
+        c.origin[k], c.principal[k], c.up[k] = origin[k], principal[k], up[k]
+    c.focal_length, c.width, c.height = focal_length, width, height
+    return c
+
+
+def sphere_trace(x, smoothing_scale, cam: Camera, threshold=0.5, max_steps=300):
+    """SphereTracer::Trace of the positions x (n, 3): (image, ray parameter),
+    both (height, width) float64."""
+    L = lib()
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    img = np.zeros((cam.height, cam.width), np.float64)
+    par = np.zeros((cam.height, cam.width), np.float64)
+    rc = L.oracle_sphere_trace(x.ctypes.data, len(x), float(smoothing_scale), C.byref(cam),
+                               float(threshold), int(max_steps), img.ctypes.data,
+                               par.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle_sphere_trace: invalid arguments")
+    return img, par
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(LIB_PATH):
         subprocess.check_call(["make", "-s", "-C", _HERE])
@@ -98,6 +128,10 @@ def lib():
         L.oracle_sorted_order.argtypes = [vp, vp, vp]
         L.oracle_sorted_order.restype = i64
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_spline_alpha3.restype = C.c_double
+        L.oracle_sphere_trace.argtypes = [vp, i64, C.c_double, C.POINTER(Camera), C.c_double,
+                                          i64, vp, vp]
+        L.oracle_sphere_trace.restype = C.c_int
         _lib = L
     return _lib
 

@@ -41,6 +41,7 @@ SOURCES = {
     "serve.hip": [],
     "checkpoint.hip": [],
     "vtk.hip": [],
+    "trace.hip": ["-ffp-contract=off"],
     "capi.hip": [],
 }
 

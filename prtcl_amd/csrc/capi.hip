@@ -11,6 +11,7 @@
 #include "checkpoint.hpp"
 #include "scene_impl.hpp"
 #include "serve.hpp"
+#include "trace.hpp"
 #include "vtk.hpp"
 
 using prtcl_amd::CtxResources;
@@ -482,6 +483,18 @@ int prtcl_hip_scene_save_vtk(prtcl_hip_ctx *ctx, int32_t scene, int32_t group, c
     const uint64_t n = prtcl_amd::save_scene_vtk(*find(ctx, scene), ctx->res, group, path);
     if (bytes_written)
       *bytes_written = n;
+    return PRTCL_HIP_OK;
+  });
+}
+
+int prtcl_hip_scene_trace(prtcl_hip_ctx *ctx, int32_t scene, int32_t group,
+                          const prtcl_hip_camera *camera, double threshold, int32_t max_steps,
+                          double *image, double *parameter) {
+  return guarded(ctx, [&]() -> int {
+    if (!camera || !image)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "camera or image is NULL");
+    prtcl_amd::trace_scene(*find(ctx, scene), ctx->res, group, *camera, threshold, max_steps,
+                           image, parameter);
     return PRTCL_HIP_OK;
   });
 }

@@ -58,7 +58,7 @@ EXPORTS = [
     "prtcl_hip_last_error", "prtcl_hip_ctx_bytes_in_use", "prtcl_hip_synchronize",
     "prtcl_hip_scene_create", "prtcl_hip_scene_destroy", "prtcl_hip_scene_footprint",
     "prtcl_hip_scene_describe", "prtcl_hip_scene_save_archive", "prtcl_hip_scene_load_archive",
-    "prtcl_hip_scene_save_vtk",
+    "prtcl_hip_scene_save_vtk", "prtcl_hip_scene_trace",
     "prtcl_hip_upload_field", "prtcl_hip_download_field", "prtcl_hip_field_id",
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
     "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
@@ -69,6 +69,21 @@ EXPORTS = [
     "prtcl_hip_copy_neighbors", "prtcl_hip_last_permutation", "prtcl_hip_profile_enable",
     "prtcl_hip_profile_read",
 ]
+
+
+class Camera(C.Structure):
+    """prtcl_hip_camera: the reference's PinholeCamera."""
+    _fields_ = [("origin", C.c_double * 3), ("principal", C.c_double * 3),
+                ("up", C.c_double * 3), ("focal_length", C.c_double),
+                ("width", C.c_int64), ("height", C.c_int64)]
+
+
+def camera(origin, principal, up, focal_length=1.0, width=800, height=600) -> Camera:
+    c = Camera()
+    for k in range(3):
+        c.origin[k], c.principal[k], c.up[k] = origin[k], principal[k], up[k]
+    c.focal_length, c.width, c.height = focal_length, width, height
+    return c
 
 
 class SceneDesc(C.Structure):
@@ -150,6 +165,8 @@ def lib():
         L.prtcl_hip_scene_load_archive.argtypes = [vp, C.c_char_p, C.POINTER(SceneDesc),
                                                    C.POINTER(i32)]
         L.prtcl_hip_scene_save_vtk.argtypes = [vp, i32, i32, C.c_char_p, C.POINTER(C.c_uint64)]
+        L.prtcl_hip_scene_trace.argtypes = [vp, i32, i32, C.POINTER(Camera), C.c_double, i32,
+                                            vp, vp]
         L.prtcl_hip_scene_evict.argtypes = [vp, i32]
         L.prtcl_hip_scene_resident.argtypes = [vp, i32, C.POINTER(i32)]
         L.prtcl_hip_serve_make_room.argtypes = [vp, C.c_uint64]
@@ -367,6 +384,17 @@ class HipScene:
                                                         os.fsencode(path), C.byref(n)),
                        "scene_save_vtk")
         return n.value
+
+    def trace(self, cam: "Camera", group="fluid", threshold=0.5, max_steps=300):
+        """SphereTracer::Trace of one group (prtcl_hip_scene_trace): (image,
+        ray parameter), both (height, width) float64."""
+        img = np.zeros((cam.height, cam.width), np.float64)
+        par = np.zeros((cam.height, cam.width), np.float64)
+        self.ctx.check(self.L.prtcl_hip_scene_trace(self.ctx.h, self.id, self._group(group),
+                                                     C.byref(cam), float(threshold),
+                                                     int(max_steps), img.ctypes.data,
+                                                     par.ctypes.data), "scene_trace")
+        return img, par
 
     def destroy(self):
         if self.id is not None:
