@@ -13,23 +13,44 @@
 namespace prtcl_amd {
 namespace {
 
-// per-block partial sums of up to three FP64 dot products
+// per-block partial sums of up to three FP64 dot products, stored at the
+// block's logical index b (the partials are summed in logical block order)
 template <int K>
 __device__ __forceinline__ void store_partials(double *partials, unsigned nblocks,
-                                               const double (&v)[K]) {
+                                               const double (&v)[K], unsigned b) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const double s = block_sum<kBlock>(v[k]);
     if (threadIdx.x == 0)
-      partials[k * nblocks + blockIdx.x] = s;
+      partials[k * nblocks + b] = s;
   }
 }
 
+template <int K>
+__device__ __forceinline__ void store_partials(double *partials, unsigned nblocks,
+                                               const double (&v)[K]) {
+  store_partials<K>(partials, nblocks, v, blockIdx.x);
+}
+
+// the sum of n partials in block order (block_sum: valid in thread 0's
+// wavefront only)
 __device__ __forceinline__ double sum_partials(const double *p, unsigned n) {
   double acc = 0.0;
   for (unsigned k = threadIdx.x; k < n; k += kBlock)
     acc += p[k];
   return block_sum<kBlock>(acc);
+}
+
+// the same sum (thread 0's value) in every thread of the block
+__device__ __forceinline__ double sum_partials_all(const double *p, unsigned n) {
+  __shared__ double total;
+  const double s = sum_partials(p, n);
+  if (threadIdx.x == 0)
+    total = s;
+  __syncthreads();
+  const double r = total;
+  __syncthreads();  // a later call writes total again
+  return r;
 }
 
 __device__ __forceinline__ void publish_cg(Ctl *c) {

@@ -26,17 +26,6 @@ namespace prtcl_amd {
 
 namespace {
 
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// works on one contiguous range of cell-ordered particles and its L2 holds
-// that range's neighbourhood.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
-  constexpr unsigned kXcd = 8;
-  const unsigned per = nblocks / kXcd, rem = nblocks % kXcd;
-  const unsigned x = b % kXcd, k = b / kXcd;
-  // XCD x owns logical blocks [x*per + min(x, rem), ...) of length per + (x < rem)
-  return x * per + (x < rem ? x : rem) + k;
-}
-
 template <class T>
 __device__ __forceinline__ T rsqrt_fast(T x);
 template <>

@@ -15,8 +15,12 @@ struct FusedSchedule {
   bool pressure_lds = false;
   // each IISPH iteration is replayed from a captured hipGraph unless eager
   bool eager = false;
+  // the WKBB18 CG iterations launch their one-block alpha and beta kernels
+  // even where the vector kernels can derive the scalars themselves
+  bool cg_split = false;
   int mask() const {
-    return (setup_lds ? 1 : 0) | (accel_lds ? 2 : 0) | (pressure_lds ? 4 : 0) | (eager ? 8 : 0);
+    return (setup_lds ? 1 : 0) | (accel_lds ? 2 : 0) | (pressure_lds ? 4 : 0) | (eager ? 8 : 0) |
+           (cg_split ? 16 : 0);
   }
   static FusedSchedule from_mask(int m) {
     FusedSchedule s;
@@ -24,6 +28,7 @@ struct FusedSchedule {
     s.accel_lds = (m & 2) != 0;
     s.pressure_lds = (m & 4) != 0;
     s.eager = (m & 8) != 0;
+    s.cg_split = (m & 16) != 0;
     return s;
   }
 };

@@ -56,6 +56,7 @@ struct Ctl {
   // PCG scalars (FP64)
   double cg_b_nsq, cg_thr, cg_r_nsq, cg_ry, cg_ry_prev, cg_pq, cg_alpha, cg_beta;
   int cg_iter, cg_done, cg_max_k;
+  int cg_iter_next;  // cg_iter + 1, set beside alpha (read while cg_iter is written)
   double cg_tol;
 };
 
@@ -120,8 +121,11 @@ struct SceneView {
   uint32_t *bcell;
   Ctl *ctl;
   double *partials;  // one per block of the largest fluid launch
-  // PCG vectors (3-vectors in real4)
-  V4 *cx, *cr, *cp, *cq, *cy, *cb;
+  // PCG vectors (3-vectors in real4); the direction p is interleaved with
+  // the (x, m / rho) records the list product gathers beside it:
+  // cxp[2 j] = (x_j, m_j / rho_j), cxp[2 j + 1] = p_j, so both halves of a
+  // neighbour come from one cache line
+  V4 *cx, *cr, *cxp, *cq, *cy, *cb;
 };
 
 struct BlockRanges;
@@ -153,6 +157,18 @@ constexpr uint32_t kBatchLocalMask = (1u << kBatchLocalBits) - 1u;
 
 inline unsigned grid_blocks(long n, int bs = kBlock) {
   return static_cast<unsigned>((n + bs - 1) / bs);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; a kernel whose blocks
+// gather from cell-ordered neighbours remaps its block index so that each
+// XCD works on one contiguous range of particles and its L2 holds that
+// range's neighbourhood (a bijection of [0, nblocks)).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks) {
+  constexpr unsigned kXcd = 8;
+  const unsigned per = nblocks / kXcd, rem = nblocks % kXcd;
+  const unsigned x = b % kXcd, k = b / kXcd;
+  // XCD x owns logical blocks [x*per + min(x, rem), ...) of length per + (x < rem)
+  return x * per + (x < rem ? x : rem) + k;
 }
 
 // host-side error helper

@@ -816,7 +816,7 @@ class SceneImpl final : public SceneBase {
     S.partials = P_.partials;
     S.cx = P_.cgx;
     S.cr = P_.cgr;
-    S.cp = P_.cgp;
+    S.cxp = P_.cgp;
     S.cq = P_.cgq;
     S.cy = P_.cgy;
     S.cb = P_.cgb;
@@ -912,11 +912,17 @@ class SceneImpl final : public SceneBase {
     // takes every decision, the host keeps one iteration queued ahead
     const unsigned seq = ++seq_;
     const int max_k = std::max(0, desc.wkbb18_maximum_iterations);
+    const bool split = tuner_.current().cg_split;
     int r = pf.begin(st, kStagePcgVector);
     Wkbb18Ops<T>::setup(st, S, pl, gc, P_.cg_part, desc.wkbb18_maximum_error, max_k, seq);
     pf.end(st, r);
-    const Progress p = cg_loop(seq, max_k, predicted_cg_, [&] {
-      Wkbb18Ops<T>::iteration(st, S, pl, gc, P_.cg_part);
+    const Progress p = cg_loop(seq, max_k, predicted_cg_, [&](int tag) {
+      int rp = pf.begin(st, kStagePcgSpmv, tag);
+      Wkbb18Ops<T>::iteration_product(st, S, pl, gc, P_.cg_part);
+      pf.end(st, rp);
+      rp = pf.begin(st, kStagePcgVector, tag);
+      Wkbb18Ops<T>::iteration_vectors(st, S, P_.cg_part, split);
+      pf.end(st, rp);
     });
     r = pf.begin(st, kStagePcgVector);
     Wkbb18Ops<T>::apply(st, S);
@@ -935,15 +941,10 @@ class SceneImpl final : public SceneBase {
   // iteration kept queued ahead while the host polls the progress word.
   template <class F>
   auto cg_loop(unsigned seq, int max_k, int predicted, F &&iteration) {
-    hipStream_t st = res_->stream;
-    Profiler &pf = res_->prof;
     int enq = 0;
     auto enqueue = [&](int n) {
-      for (int k = 0; k < n; ++k) {
-        int rr = pf.begin(st, kStagePcgSpmv, enq + k);
-        iteration();
-        pf.end(st, rr);
-      }
+      for (int k = 0; k < n; ++k)
+        iteration(enq + k);  // the iteration's index tags its profiler records
       enq += n;
     };
     Progress p;
@@ -983,8 +984,10 @@ class SceneImpl final : public SceneBase {
         int r = pf.begin(st, kStagePcgVector);
         Pt16Ops<T>::cg_setup(st, S, which, dim, P_.cg_part, pt16_tol_, max_k, seq);
         pf.end(st, r);
-        const Progress p = cg_loop(seq, max_k, predicted_pt16_[which], [&] {
+        const Progress p = cg_loop(seq, max_k, predicted_pt16_[which], [&](int tag) {
+          const int rp = pf.begin(st, kStagePcgSpmv, tag);
           Pt16Ops<T>::cg_iteration(st, S, which, P_.cg_part);
+          pf.end(st, rp);
         });
         r = pf.begin(st, kStagePcgVector);
         Pt16Ops<T>::cg_apply(st, S, which, dim);
@@ -1726,8 +1729,9 @@ class SceneImpl final : public SceneBase {
     A.take(P.cg_part, 3 * static_cast<size_t>(grid_blocks(static_cast<long>(f))) + 3);
     A.take(P.ctl, 1);
     const size_t nv = pcg ? f : 1;
-    for (V4 **q : {&P.cgx, &P.cgr, &P.cgp, &P.cgq, &P.cgy, &P.cgb})
+    for (V4 **q : {&P.cgx, &P.cgr, &P.cgq, &P.cgy, &P.cgb})
       A.take(*q, nv);
+    A.take(P.cgp, 2 * nv);  // interleaved (x, m / rho) and p records
     const size_t np = pt16 ? f : 1;
     for (V4 **q : {&P.om, &P.orhs, &P.vrhs})
       A.take(*q, np);

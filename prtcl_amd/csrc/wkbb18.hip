@@ -168,7 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_corr_identity(SceneView<T> S) {
 template <class T, bool kList>
 __global__ __launch_bounds__(kBlock) void k_corr(SceneView<T> S, NbrList L) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xi = S.xm[i];
@@ -215,7 +216,8 @@ __device__ __forceinline__ void apply_gc(const T *L, const T g[3], T out[3]) {
 template <class T, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xi = S.xm[i];
@@ -277,10 +279,11 @@ __global__ __launch_bounds__(kBlock) void k_diag_exact(SceneView<T> S) {
 // every term is (s L_i) gradW (wkbb18_gc.hpp:362-390)
 template <class T, bool kGC>
 __device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
-                                              const vec4_t<T> *__restrict__ w, T out[3]) {
+                                              const vec4_t<T> *__restrict__ w, int ws,
+                                              T out[3]) {
 #pragma clang fp contract(off)
   const vec4_t<T> xi = S.xm[i];
-  const vec4_t<T> wi = w[i];
+  const vec4_t<T> wi = w[static_cast<size_t>(ws) * i];
   const T rhoi = S.rho[i];
   const uint32_t cell = S.fcell[i];
   T l0 = 0, l1 = 0, l2 = 0;
@@ -310,7 +313,7 @@ __device__ __forceinline__ void product_exact(const SceneView<T> &S, int i,
   };
   nbrs_exact(S, xi, cell, S.fstart, S.xm, [&](uint32_t j, T d0, T d1, T d2, T r2,
                                                 const vec4_t<T> &pj) {
-    const vec4_t<T> wj = w[j];
+    const vec4_t<T> wj = w[static_cast<size_t>(ws) * j];
     const T f = S.K.gradf(r2);
     const T s = (((T(10) * (S.mu_f / rhoi)) * (pj.w / S.rho[j])) *
                  dot3_exact(wi.x - wj.x, wi.y - wj.y, wi.z - wj.z, d0, d1, d2)) /
@@ -349,7 +352,8 @@ __device__ __forceinline__ T gradf_fast(const SceneView<T> &S, T r2) {
 // vector (equal in real arithmetic to applying it per pair)
 template <class T, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xi = S.xm[i];
@@ -424,7 +428,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_list(SceneView<T> S, NbrList L)
   // B_i for the list product, (x_i, m_i / rho_i) for its gathers
   S.cb[i] = make4<vec4_t<T>>(B[0], B[1], B[2], T(0));
   S.G[i] = make4<vec4_t<T>>(B[3], B[4], B[5], T(0));
-  S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w / rhoi);
+  const vec4_t<T> q = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w / rhoi);
+  S.xq[i] = q;
+  S.cxp[2 * static_cast<size_t>(i)] = q;
 }
 
 template <class T>
@@ -450,9 +456,10 @@ __device__ __forceinline__ double rcp_hw<double>(double x) {
 
 template <class T, bool kGC>
 __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrList &L, int i,
-                                             const vec4_t<T> *__restrict__ w, T out[3]) {
+                                             const vec4_t<T> *__restrict__ w, int ws,
+                                             T out[3]) {
   const vec4_t<T> xq = S.xq[i];  // (x_i, m_i / rho_i)
-  const vec4_t<T> wi = w[i];
+  const vec4_t<T> wi = w[static_cast<size_t>(ws) * i];
   const T rhoi = S.rho[i];
   const T cf = T(10) * S.mu_f / rhoi;
   const size_t n = static_cast<size_t>(S.nf);
@@ -482,12 +489,15 @@ __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrLis
   const uint32_t info = L.info[i];
   if (!(info & kListOverflow)) {
     list_gather<4>(L, Spans::of(L.blk[i / kBlock]), n, i, info,
-                   [&](uint32_t j) { return XW{S.xq[j], w[j]}; }, use);
+                   [&](uint32_t j) {
+                     return XW{S.cxp[2 * static_cast<size_t>(j)], w[static_cast<size_t>(ws) * j]};
+                   },
+                   use);
   } else {
     nbrs_exact(S, xq, S.fcell[i], S.fstart, S.xq,
                [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
                  if (j != static_cast<uint32_t>(i))
-                   use(XW{S.xq[j], w[j]});
+                   use(XW{S.xq[j], w[static_cast<size_t>(ws) * j]});
                });
   }
   const vec4_t<T> bd = S.cb[i], bo = S.G[i];
@@ -529,15 +539,16 @@ __device__ __forceinline__ vec4_t<T> precond(const T *M, const vec4_t<T> &w) {
 // y = P r, p = -y; partials of b.b, r.r, r.y
 template <class T, bool kExact, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, double *part) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < S.nf) {
     const vec4_t<T> v = S.v[i];
     T ax[3];
     if (kExact)
-      product_exact<T, kGC>(S, i, S.v, ax);
+      product_exact<T, kGC>(S, i, S.v, 1, ax);
     else
-      product_list<T, kGC>(S, L, i, S.v, ax);
+      product_list<T, kGC>(S, L, i, S.v, 1, ax);
     vec4_t<T> b = v;
     b.w = T(0);
     vec4_t<T> r;
@@ -549,12 +560,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, 
     S.cx[i] = b;
     S.cr[i] = r;
     S.cy[i] = y;
-    S.cp[i] = make4<vec4_t<T>>(-y.x, -y.y, -y.z, T(0));
+    S.cxp[2 * static_cast<size_t>(i) + 1] = make4<vec4_t<T>>(-y.x, -y.y, -y.z, T(0));
     acc[0] = static_cast<double>(dot3r<T>(b, b));
     acc[1] = static_cast<double>(dot3r<T>(r, r));
     acc[2] = static_cast<double>(dot3r<T>(r, y));
   }
-  store_partials<3>(part, gridDim.x, acc);
+  store_partials<3>(part, gridDim.x, acc, lb);
 }
 
 // q = A p, partial p.q
@@ -562,31 +573,29 @@ template <class T, bool kExact, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, double *part) {
   if (S.ctl->cg_done)
     return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[1] = {0.0};
   if (i < S.nf) {
     T q[3];
     if (kExact)
-      product_exact<T, kGC>(S, i, S.cp, q);
+      product_exact<T, kGC>(S, i, S.cxp + 1, 2, q);
     else
-      product_list<T, kGC>(S, L, i, S.cp, q);
+      product_list<T, kGC>(S, L, i, S.cxp + 1, 2, q);
     const vec4_t<T> qv = make4<vec4_t<T>>(q[0], q[1], q[2], T(0));
     S.cq[i] = qv;
-    acc[0] = static_cast<double>(dot3r<T>(S.cp[i], qv));
+    acc[0] = static_cast<double>(dot3r<T>(S.cxp[2 * static_cast<size_t>(i) + 1], qv));
   }
-  store_partials<1>(part, gridDim.x, acc);
+  store_partials<1>(part, gridDim.x, acc, lb);
 }
 
 // x += alpha p, r += alpha q, y = P r; partials r.r, r.y
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(SceneView<T> S, double *part) {
-  if (S.ctl->cg_done)
-    return;
+__device__ __forceinline__ void cg_update_body(const SceneView<T> &S, T alpha, double *part) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double acc[2] = {0.0, 0.0};
   if (i < S.nf) {
-    const T alpha = static_cast<T>(S.ctl->cg_alpha);
-    const vec4_t<T> p = S.cp[i], q = S.cq[i];
+    const vec4_t<T> p = S.cxp[2 * static_cast<size_t>(i) + 1], q = S.cq[i];
     vec4_t<T> x = S.cx[i], r = S.cr[i];
     x.x += alpha * p.x;
     x.y += alpha * p.y;
@@ -604,21 +613,98 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(SceneView<T> S, double *pa
   store_partials<2>(part, gridDim.x, acc);
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(SceneView<T> S, double *part) {
+  if (S.ctl->cg_done)
+    return;
+  cg_update_body(S, static_cast<T>(S.ctl->cg_alpha), part);
+}
+
 // p = -y + beta p
+template <class T>
+__device__ __forceinline__ void cg_direction_body(const SceneView<T> &S, T beta) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> y = S.cy[i];
+  vec4_t<T> p = S.cxp[2 * static_cast<size_t>(i) + 1];
+  p.x = -y.x + beta * p.x;
+  p.y = -y.y + beta * p.y;
+  p.z = -y.z + beta * p.z;
+  S.cxp[2 * static_cast<size_t>(i) + 1] = p;
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(SceneView<T> S) {
   if (S.ctl->cg_done)
     return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= S.nf)
+  cg_direction_body(S, static_cast<T>(S.ctl->cg_beta));
+}
+
+// The same iteration without the two one-block finalize kernels: every block
+// sums all partials of the previous kernel itself, in the finalize kernels'
+// order (sum_partials), so every block derives the same alpha (beta) and the
+// same break decision, and block 0 writes them to the control block.
+// Nothing a block reads from the control block is written in the same
+// kernel. Used while a launch has at most kFusedCgBlocks blocks (each block
+// then reads at most kFusedCgBlocks partials).
+constexpr unsigned kFusedCgBlocks = 1024;
+
+// k_cg_alpha (|p.q| < tol breaks; alpha = (r.y) / (p.q)) and k_cg_update
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_alpha(SceneView<T> S, const double *pq_part,
+                                                            double *part) {
+  Ctl *c = S.ctl;
+  if (c->cg_done)
     return;
-  const T beta = static_cast<T>(S.ctl->cg_beta);
-  const vec4_t<T> y = S.cy[i];
-  vec4_t<T> p = S.cp[i];
-  p.x = -y.x + beta * p.x;
-  p.y = -y.y + beta * p.y;
-  p.z = -y.z + beta * p.z;
-  S.cp[i] = p;
+  const double pq = sum_partials_all(pq_part, gridDim.x);
+  const double ry_prev = c->cg_ry;
+  const bool brk = fabs(pq) < c->cg_tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->cg_ry_prev = ry_prev;
+    c->cg_pq = pq;
+    c->cg_iter_next = c->cg_iter + 1;
+    if (brk) {
+      c->cg_done = 1;
+      publish_cg(c);
+    } else {
+      c->cg_alpha = ry_prev / pq;
+    }
+  }
+  if (brk)
+    return;
+  cg_update_body(S, static_cast<T>(ry_prev / pq), part);
+}
+
+// k_cg_beta (|r.y|_prev < tol breaks; beta; ++k; ok(k, |r|^2)) and
+// k_cg_direction
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_direction_beta(SceneView<T> S, const double *part) {
+  Ctl *c = S.ctl;
+  if (c->cg_done)
+    return;
+  const unsigned nb = gridDim.x;
+  const double rr = sum_partials_all(part, nb);
+  const double ry = sum_partials_all(part + nb, nb);
+  const double ry_prev = c->cg_ry_prev;
+  const bool brk = fabs(ry_prev) < c->cg_tol;
+  const double beta = brk ? 0.0 : ry / ry_prev;
+  const int k = c->cg_iter_next;
+  const bool done = brk || k > c->cg_max_k || rr < c->cg_thr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->cg_r_nsq = rr;
+    c->cg_ry = ry;
+    if (!brk) {
+      c->cg_beta = beta;
+      c->cg_iter = k;
+    }
+    if (done)
+      c->cg_done = 1;
+    publish_cg(c);
+  }
+  if (done)
+    return;
+  cg_direction_body(S, static_cast<T>(beta));
 }
 
 // apply (wkbb18.hpp:401-413): vd = (x - v) H(t - t_birth - dt_fade), a = vd / dt
@@ -676,12 +762,11 @@ void Wkbb18Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L
 }
 
 template <class T>
-void Wkbb18Ops<T>::iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
-                             double *part) {
+void Wkbb18Ops<T>::iteration_product(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+                                     bool gc, double *part) {
   if (S.nf <= 0)
     return;
-  const unsigned nb = grid_blocks(S.nf);
-  const dim3 gr(nb), bl(kBlock);
+  const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
   const NbrList Lv = L ? *L : NbrList{};
   if (L && gc)
     hipLaunchKernelGGL((k_cg_spmv<T, false, true>), gr, bl, 0, st, S, Lv, part);
@@ -691,6 +776,21 @@ void Wkbb18Ops<T>::iteration(hipStream_t st, const SceneView<T> &S, const NbrLis
     hipLaunchKernelGGL((k_cg_spmv<T, true, true>), gr, bl, 0, st, S, Lv, part);
   else
     hipLaunchKernelGGL((k_cg_spmv<T, true, false>), gr, bl, 0, st, S, Lv, part);
+}
+
+template <class T>
+void Wkbb18Ops<T>::iteration_vectors(hipStream_t st, const SceneView<T> &S, double *part,
+                                     bool split) {
+  if (S.nf <= 0)
+    return;
+  const unsigned nb = grid_blocks(S.nf);
+  const dim3 gr(nb), bl(kBlock);
+  if (!split && nb <= kFusedCgBlocks) {
+    // p.q partials at part[0, nb), r.r and r.y at part[nb, 3 nb)
+    hipLaunchKernelGGL(k_cg_update_alpha<T>, gr, bl, 0, st, S, part, part + nb);
+    hipLaunchKernelGGL(k_cg_direction_beta<T>, gr, bl, 0, st, S, part + nb);
+    return;
+  }
   hipLaunchKernelGGL(k_cg_alpha, dim3(1), bl, 0, st, S.ctl, part, nb);
   hipLaunchKernelGGL(k_cg_update<T>, gr, bl, 0, st, S, part);
   hipLaunchKernelGGL(k_cg_beta, dim3(1), bl, 0, st, S.ctl, part, nb);

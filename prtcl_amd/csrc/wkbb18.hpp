@@ -15,9 +15,14 @@ struct Wkbb18Ops {
   // x = v, r = A x - b, y = P r, p = -y and the FP64 set-up scalars
   static void setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
                     double *part, double tol, int max_k, unsigned seq);
-  // one CG iteration (five kernels; all exit at once after convergence)
-  static void iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
-                        double *part);
+  // one CG iteration in two parts, the product q = A p and the vector work
+  // (alpha, x, r, y, beta, p); every kernel exits at once after convergence
+  static void iteration_product(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+                                bool gc, double *part);
+  // split: the alpha and beta reductions in one-block kernels of their own
+  // (always so above kFusedCgBlocks blocks); the results do not depend on it
+  static void iteration_vectors(hipStream_t st, const SceneView<T> &S, double *part,
+                                bool split);
   // vd = (x - v) H, a = vd / dt
   static void apply(hipStream_t st, const SceneView<T> &S);
   // correction.compute_gradient_correction: L_i = pinv(sum_f (m_j / rho_i)

@@ -67,3 +67,33 @@ def test_viscous_steps_vs_oracle(hip_ctx, oracle_lib, step_mode):
         assert np.all(np.isfinite(x))
         err = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
         assert err < tol, (field, err)
+
+
+@pytest.mark.parametrize("implicit_viscosity", [1, 2], ids=["wkbb18", "wkbb18_gc"])
+def test_cg_reductions_in_vector_kernels_equal_split(hip_ctx, implicit_viscosity):
+    """The CG iterations whose vector kernels derive alpha and beta themselves
+    (every block sums all partials, in the order of the one-block kernels)
+    equal, bit for bit, the iterations with separate one-block alpha and beta
+    kernels (schedule bit 4), over steps of the viscous column."""
+    from prtcl_amd import hip
+    sc = viscous_column()
+    sc.implicit_viscosity = implicit_viscosity
+    sc.permute_every = 0
+    print(f"case: viscous column, implicit_viscosity {implicit_viscosity}, folded vs split",
+          flush=True)
+    out = []
+    for mask in (8, 8 | 16):
+        g = hip.HipScene(hip_ctx, sc)
+        g.set_schedule(mask)
+        g.prepare()
+        iters = []
+        for _ in range(4):
+            g.step(1)
+            iters.append(g.step_info()["wkbb18_iterations"])
+        assert g.schedule() == mask
+        out.append((iters, {f: g.download("fluid", f) for f in ("position", "velocity")}))
+        g.destroy()
+    (ia, fa), (ib, fb) = out
+    assert ia == ib and min(ia) >= 1
+    for f in fa:
+        np.testing.assert_array_equal(fa[f], fb[f], err_msg=f)
