@@ -71,6 +71,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dambreak_1m",
                     choices=["dambreak_1m", "dambreak_8k", "viscous_250k", "viscous_gc_250k",
+                             "pt16_250k",
                              "served_stream", "checkpoint_1m"])
     ap.add_argument("--duration", type=float, default=30.0,
                     help="served stream: length of the arrival trace in seconds")
@@ -109,7 +110,13 @@ def main():
     from prtcl_amd import hip, scenes
 
     sc = {"dambreak_1m": scenes.config1, "dambreak_8k": scenes.config0,
-          "viscous_250k": scenes.config2, "viscous_gc_250k": scenes.config2}[args.workload]()
+          "viscous_250k": scenes.config2, "viscous_gc_250k": scenes.config2,
+          "pt16_250k": scenes.config2}[args.workload]()
+    if args.workload == "pt16_250k":
+        # the same tank with pt16 (vorticity diffusion and velocity
+        # reconstruction, the solver settings of prtcl-lua.bunny.lua:150-155)
+        sc.implicit_viscosity = 3
+        sc.name = "pt16_column_250k"
     if args.workload == "viscous_gc_250k":
         # the same tank with the gradient correction and wkbb18_gc
         sc.implicit_viscosity = 2

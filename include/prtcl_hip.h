@@ -362,10 +362,10 @@ int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *step
  * mask: bit 0 setup, bit 1 pressure acceleration, bit 2 pressure in LDS;
  * bit 3 launches each IISPH iteration kernel by kernel instead of replaying
  * it from a captured hipGraph; bit 4 gives the alpha and beta reductions of
- * every WKBB18 CG iteration one-block kernels of their own (they are
- * otherwise folded into the vector kernels of scenes of up to 262,144 fluid
- * particles); -1 restarts the tuner. scene < 0 addresses the schedule of
- * batched steps. */
+ * every CG iteration (WKBB18 and pt16 solves) one-block kernels of their own
+ * (they are otherwise folded into the vector kernels of scenes of up to
+ * 262,144 fluid particles); -1 restarts the tuner. scene < 0 addresses the
+ * schedule of batched steps. */
 int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask);
 int prtcl_hip_get_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t *mask);
 

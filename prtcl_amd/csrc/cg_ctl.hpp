@@ -123,5 +123,62 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, 
   }
 }
 
+// The same decisions inside the vector kernels of an iteration, for
+// launches of at most kFusedCgBlocks blocks: every block sums all partials of
+// the kernel before it (sum_partials_all: the order of the one-block
+// kernels, so every block derives the same scalars and decisions), and block
+// 0 writes them to the control block. Nothing a block reads from the control
+// block is written in the same kernel: the iteration count the stop test
+// needs is staged in cg_iter_next beside alpha.
+constexpr unsigned kFusedCgBlocks = 1024;
+
+// k_cg_alpha in every block of the update kernel: false when the solve
+// breaks (|p.q| < tol), else alpha
+__device__ __forceinline__ bool cg_alpha_in_block(Ctl *c, const double *pq_part, unsigned nb,
+                                                  double *alpha) {
+  const double pq = sum_partials_all(pq_part, nb);
+  const double ry_prev = c->cg_ry;
+  const bool brk = fabs(pq) < c->cg_tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->cg_ry_prev = ry_prev;
+    c->cg_pq = pq;
+    c->cg_iter_next = c->cg_iter + 1;
+    if (brk) {
+      c->cg_done = 1;
+      publish_cg(c);
+    } else {
+      c->cg_alpha = ry_prev / pq;
+    }
+  }
+  *alpha = brk ? 0.0 : ry_prev / pq;
+  return !brk;
+}
+
+// k_cg_beta in every block of the direction kernel (r.r partials at part,
+// r.y at part + nb): false when the solve is done, else beta
+__device__ __forceinline__ bool cg_beta_in_block(Ctl *c, const double *part, unsigned nb,
+                                                 double *beta) {
+  const double rr = sum_partials_all(part, nb);
+  const double ry = sum_partials_all(part + nb, nb);
+  const double ry_prev = c->cg_ry_prev;
+  const bool brk = fabs(ry_prev) < c->cg_tol;
+  const double b = brk ? 0.0 : ry / ry_prev;
+  const int k = c->cg_iter_next;
+  const bool done = brk || k > c->cg_max_k || rr < c->cg_thr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->cg_r_nsq = rr;
+    c->cg_ry = ry;
+    if (!brk) {
+      c->cg_beta = b;
+      c->cg_iter = k;
+    }
+    if (done)
+      c->cg_done = 1;
+    publish_cg(c);
+  }
+  *beta = b;
+  return !done;
+}
+
 }  // namespace
 }  // namespace prtcl_amd

@@ -15,8 +15,8 @@ struct FusedSchedule {
   bool pressure_lds = false;
   // each IISPH iteration is replayed from a captured hipGraph unless eager
   bool eager = false;
-  // the WKBB18 CG iterations launch their one-block alpha and beta kernels
-  // even where the vector kernels can derive the scalars themselves
+  // the CG iterations (WKBB18, pt16) launch their one-block alpha and beta
+  // kernels even where the vector kernels can derive the scalars themselves
   bool cg_split = false;
   int mask() const {
     return (setup_lds ? 1 : 0) | (accel_lds ? 2 : 0) | (pressure_lds ? 4 : 0) | (eager ? 8 : 0) |

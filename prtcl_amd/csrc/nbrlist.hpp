@@ -101,13 +101,18 @@ __device__ __forceinline__ uint32_t list_entry(const NbrList &L, size_t n, int i
                e % kChunk];
 }
 
-// Visits the global slots of the list entries of particle i (not
-// overflowed) in list order: fn(j). R is the span table of i's block. The
-// slot is b0 + entry + [e >= c0] (b1 - b0) + [e >= c01] (b2 - b1) (two selects
-// against register values).
+// entries of a (not overflowed) list: the sum of its three slab counts
+__device__ __forceinline__ uint32_t list_count(uint32_t info) {
+  return (info & 0xFFu) + ((info >> 8) & 0xFFu) + ((info >> 16) & 0xFFu);
+}
+
+// Visits the list entries of particle i (not overflowed) in list order:
+// fn(e, j) with e the entry's index and j its global slot. R is the span
+// table of i's block. The slot is b0 + entry + [e >= c0] (b1 - b0) +
+// [e >= c01] (b2 - b1) (two selects against register values).
 template <class Fn>
-__device__ __forceinline__ void list_visit(const NbrList &L, const Spans &R, size_t n, int i,
-                                           uint32_t info, Fn &&fn) {
+__device__ __forceinline__ void list_visit_indexed(const NbrList &L, const Spans &R, size_t n,
+                                                   int i, uint32_t info, Fn &&fn) {
   const uint32_t c0 = info & 0xFFu;
   const uint32_t c01 = c0 + ((info >> 8) & 0xFFu);
   const uint32_t c = c01 + ((info >> 16) & 0xFFu);
@@ -118,9 +123,16 @@ __device__ __forceinline__ void list_visit(const NbrList &L, const Spans &R, siz
     for (int k = 0; k < static_cast<int>(kChunk); ++k) {
       const uint32_t e = g * kChunk + k;
       if (e < c)
-        fn(R.b0 + chunk_entry(q, k) + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u));
+        fn(e, R.b0 + chunk_entry(q, k) + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u));
     }
   }
+}
+
+// the same with the slot only: fn(j)
+template <class Fn>
+__device__ __forceinline__ void list_visit(const NbrList &L, const Spans &R, size_t n, int i,
+                                           uint32_t info, Fn &&fn) {
+  list_visit_indexed(L, R, n, i, info, [&](uint32_t, uint32_t j) { fn(j); });
 }
 
 // The same walk one chunk (8 entries) at a time with the memory traffic of

@@ -3,12 +3,16 @@
 // particle itself included, as CopyNeighbors) and is evaluated in the
 // reference's FP32 operation order (this file is built without FMA
 // contraction); matrices are 3x3 column-major, Eigen's fixed-size sums of
-// three terms are x0 + (x1 + x2). The two solves run the CG of
+// three terms are x0 + (x1 + x2). In fused steps the CG products use the
+// step's neighbour list instead: the pair coefficients m_j W_ij are written
+// once per step and streamed by every product (the self term first, then
+// the list order). The two solves run the CG of
 // cg_openmp.hpp:32-215 once per vector component, with the FP64 device
 // control of cg_ctl.hpp, as the reference's `foreach dimension` loop of
 // scalar CGOpenMP<T> solves (pt16.hpp:420-519, 608-760).
 #include "cg_ctl.hpp"
 #include "exact_walk.hpp"
+#include "nbrlist.hpp"
 #include "pt16.hpp"
 
 namespace prtcl_amd {
@@ -181,24 +185,112 @@ __device__ __forceinline__ T product_p(const SceneView<T> &S, int i, T r) {
   return r / (S.psrho[i] - S.xm[i].w * S.K.w(T(0)));
 }
 
-// x = guess, r = A x - b, y = P r, p = -y; partials b.b, r.r, r.y
+// ---- list form of the products (fused steps)
+// index of the coefficient of list entry e of particle i (chunks of 4
+// entries, so a lane reads 4 coefficients with one load and a wavefront's
+// load is contiguous)
+__device__ __forceinline__ size_t coef_index(size_t n, int i, uint32_t e) {
+  return ((static_cast<size_t>(e / 4) * n + static_cast<size_t>(i)) * 4) + e % 4;
+}
+
+// m_j W_ij of every list entry (particles whose list overflowed keep the
+// grid walk)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, int which, int dim,
-                                                          double *part) {
+__global__ __launch_bounds__(kBlock) void k_pt16_coef(SceneView<T> S, NbrList L) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
+  if (i >= S.nf)
+    return;
+  const uint32_t info = L.info[i];
+  if (info & kListOverflow)
+    return;
+  const vec4_t<T> xv = S.xm[i];
+  const size_t n = static_cast<size_t>(S.nf);
+  list_visit_indexed(L, Spans::of(L.blk[lb]), n, i, info, [&](uint32_t e, uint32_t j) {
+    const vec4_t<T> pj = S.xm[j];
+    const T d0 = xv.x - pj.x, d1 = xv.y - pj.y, d2 = xv.z - pj.z;
+    const T r2 = d0 * d0 + (d1 * d1 + d2 * d2);
+    S.pcoef[coef_index(n, i, e)] = pj.w * S.K.w(r2);
+  });
+}
+
+// (A w)_i = c_i w_i - m_i W(0) w_i - sum_list (m_j W_ij) w_j with w_j at
+// w[ws * j] (ws = 4: one component of a real4 field); the coefficients of a
+// chunk of 8 entries and the 8 gathers are issued before the arithmetic
+template <class T>
+__device__ __forceinline__ T product_s_list(const SceneView<T> &S, const NbrList &L, int which,
+                                            int i, unsigned lb, const T *__restrict__ w,
+                                            int ws) {
+#pragma clang fp contract(off)
+  const vec4_t<T> xv = S.xm[i];
+  const T wi = w[static_cast<size_t>(ws) * i];
+  T out = (which == 0 ? S.psrho[i] : S.prho[i]) * wi;
+  const uint32_t info = L.info[i];
+  if (info & kListOverflow) {
+    nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+               [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
+                 out -= (pj.w * S.K.w(r2)) * w[static_cast<size_t>(ws) * j];
+               });
+    return out;
+  }
+  out -= (xv.w * S.K.w(T(0))) * wi;
+  const size_t n = static_cast<size_t>(S.nf);
+  const Spans R = Spans::of(L.blk[lb]);
+  const uint32_t c0 = info & 0xFFu;
+  const uint32_t c01 = c0 + ((info >> 8) & 0xFFu);
+  const uint32_t c = list_count(info);
+  const uint32_t d1 = R.b1 - R.b0, d2 = R.b2 - R.b1;
+  const vec4_t<T> *__restrict__ coef = reinterpret_cast<const vec4_t<T> *>(S.pcoef);
+  for (uint32_t g = 0; g * kChunk < c; ++g) {
+    const uint4 q = list_chunk(L, n, i, g);
+    const vec4_t<T> ca = coef[(2 * static_cast<size_t>(g)) * n + i];
+    vec4_t<T> cb = ca;
+    if (g * kChunk + 4 < c)
+      cb = coef[(2 * static_cast<size_t>(g) + 1) * n + i];
+    T wj[kChunk];
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(kChunk); ++k) {
+      const uint32_t e = g * kChunk + k;
+      if (e < c) {
+        const uint32_t j = R.b0 + chunk_entry(q, k) + (e >= c0 ? d1 : 0u) + (e >= c01 ? d2 : 0u);
+        wj[k] = w[static_cast<size_t>(ws) * j];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const T cf[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(kChunk); ++k)
+      if (g * kChunk + k < c)
+        out -= cf[k] * wj[k];
+  }
+  return out;
+}
+
+// x = guess, r = A x - b, y = P r, p = -y; partials b.b, r.r, r.y
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, NbrList L, int which,
+                                                          int dim, double *part) {
+#pragma clang fp contract(off)
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < S.nf) {
     const vec4_t<T> *guess = which == 0 ? S.om : S.v;
     const T b = comp<T>((which == 0 ? S.orhs : S.vrhs) + i, dim);
     const T x0 = comp<T>(guess + i, dim);
     // A x0 over the guess field's component
-    const vec4_t<T> xv = S.xm[i];
-    T ax = (which == 0 ? S.psrho[i] : S.prho[i]) * x0;
-    nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
-               [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
-                 ax -= (pj.w * S.K.w(r2)) * comp<T>(guess + j, dim);
-               });
+    T ax;
+    if (kList) {
+      ax = product_s_list(S, L, which, i, lb, reinterpret_cast<const T *>(guess) + dim, 4);
+    } else {
+      const vec4_t<T> xv = S.xm[i];
+      ax = (which == 0 ? S.psrho[i] : S.prho[i]) * x0;
+      nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+                 [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
+                   ax -= (pj.w * S.K.w(r2)) * comp<T>(guess + j, dim);
+                 });
+    }
     const T r = ax - b;
     const T y = product_p(S, i, r);
     S.scx[i] = x0;
@@ -209,35 +301,35 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, int wh
     acc[1] = static_cast<double>(r * r);
     acc[2] = static_cast<double>(r * y);
   }
-  store_partials<3>(part, gridDim.x, acc);
+  store_partials<3>(part, gridDim.x, acc, lb);
 }
 
 // q = A p, partial p.q
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv(SceneView<T> S, int which, double *part) {
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv(SceneView<T> S, NbrList L, int which,
+                                                         double *part) {
 #pragma clang fp contract(off)
   if (S.ctl->cg_done)
     return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[1] = {0.0};
   if (i < S.nf) {
-    const T q = product_s(S, which, i, S.scp);
+    const T q = kList ? product_s_list(S, L, which, i, lb, S.scp, 1)
+                      : product_s(S, which, i, S.scp);
     S.scq[i] = q;
     acc[0] = static_cast<double>(S.scp[i] * q);
   }
-  store_partials<1>(part, gridDim.x, acc);
+  store_partials<1>(part, gridDim.x, acc, lb);
 }
 
 // x += alpha p, r += alpha q, y = P r; partials r.r, r.y
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_update(SceneView<T> S, double *part) {
+__device__ __forceinline__ void pt16_update_body(const SceneView<T> &S, T alpha, double *part) {
 #pragma clang fp contract(off)
-  if (S.ctl->cg_done)
-    return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double acc[2] = {0.0, 0.0};
   if (i < S.nf) {
-    const T alpha = static_cast<T>(S.ctl->cg_alpha);
     const T x = S.scx[i] + alpha * S.scp[i];
     const T r = S.scr[i] + alpha * S.scq[i];
     const T y = product_p(S, i, r);
@@ -250,17 +342,53 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_update(SceneView<T> S, doubl
   store_partials<2>(part, gridDim.x, acc);
 }
 
-// p = -y + beta p
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_direction(SceneView<T> S) {
-#pragma clang fp contract(off)
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_update(SceneView<T> S, double *part) {
   if (S.ctl->cg_done)
     return;
+  pt16_update_body(S, static_cast<T>(S.ctl->cg_alpha), part);
+}
+
+// p = -y + beta p
+template <class T>
+__device__ __forceinline__ void pt16_direction_body(const SceneView<T> &S, T beta) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.nf)
     return;
-  const T beta = static_cast<T>(S.ctl->cg_beta);
   S.scp[i] = -S.scy[i] + beta * S.scp[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_direction(SceneView<T> S) {
+  if (S.ctl->cg_done)
+    return;
+  pt16_direction_body(S, static_cast<T>(S.ctl->cg_beta));
+}
+
+// the iteration without its two one-block kernels (cg_alpha_in_block,
+// cg_beta_in_block)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_update_alpha(SceneView<T> S,
+                                                                 const double *pq_part,
+                                                                 double *part) {
+  if (S.ctl->cg_done)
+    return;
+  double alpha;
+  if (!cg_alpha_in_block(S.ctl, pq_part, gridDim.x, &alpha))
+    return;
+  pt16_update_body(S, static_cast<T>(alpha), part);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_direction_beta(SceneView<T> S,
+                                                                   const double *part) {
+  if (S.ctl->cg_done)
+    return;
+  double beta;
+  if (!cg_beta_in_block(S.ctl, part, gridDim.x, &beta))
+    return;
+  pt16_direction_body(S, static_cast<T>(beta));
 }
 
 // apply (pt16.hpp:491-502, 700-717): component dim of the vorticity (which 0)
@@ -308,22 +436,44 @@ void Pt16Ops<T>::spin(hipStream_t st, const SceneView<T> &S) {
 }
 
 template <class T>
-void Pt16Ops<T>::cg_setup(hipStream_t st, const SceneView<T> &S, int which, int dim,
-                          double *part, double tol, int max_k, unsigned seq) {
-  const unsigned nb = grid_blocks(S.nf);
+void Pt16Ops<T>::coefficients(hipStream_t st, const SceneView<T> &S, const NbrList &L) {
   if (S.nf > 0)
-    hipLaunchKernelGGL(k_pt16_cg_setup<T>, dim3(nb), dim3(kBlock), 0, st, S, which, dim, part);
+    hipLaunchKernelGGL(k_pt16_coef<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, L);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, int which,
+                          int dim, double *part, double tol, int max_k, unsigned seq) {
+  const unsigned nb = grid_blocks(S.nf);
+  if (S.nf > 0) {
+    if (L)
+      hipLaunchKernelGGL((k_pt16_cg_setup<T, true>), dim3(nb), dim3(kBlock), 0, st, S, *L,
+                         which, dim, part);
+    else
+      hipLaunchKernelGGL((k_pt16_cg_setup<T, false>), dim3(nb), dim3(kBlock), 0, st, S,
+                         NbrList{}, which, dim, part);
+  }
   hipLaunchKernelGGL(k_cg_setup_fin, dim3(1), dim3(kBlock), 0, st, S.ctl, part, nb, tol, max_k,
                      seq);
 }
 
 template <class T>
-void Pt16Ops<T>::cg_iteration(hipStream_t st, const SceneView<T> &S, int which, double *part) {
+void Pt16Ops<T>::cg_iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L,
+                              int which, double *part, bool split) {
   if (S.nf <= 0)
     return;
   const unsigned nb = grid_blocks(S.nf);
   const dim3 gr(nb), bl(kBlock);
-  hipLaunchKernelGGL(k_pt16_cg_spmv<T>, gr, bl, 0, st, S, which, part);
+  if (L)
+    hipLaunchKernelGGL((k_pt16_cg_spmv<T, true>), gr, bl, 0, st, S, *L, which, part);
+  else
+    hipLaunchKernelGGL((k_pt16_cg_spmv<T, false>), gr, bl, 0, st, S, NbrList{}, which, part);
+  if (!split && nb <= kFusedCgBlocks) {
+    // p.q partials at part[0, nb), r.r and r.y at part[nb, 3 nb)
+    hipLaunchKernelGGL(k_pt16_cg_update_alpha<T>, gr, bl, 0, st, S, part, part + nb);
+    hipLaunchKernelGGL(k_pt16_cg_direction_beta<T>, gr, bl, 0, st, S, part + nb);
+    return;
+  }
   hipLaunchKernelGGL(k_cg_alpha, dim3(1), bl, 0, st, S.ctl, part, nb);
   hipLaunchKernelGGL(k_pt16_cg_update<T>, gr, bl, 0, st, S, part);
   hipLaunchKernelGGL(k_cg_beta, dim3(1), bl, 0, st, S.ctl, part, nb);

@@ -6,6 +6,7 @@
 // cg_openmp.hpp:32-215 over one vector component at a time.
 #pragma once
 
+#include "nbrlist.hpp"
 #include "scene.hpp"
 
 namespace prtcl_amd {
@@ -24,9 +25,16 @@ struct Pt16Ops {
   // which = 0: vorticity diffusion (system diagonal pt16_same_density, guess
   // and result in vorticity[dim]); which = 1: velocity reconstruction
   // (diagonal pt16_density, guess and result in velocity[dim])
-  static void cg_setup(hipStream_t st, const SceneView<T> &S, int which, int dim,
-                       double *part, double tol, int max_k, unsigned seq);
-  static void cg_iteration(hipStream_t st, const SceneView<T> &S, int which, double *part);
+  // With a neighbour list L (fused steps), the products use the pair
+  // coefficients written by coefficients() for the step's list; without, they
+  // walk the grid in the reference's order.
+  static void coefficients(hipStream_t st, const SceneView<T> &S, const NbrList &L);
+  static void cg_setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, int which,
+                       int dim, double *part, double tol, int max_k, unsigned seq);
+  // split: the alpha and beta reductions in one-block kernels of their own
+  // (always so above kFusedCgBlocks blocks); the results do not depend on it
+  static void cg_iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L, int which,
+                           double *part, bool split);
   static void cg_apply(hipStream_t st, const SceneView<T> &S, int which, int dim);
 };
 

@@ -114,6 +114,9 @@ struct SceneView {
   V4 *om, *orhs, *vrhs;
   T *tvg, *prho, *psrho;
   T *scx, *scr, *scp, *scq, *scy;
+  // list form of the pt16 products (fused steps): m_j W_ij of list entry e of
+  // particle i at pcoef[((e / 4) * nf + i) * 4 + e % 4], written once per step
+  T *pcoef;
   int32_t *id;
   uint32_t *fcell;
   V4 *bxV;

@@ -807,6 +807,7 @@ class SceneImpl final : public SceneBase {
     S.scp = P_.scp;
     S.scq = P_.scq;
     S.scy = P_.scy;
+    S.pcoef = P_.pcoef;
     S.id = P_.id;
     S.fcell = P_.fcell;
     S.bxV = P_.bxV;
@@ -970,23 +971,28 @@ class SceneImpl final : public SceneBase {
   // solve_velocity_reconstruction (pt16.hpp:213-760); each solve is one scalar
   // PCG per vector component with tolerance pt16_maximum_error and cap
   // pt16_maximum_iterations, pt16_iterations = the last component's count
-  void run_pt16(int proc) {
+  // with `fused` the CG products use the step's neighbour list (the pair
+  // coefficients are written by the set-up)
+  void run_pt16(int proc, bool fused = false) {
     if (desc.implicit_viscosity != 3)
       throw std::logic_error("the scene was created without pt16");
     hipStream_t st = res_->stream;
     const SceneView<T> S = view();
+    const NbrList L = nbr_list();
+    const NbrList *pl = fused ? &L : nullptr;
     Profiler &pf = res_->prof;
     const T xi = static_cast<T>(pt16_xi_);
+    const bool split = tuner_.current().cg_split;
     auto solve = [&](int which) {
       for (int dim = 0; dim < 3; ++dim) {
         const unsigned seq = ++seq_;
         const int max_k = std::max(0, pt16_max_k_);
         int r = pf.begin(st, kStagePcgVector);
-        Pt16Ops<T>::cg_setup(st, S, which, dim, P_.cg_part, pt16_tol_, max_k, seq);
+        Pt16Ops<T>::cg_setup(st, S, pl, which, dim, P_.cg_part, pt16_tol_, max_k, seq);
         pf.end(st, r);
         const Progress p = cg_loop(seq, max_k, predicted_pt16_[which], [&](int tag) {
           const int rp = pf.begin(st, kStagePcgSpmv, tag);
-          Pt16Ops<T>::cg_iteration(st, S, which, P_.cg_part);
+          Pt16Ops<T>::cg_iteration(st, S, pl, which, P_.cg_part, split);
           pf.end(st, rp);
         });
         r = pf.begin(st, kStagePcgVector);
@@ -1005,6 +1011,8 @@ class SceneImpl final : public SceneBase {
     switch (proc) {
       case PRTCL_HIP_P_PT16_SETUP:
         Pt16Ops<T>::setup(st, S, xi);
+        if (fused)
+          Pt16Ops<T>::coefficients(st, S, L);
         pf.end(st, r);
         break;
       case PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION:
@@ -1270,13 +1278,13 @@ class SceneImpl final : public SceneBase {
           // example.lua:435-445: setup, vorticity diffusion and velocity
           // reconstruction with the solver settings of each (the latter
           // writes the velocity; no integration of v after it)
-          run_pt16(PRTCL_HIP_P_PT16_SETUP);
+          run_pt16(PRTCL_HIP_P_PT16_SETUP, fused);
           pt16_tol_ = desc.pt16_vorticity_maximum_error;
           pt16_max_k_ = desc.pt16_vorticity_maximum_iterations;
-          run_pt16(PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION);
+          run_pt16(PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION, fused);
           pt16_tol_ = desc.pt16_velocity_maximum_error;
           pt16_max_k_ = desc.pt16_velocity_maximum_iterations;
-          run_pt16(PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION);
+          run_pt16(PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION, fused);
         } else if (desc.implicit_viscosity == 2) {
           // prtcl-lua.bunny.lua:279-281, 321-327: the gradient correction of
           // the step's positions and densities (computed here, right before
@@ -1676,7 +1684,7 @@ class SceneImpl final : public SceneBase {
     V4 *xm, *v, *a, *c, *G, *vd, *tmpV4, *xq, *xa;
     T *rho, *p, *pr, *s, *AA, *Ap, *tmpT, *sp, *tmpSp, *gc;
     V4 *om, *orhs, *vrhs;
-    T *tvg, *prho, *psrho, *pone, *odiag, *vdiag, *scx, *scr, *scp, *scq, *scy;
+    T *tvg, *prho, *psrho, *pone, *odiag, *vdiag, *scx, *scr, *scp, *scq, *scy, *pcoef;
     int32_t *id, *tmpI;
     uint32_t *fcell, *frank, *fperm, *ninfo;
     uint16_t *nidx;
@@ -1736,6 +1744,7 @@ class SceneImpl final : public SceneBase {
     for (V4 **q : {&P.om, &P.orhs, &P.vrhs})
       A.take(*q, np);
     A.take(P.tvg, 9 * np);
+    A.take(P.pcoef, pt16 ? static_cast<size_t>(kNbrCap) * f : 1);
     for (T **q : {&P.prho, &P.psrho, &P.pone, &P.odiag, &P.vdiag, &P.scx, &P.scr, &P.scp,
                   &P.scq, &P.scy})
       A.take(*q, np);
@@ -1750,7 +1759,7 @@ class SceneImpl final : public SceneBase {
       f(reinterpret_cast<void **>(q));
     for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp,
                   &P_.gc, &P_.tvg, &P_.prho, &P_.psrho, &P_.pone, &P_.odiag, &P_.vdiag,
-                  &P_.scx, &P_.scr, &P_.scp, &P_.scq, &P_.scy})
+                  &P_.scx, &P_.scr, &P_.scp, &P_.scq, &P_.scy, &P_.pcoef})
       f(reinterpret_cast<void **>(q));
     for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
       f(reinterpret_cast<void **>(q));
@@ -1794,7 +1803,7 @@ class SceneImpl final : public SceneBase {
       Arena::rebase(*q, base);
     for (T **q : {&P_.rho, &P_.p, &P_.pr, &P_.s, &P_.AA, &P_.Ap, &P_.tmpT, &P_.sp, &P_.tmpSp,
                   &P_.gc, &P_.tvg, &P_.prho, &P_.psrho, &P_.pone, &P_.odiag, &P_.vdiag,
-                  &P_.scx, &P_.scr, &P_.scp, &P_.scq, &P_.scy})
+                  &P_.scx, &P_.scr, &P_.scp, &P_.scq, &P_.scy, &P_.pcoef})
       Arena::rebase(*q, base);
     for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
       Arena::rebase(*q, base);

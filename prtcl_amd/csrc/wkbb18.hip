@@ -641,68 +641,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(SceneView<T> S) {
   cg_direction_body(S, static_cast<T>(S.ctl->cg_beta));
 }
 
-// The same iteration without the two one-block finalize kernels: every block
-// sums all partials of the previous kernel itself, in the finalize kernels'
-// order (sum_partials), so every block derives the same alpha (beta) and the
-// same break decision, and block 0 writes them to the control block.
-// Nothing a block reads from the control block is written in the same
-// kernel. Used while a launch has at most kFusedCgBlocks blocks (each block
-// then reads at most kFusedCgBlocks partials).
-constexpr unsigned kFusedCgBlocks = 1024;
-
-// k_cg_alpha (|p.q| < tol breaks; alpha = (r.y) / (p.q)) and k_cg_update
+// The iteration without its two one-block kernels (cg_alpha_in_block,
+// cg_beta_in_block): k_cg_alpha with k_cg_update, k_cg_beta with
+// k_cg_direction
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_update_alpha(SceneView<T> S, const double *pq_part,
                                                             double *part) {
-  Ctl *c = S.ctl;
-  if (c->cg_done)
+  if (S.ctl->cg_done)
     return;
-  const double pq = sum_partials_all(pq_part, gridDim.x);
-  const double ry_prev = c->cg_ry;
-  const bool brk = fabs(pq) < c->cg_tol;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c->cg_ry_prev = ry_prev;
-    c->cg_pq = pq;
-    c->cg_iter_next = c->cg_iter + 1;
-    if (brk) {
-      c->cg_done = 1;
-      publish_cg(c);
-    } else {
-      c->cg_alpha = ry_prev / pq;
-    }
-  }
-  if (brk)
+  double alpha;
+  if (!cg_alpha_in_block(S.ctl, pq_part, gridDim.x, &alpha))
     return;
-  cg_update_body(S, static_cast<T>(ry_prev / pq), part);
+  cg_update_body(S, static_cast<T>(alpha), part);
 }
 
-// k_cg_beta (|r.y|_prev < tol breaks; beta; ++k; ok(k, |r|^2)) and
-// k_cg_direction
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction_beta(SceneView<T> S, const double *part) {
-  Ctl *c = S.ctl;
-  if (c->cg_done)
+  if (S.ctl->cg_done)
     return;
-  const unsigned nb = gridDim.x;
-  const double rr = sum_partials_all(part, nb);
-  const double ry = sum_partials_all(part + nb, nb);
-  const double ry_prev = c->cg_ry_prev;
-  const bool brk = fabs(ry_prev) < c->cg_tol;
-  const double beta = brk ? 0.0 : ry / ry_prev;
-  const int k = c->cg_iter_next;
-  const bool done = brk || k > c->cg_max_k || rr < c->cg_thr;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c->cg_r_nsq = rr;
-    c->cg_ry = ry;
-    if (!brk) {
-      c->cg_beta = beta;
-      c->cg_iter = k;
-    }
-    if (done)
-      c->cg_done = 1;
-    publish_cg(c);
-  }
-  if (done)
+  double beta;
+  if (!cg_beta_in_block(S.ctl, part, gridDim.x, &beta))
     return;
   cg_direction_body(S, static_cast<T>(beta));
 }

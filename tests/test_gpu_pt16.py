@@ -99,3 +99,27 @@ def test_pt16_scene_evict_restore(hip_ctx):
                                                        "target_velocity_gradient")})
     for f in out[0]:
         assert np.array_equal(out[0][f], out[1][f]), f
+
+
+@pytest.mark.parametrize("step_mode", [1, 0], ids=["procedures", "fused"])
+def test_pt16_cg_reductions_in_vector_kernels_equal_split(hip_ctx, step_mode):
+    """The pt16 solves whose CG vector kernels derive alpha and beta
+    themselves equal, bit for bit, the solves with separate one-block alpha
+    and beta kernels (schedule bit 4)."""
+    from prtcl_amd import hip
+    sc = pt16_block()
+    sc.permute_every = 0
+    print(f"case: pt16 block, step_mode {step_mode}, folded vs split", flush=True)
+    out = []
+    for mask in (8, 8 | 16):
+        g = hip.HipScene(hip_ctx, sc, step_mode=step_mode)
+        g.set_schedule(mask)
+        g.prepare()
+        g.step(3)
+        out.append((g.get_global("pt16_iterations"),
+                    {f: g.download("fluid", f) for f in ("velocity", "vorticity")}))
+        g.destroy()
+    (ka, fa), (kb, fb) = out
+    assert ka == kb and ka >= 1
+    for f in fa:
+        np.testing.assert_array_equal(fa[f], fb[f], err_msg=f)
