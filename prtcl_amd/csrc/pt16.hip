@@ -32,44 +32,101 @@ __device__ __forceinline__ T *comp_ptr(vec4_t<T> *p, int d) {
   return reinterpret_cast<T *>(p) + d;
 }
 
-// ---- pt16.setup
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, T xi) {
+// ---- list form (fused steps)
+// index of the coefficient m_j W_ij of list entry e of particle i (chunks of
+// 4 entries, so a lane reads 4 with one load and a wavefront's load is
+// contiguous); written by the set-up, read by the products and the
+// vorticity right-hand side of the same step
+__device__ __forceinline__ size_t coef_index(size_t n, int i, uint32_t e) {
+  return ((static_cast<size_t>(e / 4) * n + static_cast<size_t>(i)) * 4) + e % 4;
+}
+
+constexpr uint32_t kSelfEntry = 0xFFFFFFFEu;  // the particle itself
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;    // a grid-walk neighbour (list overflowed)
+
+// The fluid neighbours of particle i in a fused step: fn(j, d0, d1, d2, r2,
+// x_j, e) for the particle itself first (d = 0, e = kSelfEntry), then the
+// list entries in list order (e = entry index); a particle whose list
+// overflowed walks the grid (itself included, e = kNoEntry).
+template <class T, class Fn>
+__device__ __forceinline__ void fluid_pairs_list(const SceneView<T> &S, const NbrList &L,
+                                                 unsigned lb, int i, const vec4_t<T> &xv,
+                                                 Fn &&fn) {
+  const uint32_t info = L.info[i];
+  if (info & kListOverflow) {
+    nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+               [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
+                 fn(j, d0, d1, d2, r2, pj, kNoEntry);
+               });
+    return;
+  }
+  fn(static_cast<uint32_t>(i), T(0), T(0), T(0), T(0), xv, kSelfEntry);
+  list_visit_indexed(L, Spans::of(L.blk[lb]), static_cast<size_t>(S.nf), i, info,
+                     [&](uint32_t e, uint32_t j) {
+                       const vec4_t<T> pj = S.xm[j];
+                       const T d0 = xv.x - pj.x, d1 = xv.y - pj.y, d2 = xv.z - pj.z;
+                       fn(j, d0, d1, d2, d0 * d0 + (d1 * d1 + d2 * d2), pj, e);
+                     });
+}
+
+// ---- pt16.setup; with kList (fused steps) one list pass for the fluid sums
+// that also writes the step's pair coefficients m_j W_ij
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, NbrList L, T xi) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xv = S.xm[i];
   const vec4_t<T> vi = S.v[i];
   const T rhoi = S.rho[i];
   const uint32_t cell = S.fcell[i];
-  // same-group density and the smoothed density (pt16.prtcl:53-71)
+  // same-group density and the smoothed density (pt16.prtcl:53-71), and the
+  // velocity gradient sum_f m_j (v_j - v_i) gradW_ij^T / rho_i
+  // (pt16.prtcl:76-88)
   T same = 0, prho = 0;
-  nbrs_exact(S, xv, cell, S.fstart, S.xm,
-             [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pj) {
-               same += pj.w * S.K.w(r2);
-             });
-  nbrs_exact(S, xv, cell, S.fstart, S.xm,
-             [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pj) {
-               prho += (pj.w / S.rho0) * S.K.w(r2);
-             });
-  nbrs_exact(S, xv, cell, S.bstart, S.bxV,
-             [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pb) { prho += pb.w * S.K.w(r2); });
-  prho *= S.rho0;
-  // velocity gradient sum_f m_j (v_j - v_i) gradW_ij^T / rho_i (pt16.prtcl:76-88)
   T vg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  nbrs_exact(S, xv, cell, S.fstart, S.xm,
-             [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
-               const vec4_t<T> vj = S.v[j];
-               const T dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
-               const T f = S.K.gradf(r2);
-               const T g[3] = {f * d0, f * d1, f * d2};
+  auto grad_term = [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
+    const vec4_t<T> vj = S.v[j];
+    const T dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+    const T f = S.K.gradf(r2);
+    const T g[3] = {f * d0, f * d1, f * d2};
 #pragma unroll
-               for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-                 for (int r = 0; r < 3; ++r)
-                   vg[r + 3 * c] += pj.w * (dv[r] * g[c]);
-             });
+      for (int r = 0; r < 3; ++r)
+        vg[r + 3 * c] += pj.w * (dv[r] * g[c]);
+  };
+  if (kList) {
+    const size_t n = static_cast<size_t>(S.nf);
+    fluid_pairs_list(S, L, lb, i, xv,
+                     [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj, uint32_t e) {
+                       const T w = S.K.w(r2);
+                       same += pj.w * w;
+                       prho += (pj.w / S.rho0) * w;
+                       if (e < kSelfEntry)
+                         S.pcoef[coef_index(n, i, e)] = pj.w * w;
+                       if (e != kSelfEntry)  // the particle's own term is zero
+                         grad_term(j, d0, d1, d2, r2, pj);
+                     });
+    nbrs_exact(S, xv, cell, S.bstart, S.bxV,
+               [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pb) { prho += pb.w * S.K.w(r2); });
+    prho *= S.rho0;
+  } else {
+    nbrs_exact(S, xv, cell, S.fstart, S.xm,
+               [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pj) {
+                 same += pj.w * S.K.w(r2);
+               });
+    nbrs_exact(S, xv, cell, S.fstart, S.xm,
+               [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pj) {
+                 prho += (pj.w / S.rho0) * S.K.w(r2);
+               });
+    nbrs_exact(S, xv, cell, S.bstart, S.bxV,
+               [&](uint32_t, T, T, T, T r2, const vec4_t<T> &pb) { prho += pb.w * S.K.w(r2); });
+    prho *= S.rho0;
+    nbrs_exact(S, xv, cell, S.fstart, S.xm, grad_term);
+  }
 #pragma unroll
   for (int k = 0; k < 9; ++k)
     vg[k] /= rhoi;
@@ -96,32 +153,50 @@ __global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, T xi) {
   S.prho[i] = prho;
 }
 
-// ---- vorticity diffusion rhs
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs(SceneView<T> S, T xi) {
+// ---- vorticity diffusion rhs; with kList the coefficients m_j W_ij of the
+// set-up are streamed and only w_j is gathered (the particle's own term is
+// zero)
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs(SceneView<T> S, NbrList L, T xi) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xv = S.xm[i];
   const vec4_t<T> wi = S.om[i];
   T a0 = 0, a1 = 0, a2 = 0;
-  nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
-             [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
-               const vec4_t<T> wj = S.om[j];
-               const T w = S.K.w(r2);
-               a0 += (pj.w * (wi.x - wj.x)) * w;
-               a1 += (pj.w * (wi.y - wj.y)) * w;
-               a2 += (pj.w * (wi.z - wj.z)) * w;
-             });
+  auto exact = [&] {
+    nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
+               [&](uint32_t j, T, T, T, T r2, const vec4_t<T> &pj) {
+                 const vec4_t<T> wj = S.om[j];
+                 const T w = S.K.w(r2);
+                 a0 += (pj.w * (wi.x - wj.x)) * w;
+                 a1 += (pj.w * (wi.y - wj.y)) * w;
+                 a2 += (pj.w * (wi.z - wj.z)) * w;
+               });
+  };
+  if (!kList || (L.info[i] & kListOverflow)) {
+    exact();
+  } else {
+    const size_t n = static_cast<size_t>(S.nf);
+    list_visit_indexed(L, Spans::of(L.blk[lb]), n, i, L.info[i], [&](uint32_t e, uint32_t j) {
+      const vec4_t<T> wj = S.om[j];
+      const T c = S.pcoef[coef_index(n, i, e)];
+      a0 += c * (wi.x - wj.x);
+      a1 += c * (wi.y - wj.y);
+      a2 += c * (wi.z - wj.z);
+    });
+  }
   S.orhs[i] = make4<vec4_t<T>>(a0 * xi, a1 * xi, a2 * xi, T(0));
 }
 
 // ---- velocity reconstruction rhs
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S) {
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S, NbrList L) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned lb = kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xv = S.xm[i];
@@ -130,19 +205,26 @@ __global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S) {
   for (int k = 0; k < 9; ++k)
     Gi[k] = S.tvg[9 * static_cast<size_t>(i) + k];
   T a[3] = {0, 0, 0};
-  nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm,
-             [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
-               const T *Gj = S.tvg + 9 * static_cast<size_t>(j);
-               T M[9];
+  auto term = [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj) {
+    const T *Gj = S.tvg + 9 * static_cast<size_t>(j);
+    T M[9];
 #pragma unroll
-               for (int k = 0; k < 9; ++k)
-                 M[k] = Gi[k] + Gj[k];
-               const T mh = pj.w * T(0.5);
-               const T w = S.K.w(r2);
+    for (int k = 0; k < 9; ++k)
+      M[k] = Gi[k] + Gj[k];
+    const T mh = pj.w * T(0.5);
+    const T w = S.K.w(r2);
 #pragma unroll
-               for (int r = 0; r < 3; ++r)
-                 a[r] += (mh * (M[r] * d0 + (M[r + 3] * d1 + M[r + 6] * d2))) * w;
-             });
+    for (int r = 0; r < 3; ++r)
+      a[r] += (mh * (M[r] * d0 + (M[r + 3] * d1 + M[r + 6] * d2))) * w;
+  };
+  if (kList)
+    fluid_pairs_list(S, L, lb, i, xv,
+                     [&](uint32_t j, T d0, T d1, T d2, T r2, const vec4_t<T> &pj, uint32_t e) {
+                       if (e != kSelfEntry)  // x_ii = 0: the particle's own term is zero
+                         term(j, d0, d1, d2, r2, pj);
+                     });
+  else
+    nbrs_exact(S, xv, S.fcell[i], S.fstart, S.xm, term);
   S.vrhs[i] = make4<vec4_t<T>>(a[0], a[1], a[2], T(0));
 }
 
@@ -186,35 +268,6 @@ __device__ __forceinline__ T product_p(const SceneView<T> &S, int i, T r) {
 }
 
 // ---- list form of the products (fused steps)
-// index of the coefficient of list entry e of particle i (chunks of 4
-// entries, so a lane reads 4 coefficients with one load and a wavefront's
-// load is contiguous)
-__device__ __forceinline__ size_t coef_index(size_t n, int i, uint32_t e) {
-  return ((static_cast<size_t>(e / 4) * n + static_cast<size_t>(i)) * 4) + e % 4;
-}
-
-// m_j W_ij of every list entry (particles whose list overflowed keep the
-// grid walk)
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_coef(SceneView<T> S, NbrList L) {
-#pragma clang fp contract(off)
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
-  if (i >= S.nf)
-    return;
-  const uint32_t info = L.info[i];
-  if (info & kListOverflow)
-    return;
-  const vec4_t<T> xv = S.xm[i];
-  const size_t n = static_cast<size_t>(S.nf);
-  list_visit_indexed(L, Spans::of(L.blk[lb]), n, i, info, [&](uint32_t e, uint32_t j) {
-    const vec4_t<T> pj = S.xm[j];
-    const T d0 = xv.x - pj.x, d1 = xv.y - pj.y, d2 = xv.z - pj.z;
-    const T r2 = d0 * d0 + (d1 * d1 + d2 * d2);
-    S.pcoef[coef_index(n, i, e)] = pj.w * S.K.w(r2);
-  });
-}
-
 // (A w)_i = c_i w_i - m_i W(0) w_i - sum_list (m_j W_ij) w_j with w_j at
 // w[ws * j] (ws = 4: one component of a real4 field); the coefficients of a
 // chunk of 8 entries and the 8 gathers are issued before the arithmetic
@@ -411,34 +464,45 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_apply(SceneView<T> S, int wh
 }  // namespace
 
 template <class T>
-void Pt16Ops<T>::setup(hipStream_t st, const SceneView<T> &S, T xi) {
-  if (S.nf > 0)
-    hipLaunchKernelGGL(k_pt16_setup<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, xi);
+void Pt16Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, T xi) {
+  if (S.nf <= 0)
+    return;
+  if (L)
+    hipLaunchKernelGGL((k_pt16_setup<T, true>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S,
+                       *L, xi);
+  else
+    hipLaunchKernelGGL((k_pt16_setup<T, false>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st,
+                       S, NbrList{}, xi);
 }
 
 template <class T>
-void Pt16Ops<T>::omega_rhs(hipStream_t st, const SceneView<T> &S, T xi) {
-  if (S.nf > 0)
-    hipLaunchKernelGGL(k_pt16_omega_rhs<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S,
-                       xi);
+void Pt16Ops<T>::omega_rhs(hipStream_t st, const SceneView<T> &S, const NbrList *L, T xi) {
+  if (S.nf <= 0)
+    return;
+  if (L)
+    hipLaunchKernelGGL((k_pt16_omega_rhs<T, true>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0,
+                       st, S, *L, xi);
+  else
+    hipLaunchKernelGGL((k_pt16_omega_rhs<T, false>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0,
+                       st, S, NbrList{}, xi);
 }
 
 template <class T>
-void Pt16Ops<T>::vrec_rhs(hipStream_t st, const SceneView<T> &S) {
-  if (S.nf > 0)
-    hipLaunchKernelGGL(k_pt16_vrec_rhs<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+void Pt16Ops<T>::vrec_rhs(hipStream_t st, const SceneView<T> &S, const NbrList *L) {
+  if (S.nf <= 0)
+    return;
+  if (L)
+    hipLaunchKernelGGL((k_pt16_vrec_rhs<T, true>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st,
+                       S, *L);
+  else
+    hipLaunchKernelGGL((k_pt16_vrec_rhs<T, false>), dim3(grid_blocks(S.nf)), dim3(kBlock), 0,
+                       st, S, NbrList{});
 }
 
 template <class T>
 void Pt16Ops<T>::spin(hipStream_t st, const SceneView<T> &S) {
   if (S.nf > 0)
     hipLaunchKernelGGL(k_pt16_spin<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
-}
-
-template <class T>
-void Pt16Ops<T>::coefficients(hipStream_t st, const SceneView<T> &S, const NbrList &L) {
-  if (S.nf > 0)
-    hipLaunchKernelGGL(k_pt16_coef<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S, L);
 }
 
 template <class T>

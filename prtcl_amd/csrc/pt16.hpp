@@ -13,22 +13,22 @@ namespace prtcl_amd {
 
 template <class T>
 struct Pt16Ops {
+  // With a neighbour list L (fused steps) the fluid sums walk the step's
+  // list, the set-up writes the pair coefficients m_j W_ij for the step, and
+  // the vorticity right-hand side and the CG products stream them; without,
+  // every sum walks the grid in the reference's order.
   // pt16.setup (pt16.hpp:213-342); xi = strain_rate_viscosity
-  static void setup(hipStream_t st, const SceneView<T> &S, T xi);
+  static void setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, T xi);
   // vorticity diffusion rhs: xi sum_f (m_j (w_i - w_j)) W_ij (pt16.hpp:378-418)
-  static void omega_rhs(hipStream_t st, const SceneView<T> &S, T xi);
+  static void omega_rhs(hipStream_t st, const SceneView<T> &S, const NbrList *L, T xi);
   // velocity reconstruction rhs: sum_f ((m_j 0.5) ((G_i + G_j) x_ij)) W_ij
   // (pt16.hpp:588-606)
-  static void vrec_rhs(hipStream_t st, const SceneView<T> &S);
+  static void vrec_rhs(hipStream_t st, const SceneView<T> &S, const NbrList *L);
   // target velocity gradient += cross_product_matrix(0.5 w) (pt16.hpp:521-535)
   static void spin(hipStream_t st, const SceneView<T> &S);
   // which = 0: vorticity diffusion (system diagonal pt16_same_density, guess
   // and result in vorticity[dim]); which = 1: velocity reconstruction
   // (diagonal pt16_density, guess and result in velocity[dim])
-  // With a neighbour list L (fused steps), the products use the pair
-  // coefficients written by coefficients() for the step's list; without, they
-  // walk the grid in the reference's order.
-  static void coefficients(hipStream_t st, const SceneView<T> &S, const NbrList &L);
   static void cg_setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, int which,
                        int dim, double *part, double tol, int max_k, unsigned seq);
   // split: the alpha and beta reductions in one-block kernels of their own

@@ -971,8 +971,8 @@ class SceneImpl final : public SceneBase {
   // solve_velocity_reconstruction (pt16.hpp:213-760); each solve is one scalar
   // PCG per vector component with tolerance pt16_maximum_error and cap
   // pt16_maximum_iterations, pt16_iterations = the last component's count
-  // with `fused` the CG products use the step's neighbour list (the pair
-  // coefficients are written by the set-up)
+  // with `fused` the fluid sums and the CG products use the step's
+  // neighbour list (the pair coefficients are written by the set-up)
   void run_pt16(int proc, bool fused = false) {
     if (desc.implicit_viscosity != 3)
       throw std::logic_error("the scene was created without pt16");
@@ -1010,13 +1010,11 @@ class SceneImpl final : public SceneBase {
     int r = pf.begin(st, kStageWkbb18Diag);
     switch (proc) {
       case PRTCL_HIP_P_PT16_SETUP:
-        Pt16Ops<T>::setup(st, S, xi);
-        if (fused)
-          Pt16Ops<T>::coefficients(st, S, L);
+        Pt16Ops<T>::setup(st, S, pl, xi);
         pf.end(st, r);
         break;
       case PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION:
-        Pt16Ops<T>::omega_rhs(st, S, xi);
+        Pt16Ops<T>::omega_rhs(st, S, pl, xi);
         pf.end(st, r);
         solve(0);
         Pt16Ops<T>::spin(st, S);
@@ -1026,7 +1024,7 @@ class SceneImpl final : public SceneBase {
         pf.end(st, r);
         break;
       case PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION:
-        Pt16Ops<T>::vrec_rhs(st, S);
+        Pt16Ops<T>::vrec_rhs(st, S, pl);
         pf.end(st, r);
         solve(1);
         break;
