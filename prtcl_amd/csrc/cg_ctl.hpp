@@ -132,6 +132,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, 
 // needs is staged in cg_iter_next beside alpha.
 constexpr unsigned kFusedCgBlocks = 1024;
 
+// cg_done as one value for the whole block. Block 0 of a folded kernel may
+// set it while the wavefronts of another block start, so each wavefront
+// reading it itself could see different values; a block that split would
+// reduce with missing wavefronts. Thread 0 reads it for the block.
+__device__ __forceinline__ bool cg_done_for_block(const Ctl *c) {
+  __shared__ int flag;
+  if (threadIdx.x == 0)
+    flag = c->cg_done;
+  __syncthreads();
+  const bool done = flag != 0;
+  __syncthreads();  // a later call writes flag again
+  return done;
+}
+
 // k_cg_alpha in every block of the update kernel: false when the solve
 // breaks (|p.q| < tol), else alpha
 __device__ __forceinline__ bool cg_alpha_in_block(Ctl *c, const double *pq_part, unsigned nb,

@@ -425,7 +425,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_pt16_cg_update_alpha(SceneView<T> S,
                                                                  const double *pq_part,
                                                                  double *part) {
-  if (S.ctl->cg_done)
+  if (cg_done_for_block(S.ctl))
     return;
   double alpha;
   if (!cg_alpha_in_block(S.ctl, pq_part, gridDim.x, &alpha))
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_update_alpha(SceneView<T> S,
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pt16_cg_direction_beta(SceneView<T> S,
                                                                    const double *part) {
-  if (S.ctl->cg_done)
+  if (cg_done_for_block(S.ctl))
     return;
   double beta;
   if (!cg_beta_in_block(S.ctl, part, gridDim.x, &beta))
