@@ -127,9 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, 
 // launches of at most kFusedCgBlocks blocks: every block sums all partials of
 // the kernel before it (sum_partials_all: the order of the one-block
 // kernels, so every block derives the same scalars and decisions), and block
-// 0 writes them to the control block. Nothing a block reads from the control
-// block is written in the same kernel: the iteration count the stop test
-// needs is staged in cg_iter_next beside alpha.
+// 0 writes them to the control block. The iteration count the stop test
+// needs is staged in cg_iter_next beside alpha, so no value a block reads is
+// written in the same kernel except cg_done, which a block reads once
+// (cg_done_for_block): a block that sees it set would have decided the same.
 constexpr unsigned kFusedCgBlocks = 1024;
 
 // cg_done as one value for the whole block. Block 0 of a folded kernel may
