@@ -45,6 +45,9 @@ STAGE_BYTES = {
     "pcg_spmv": (44, 16),                     # q = A p and p.q, per CG iteration
     "pcg_vector": (156, 0),                   # x, r, y updates, dots, new p
 }
+# the pt16 solves are scalar systems (one vector component at a time): the
+# product reads x, m of the particle, p, the diagonal and writes q
+STAGE_BYTES_PT16 = {"pcg_spmv": (28, 0)}
 
 
 def profiled_traffic(stage):
@@ -187,7 +190,8 @@ def main():
     stages = {k: v for k, v in prof.items() if v[1] > 0}
     dom = max((k for k in stages if k != "pcg_vector"), key=lambda k: stages[k][0])
     dom_ms, dom_n = stages[dom]
-    per_f, per_b = STAGE_BYTES.get(dom, (0, 0))
+    table = dict(STAGE_BYTES, **STAGE_BYTES_PT16) if sc.implicit_viscosity == 3 else STAGE_BYTES
+    per_f, per_b = table.get(dom, (0, 0))
     bytes_launch = per_f * nf + per_b * nb_active
     avg_s = dom_ms / dom_n / 1e3
     achieved = bytes_launch / avg_s / 1e9
