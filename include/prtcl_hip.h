@@ -364,8 +364,9 @@ int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *step
  * it from a captured hipGraph; bit 4 gives the alpha and beta reductions of
  * every CG iteration (WKBB18 and pt16 solves) one-block kernels of their own
  * (they are otherwise folded into the vector kernels of scenes of up to
- * 262,144 fluid particles); -1 restarts the tuner. scene < 0 addresses the
- * schedule of batched steps. */
+ * 262,144 fluid particles); bit 5 walks the neighbour list twice for the
+ * WKBB18 diagonal and the CG set-up of a fused step instead of once; -1
+ * restarts the tuner. scene < 0 addresses the schedule of batched steps. */
 int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask);
 int prtcl_hip_get_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t *mask);
 

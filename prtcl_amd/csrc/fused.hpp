@@ -18,9 +18,12 @@ struct FusedSchedule {
   // the CG iterations (WKBB18, pt16) launch their one-block alpha and beta
   // kernels even where the vector kernels can derive the scalars themselves
   bool cg_split = false;
+  // the WKBB18 diagonal and the CG set-up of a fused step in two list walks
+  // instead of one
+  bool wk_split = false;
   int mask() const {
     return (setup_lds ? 1 : 0) | (accel_lds ? 2 : 0) | (pressure_lds ? 4 : 0) | (eager ? 8 : 0) |
-           (cg_split ? 16 : 0);
+           (cg_split ? 16 : 0) | (wk_split ? 32 : 0);
   }
   static FusedSchedule from_mask(int m) {
     FusedSchedule s;
@@ -29,6 +32,7 @@ struct FusedSchedule {
     s.pressure_lds = (m & 4) != 0;
     s.eager = (m & 8) != 0;
     s.cg_split = (m & 16) != 0;
+    s.wk_split = (m & 32) != 0;
     return s;
   }
 };

@@ -892,7 +892,9 @@ class SceneImpl final : public SceneBase {
 
   // wkbb18[_gc].compute_diagonal / wkbb18[_gc].accumulate_acceleration;
   // with `fused` the products walk the step's neighbour list
-  void run_wkbb18(int proc, bool fused = false) {
+  // with_diagonal (fused steps): the accumulate procedure also computes the
+  // diagonal, in the same walk over the list as the CG set-up
+  void run_wkbb18(int proc, bool fused = false, bool with_diagonal = false) {
     if (!desc.implicit_viscosity)
       throw std::logic_error("the scene was created without implicit viscosity");
     hipStream_t st = res_->stream;
@@ -914,8 +916,15 @@ class SceneImpl final : public SceneBase {
     const unsigned seq = ++seq_;
     const int max_k = std::max(0, desc.wkbb18_maximum_iterations);
     const bool split = tuner_.current().cg_split;
-    int r = pf.begin(st, kStagePcgVector);
-    Wkbb18Ops<T>::setup(st, S, pl, gc, P_.cg_part, desc.wkbb18_maximum_error, max_k, seq);
+    int r;
+    if (with_diagonal && pl) {
+      r = pf.begin(st, kStageWkbb18Diag);
+      Wkbb18Ops<T>::diagonal_setup(st, S, *pl, gc, P_.cg_part, desc.wkbb18_maximum_error, max_k,
+                                   seq);
+    } else {
+      r = pf.begin(st, kStagePcgVector);
+      Wkbb18Ops<T>::setup(st, S, pl, gc, P_.cg_part, desc.wkbb18_maximum_error, max_k, seq);
+    }
     pf.end(st, r);
     const Progress p = cg_loop(seq, max_k, predicted_cg_, [&](int tag) {
       int rp = pf.begin(st, kStagePcgSpmv, tag);
@@ -1267,8 +1276,12 @@ class SceneImpl final : public SceneBase {
         pf.end(st, r);
         if (desc.implicit_viscosity == 1) {
           // example.lua:427-433: diagonal, PCG, integrate velocity again
-          run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL, fused);
-          run_wkbb18(PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION, fused);
+          if (fused && !sc.wk_split) {
+            run_wkbb18(PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION, fused, true);
+          } else {
+            run_wkbb18(PRTCL_HIP_P_WKBB18_COMPUTE_DIAGONAL, fused);
+            run_wkbb18(PRTCL_HIP_P_WKBB18_ACCUMULATE_ACCELERATION, fused);
+          }
           r = pf.begin(st, kStageGravityIntegrateV);
           SphOps<T>::integrate_v(st, S);
           pf.end(st, r);
@@ -1292,8 +1305,12 @@ class SceneImpl final : public SceneBase {
           const NbrList L = nbr_list();
           Wkbb18Ops<T>::correction(st, S, fused ? &L : nullptr);
           pf.end(st, r);
-          run_wkbb18(PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL, fused);
-          run_wkbb18(PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION, fused);
+          if (fused && !sc.wk_split) {
+            run_wkbb18(PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION, fused, true);
+          } else {
+            run_wkbb18(PRTCL_HIP_P_WKBB18_GC_COMPUTE_DIAGONAL, fused);
+            run_wkbb18(PRTCL_HIP_P_WKBB18_GC_ACCUMULATE_ACCELERATION, fused);
+          }
           r = pf.begin(st, kStageGravityIntegrateV);
           SphOps<T>::integrate_v(st, S);
           pf.end(st, r);

@@ -454,24 +454,19 @@ __device__ __forceinline__ double rcp_hw<double>(double x) {
   return 1.0 / x;
 }
 
-template <class T, bool kGC>
-__device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrList &L, int i,
-                                             const vec4_t<T> *__restrict__ w, int ws,
-                                             T out[3]) {
-  const vec4_t<T> xq = S.xq[i];  // (x_i, m_i / rho_i)
-  const vec4_t<T> wi = w[static_cast<size_t>(ws) * i];
-  const T rhoi = S.rho[i];
-  const T cf = T(10) * S.mu_f / rhoi;
-  const size_t n = static_cast<size_t>(S.nf);
-  // kernel constants of the closed-form gradient factor
-  const T h = S.K.h, inv_h = T(1) / h;
-  const T ag3 = T(3) * S.K.alpha / (h * h * h * h), eps2 = S.K.eps * S.K.eps;
-  T l0 = 0, l1 = 0, l2 = 0;
-  struct XW {
-    vec4_t<T> x, w;  // (x_j, m_j / rho_j) and w_j
-  };
-  auto use = [&](const XW &q) {
-    const T d0 = xq.x - q.x.x, d1 = xq.y - q.x.y, d2 = xq.z - q.x.z;
+// one fluid pair of the list product: l += (m_j / rho_j) ((w_i - w_j) . d)
+// f(r) / (r^2 + 0.01 h^2) d with d = x_i - x_j; xq = (x_i, m_i / rho_i),
+// qx = (x_j, m_j / rho_j)
+template <class T>
+struct ProductPair {
+  T inv_h, ag3, eps2, hh001;
+  __device__ __forceinline__ explicit ProductPair(const SceneView<T> &S)
+      : inv_h(T(1) / S.K.h), ag3(T(3) * S.K.alpha / (S.K.h * S.K.h * S.K.h * S.K.h)),
+        eps2(S.K.eps * S.K.eps), hh001(S.hh001) {}
+  __device__ __forceinline__ void operator()(const vec4_t<T> &xq, const vec4_t<T> &wi,
+                                             const vec4_t<T> &qx, const vec4_t<T> &wj, T &l0,
+                                             T &l1, T &l2) const {
+    const T d0 = xq.x - qx.x, d1 = xq.y - qx.y, d2 = xq.z - qx.z;
     const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
     const T rinv = rsqrt_hw(r2);
     const T qh = r2 * rinv * inv_h;
@@ -479,28 +474,22 @@ __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrLis
     T u = T(2) - T(2) * qh;
     u = u > T(0) ? u : T(0);
     const T f = r2 < eps2 ? T(0) : ag3 * (u * u - t * t) * rinv;
-    const T s = q.x.w *
-                ((wi.x - q.w.x) * d0 + (wi.y - q.w.y) * d1 + (wi.z - q.w.z) * d2) * f *
-                rcp_hw(r2 + S.hh001);
+    const T s = qx.w * ((wi.x - wj.x) * d0 + (wi.y - wj.y) * d1 + (wi.z - wj.z) * d2) * f *
+                rcp_hw(r2 + hh001);
     l0 += s * d0;
     l1 += s * d1;
     l2 += s * d2;
-  };
-  const uint32_t info = L.info[i];
-  if (!(info & kListOverflow)) {
-    list_gather<4>(L, Spans::of(L.blk[i / kBlock]), n, i, info,
-                   [&](uint32_t j) {
-                     return XW{S.cxp[2 * static_cast<size_t>(j)], w[static_cast<size_t>(ws) * j]};
-                   },
-                   use);
-  } else {
-    nbrs_exact(S, xq, S.fcell[i], S.fstart, S.xq,
-               [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
-                 if (j != static_cast<uint32_t>(i))
-                   use(XW{S.xq[j], w[static_cast<size_t>(ws) * j]});
-               });
   }
-  const vec4_t<T> bd = S.cb[i], bo = S.G[i];
+};
+
+// out = w_i - dt (10 mu_f / rho_i l + B_i w_i) from the fluid pair sum l
+// and B_i = (bd: xx yy zz, bo: xy xz yz); with kGC L_i applies to the bracket
+template <class T, bool kGC>
+__device__ __forceinline__ void product_finish(const SceneView<T> &S, int i,
+                                               const vec4_t<T> &wi, T l0, T l1, T l2,
+                                               const vec4_t<T> &bd, const vec4_t<T> &bo,
+                                               T out[3]) {
+  const T cf = T(10) * S.mu_f / S.rho[i];
   const T b0 = bd.x * wi.x + bo.x * wi.y + bo.y * wi.z;
   const T b1 = bo.x * wi.x + bd.y * wi.y + bo.z * wi.z;
   const T b2 = bo.y * wi.x + bo.z * wi.y + bd.z * wi.z;
@@ -515,6 +504,36 @@ __device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrLis
     out[1] = wi.y - S.dt * (cf * l1 + b1);
     out[2] = wi.z - S.dt * (cf * l2 + b2);
   }
+}
+
+template <class T, bool kGC>
+__device__ __forceinline__ void product_list(const SceneView<T> &S, const NbrList &L, int i,
+                                             const vec4_t<T> *__restrict__ w, int ws,
+                                             T out[3]) {
+  const vec4_t<T> xq = S.xq[i];  // (x_i, m_i / rho_i)
+  const vec4_t<T> wi = w[static_cast<size_t>(ws) * i];
+  const size_t n = static_cast<size_t>(S.nf);
+  const ProductPair<T> pair(S);
+  T l0 = 0, l1 = 0, l2 = 0;
+  struct XW {
+    vec4_t<T> x, w;  // (x_j, m_j / rho_j) and w_j
+  };
+  auto use = [&](const XW &q) { pair(xq, wi, q.x, q.w, l0, l1, l2); };
+  const uint32_t info = L.info[i];
+  if (!(info & kListOverflow)) {
+    list_gather<4>(L, Spans::of(L.blk[i / kBlock]), n, i, info,
+                   [&](uint32_t j) {
+                     return XW{S.cxp[2 * static_cast<size_t>(j)], w[static_cast<size_t>(ws) * j]};
+                   },
+                   use);
+  } else {
+    nbrs_exact(S, xq, S.fcell[i], S.fstart, S.xq,
+               [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
+                 if (j != static_cast<uint32_t>(i))
+                   use(XW{S.xq[j], w[static_cast<size_t>(ws) * j]});
+               });
+  }
+  product_finish<T, kGC>(S, i, wi, l0, l1, l2, S.cb[i], S.G[i], out);
 }
 
 template <class T>
@@ -557,6 +576,124 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, 
     r.z = ax[2] - b.z;
     r.w = T(0);
     const vec4_t<T> y = precond(S.sp + 9 * static_cast<size_t>(i), r);
+    S.cx[i] = b;
+    S.cr[i] = r;
+    S.cy[i] = y;
+    S.cxp[2 * static_cast<size_t>(i) + 1] = make4<vec4_t<T>>(-y.x, -y.y, -y.z, T(0));
+    acc[0] = static_cast<double>(dot3r<T>(b, b));
+    acc[1] = static_cast<double>(dot3r<T>(r, r));
+    acc[2] = static_cast<double>(dot3r<T>(r, y));
+  }
+  store_partials<3>(part, gridDim.x, acc, lb);
+}
+
+// k_diag_list and the list form of k_cg_setup in one walk over the list
+// (fused steps): every pair term is the expression of the two kernels, in the
+// same order, with (x_j, m_j / rho_j) formed from x_j, m_j and rho_j as the
+// diagonal pass writes it, so the results equal those of the two kernels
+template <class T, bool kGC>
+__global__ __launch_bounds__(kBlock) void k_diag_setup_list(SceneView<T> S, NbrList L,
+                                                            double *part) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < S.nf) {
+    const vec4_t<T> xi = S.xm[i];
+    const T rhoi = S.rho[i];
+    const vec4_t<T> wi = S.v[i];  // the guess x0 = v
+    const vec4_t<T> xqi = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w / rhoi);
+    const size_t n = static_cast<size_t>(S.nf);
+    T D[6] = {0, 0, 0, 0, 0, 0};  // symmetric: xx yy zz xy xz yz
+    const T cf = T(10) * S.mu_f / rhoi;
+    const ProductPair<T> pair(S);
+    T l0 = 0, l1 = 0, l2 = 0;
+    struct P {
+      vec4_t<T> x;  // (x_j, m_j)
+      T rho;
+      vec4_t<T> w;  // v_j
+    };
+    auto both = [&](const P &q) {
+      const T d0 = xi.x - q.x.x, d1 = xi.y - q.x.y, d2 = xi.z - q.x.z;
+      const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const T k = cf * q.x.w / q.rho * gradf_fast(S, r2) / (r2 + S.hh001);
+      D[0] += k * d0 * d0;
+      D[1] += k * d1 * d1;
+      D[2] += k * d2 * d2;
+      D[3] += k * d0 * d1;
+      D[4] += k * d0 * d2;
+      D[5] += k * d1 * d2;
+      pair(xqi, wi, make4<vec4_t<T>>(q.x.x, q.x.y, q.x.z, q.x.w / q.rho), q.w, l0, l1, l2);
+    };
+    const uint32_t info = L.info[i];
+    if (!(info & kListOverflow)) {
+      list_gather<4>(L, Spans::of(L.blk[i / kBlock]), n, i, info,
+                     [&](uint32_t j) { return P{S.xm[j], S.rho[j], S.v[j]}; }, both);
+    } else {
+      nbrs_exact(S, xi, S.fcell[i], S.fstart, S.xm,
+                 [&](uint32_t j, T, T, T, T, const vec4_t<T> &) {
+                   if (j != static_cast<uint32_t>(i))
+                     both(P{S.xm[j], S.rho[j], S.v[j]});
+                 });
+    }
+    T B[6] = {0, 0, 0, 0, 0, 0};
+    const T cb = T(10) * S.mu_b / rhoi * S.rho0 / rhoi;
+    nbrs_exact(S, xi, S.fcell[i], S.bstart, S.bxV, [&](uint32_t, T d0, T d1, T d2, T r2e,
+                                                         const vec4_t<T> &pj) {
+      (void)r2e;
+      const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const T k = cb * pj.w * gradf_fast(S, r2) / (r2 + S.hh001);
+      B[0] += k * d0 * d0;
+      B[1] += k * d1 * d1;
+      B[2] += k * d2 * d2;
+      B[3] += k * d0 * d1;
+      B[4] += k * d0 * d2;
+      B[5] += k * d1 * d2;
+    });
+    T M[9];
+    const T dt = S.dt;
+    const T Dt[6] = {D[0] + B[0], D[1] + B[1], D[2] + B[2],
+                     D[3] + B[3], D[4] + B[4], D[5] + B[5]};
+    if (kGC) {
+      const T Ds[9] = {Dt[0], Dt[3], Dt[4], Dt[3], Dt[1], Dt[5], Dt[4], Dt[5], Dt[2]};
+      const T *Li = S.gc + 9 * static_cast<size_t>(i);
+      T Lr[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        Lr[k] = Li[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          M[r + 3 * c] = ((r == c) ? T(1) : T(0)) -
+                         dt * (Lr[r] * Ds[3 * c] + Lr[r + 3] * Ds[1 + 3 * c] +
+                               Lr[r + 6] * Ds[2 + 3 * c]);
+    } else {
+      M[0] = T(1) - dt * Dt[0];
+      M[4] = T(1) - dt * Dt[1];
+      M[8] = T(1) - dt * Dt[2];
+      M[1] = M[3] = -dt * Dt[3];
+      M[2] = M[6] = -dt * Dt[4];
+      M[5] = M[7] = -dt * Dt[5];
+    }
+    T *spi = S.sp + 9 * static_cast<size_t>(i);
+    invert3(M, spi);
+    const vec4_t<T> bd = make4<vec4_t<T>>(B[0], B[1], B[2], T(0));
+    const vec4_t<T> bo = make4<vec4_t<T>>(B[3], B[4], B[5], T(0));
+    S.cb[i] = bd;
+    S.G[i] = bo;
+    S.xq[i] = xqi;
+    S.cxp[2 * static_cast<size_t>(i)] = xqi;
+    // the CG set-up (k_cg_setup)
+    T ax[3];
+    product_finish<T, kGC>(S, i, wi, l0, l1, l2, bd, bo, ax);
+    vec4_t<T> b = wi;
+    b.w = T(0);
+    vec4_t<T> r;
+    r.x = ax[0] - b.x;
+    r.y = ax[1] - b.y;
+    r.z = ax[2] - b.z;
+    r.w = T(0);
+    const vec4_t<T> y = precond(spi, r);
     S.cx[i] = b;
     S.cr[i] = r;
     S.cy[i] = y;
@@ -714,6 +851,21 @@ void Wkbb18Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L
       hipLaunchKernelGGL((k_cg_setup<T, true, true>), gr, bl, 0, st, S, Lv, part);
     else
       hipLaunchKernelGGL((k_cg_setup<T, true, false>), gr, bl, 0, st, S, Lv, part);
+  }
+  hipLaunchKernelGGL(k_cg_setup_fin, dim3(1), dim3(kBlock), 0, st, S.ctl, part, nb, tol, max_k,
+                     seq);
+}
+
+template <class T>
+void Wkbb18Ops<T>::diagonal_setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
+                                  bool gc, double *part, double tol, int max_k, unsigned seq) {
+  const unsigned nb = grid_blocks(S.nf);
+  if (S.nf > 0) {
+    if (gc)
+      hipLaunchKernelGGL((k_diag_setup_list<T, true>), dim3(nb), dim3(kBlock), 0, st, S, L, part);
+    else
+      hipLaunchKernelGGL((k_diag_setup_list<T, false>), dim3(nb), dim3(kBlock), 0, st, S, L,
+                         part);
   }
   hipLaunchKernelGGL(k_cg_setup_fin, dim3(1), dim3(kBlock), 0, st, S.ctl, part, nb, tol, max_k,
                      seq);

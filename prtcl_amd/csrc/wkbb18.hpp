@@ -15,6 +15,9 @@ struct Wkbb18Ops {
   // x = v, r = A x - b, y = P r, p = -y and the FP64 set-up scalars
   static void setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, bool gc,
                     double *part, double tol, int max_k, unsigned seq);
+  // diagonal() and setup() over the list L in one walk (the same results)
+  static void diagonal_setup(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool gc,
+                             double *part, double tol, int max_k, unsigned seq);
   // one CG iteration in two parts, the product q = A p and the vector work
   // (alpha, x, r, y, beta, p); every kernel exits at once after convergence
   static void iteration_product(hipStream_t st, const SceneView<T> &S, const NbrList *L,

@@ -74,7 +74,9 @@ def test_cg_reductions_in_vector_kernels_equal_split(hip_ctx, implicit_viscosity
     """The CG iterations whose vector kernels derive alpha and beta themselves
     (every block sums all partials, in the order of the one-block kernels)
     equal, bit for bit, the iterations with separate one-block alpha and beta
-    kernels (schedule bit 4), over steps of the viscous column."""
+    kernels (schedule bit 4), and the one list walk for the diagonal and the
+    CG set-up equals the two walks (schedule bit 5), over steps of the viscous
+    column."""
     from prtcl_amd import hip
     sc = viscous_column()
     sc.implicit_viscosity = implicit_viscosity
@@ -82,7 +84,8 @@ def test_cg_reductions_in_vector_kernels_equal_split(hip_ctx, implicit_viscosity
     print(f"case: viscous column, implicit_viscosity {implicit_viscosity}, folded vs split",
           flush=True)
     out = []
-    for mask in (8, 8 | 16):
+    masks = (8, 8 | 16, 8 | 32, 8 | 16 | 32)
+    for mask in masks:
         g = hip.HipScene(hip_ctx, sc)
         g.set_schedule(mask)
         g.prepare()
@@ -93,7 +96,9 @@ def test_cg_reductions_in_vector_kernels_equal_split(hip_ctx, implicit_viscosity
         assert g.schedule() == mask
         out.append((iters, {f: g.download("fluid", f) for f in ("position", "velocity")}))
         g.destroy()
-    (ia, fa), (ib, fb) = out
-    assert ia == ib and min(ia) >= 1
-    for f in fa:
-        np.testing.assert_array_equal(fa[f], fb[f], err_msg=f)
+    ia, fa = out[0]
+    assert min(ia) >= 1
+    for mask, (ib, fb) in zip(masks[1:], out[1:]):
+        assert ia == ib, mask
+        for f in fa:
+            np.testing.assert_array_equal(fa[f], fb[f], err_msg=f"{f}, mask {mask}")
