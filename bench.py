@@ -41,6 +41,9 @@ STAGE_BYTES = {
     "nsearch_sort": (96, 0),
     "gravity_integrate_v": (40, 0),           # v, a, t_birth -> v
     "wkbb18_diagonal": (56, 16),              # x, rho, m -> sp
+    # the diagonal and the CG set-up in one walk (fused steps): x, m, rho, v
+    # in; sp, the boundary matrix B, the (x, m/rho) record, x0, r0, y0, p0 out
+    "wkbb18_diagonal_setup": (156, 16),
     "gradient_correction": (56, 0),           # x, rho, m -> L (3x3)
     "pcg_spmv": (44, 16),                     # q = A p and p.q, per CG iteration
     "pcg_vector": (156, 0),                   # x, r, y updates, dots, new p

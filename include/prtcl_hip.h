@@ -123,7 +123,12 @@ enum prtcl_hip_proc {
   PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION = 18,
   PRTCL_HIP_P_PT16_VORTICITY_PRESERVATION = 19,
   PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION = 20,
-  PRTCL_HIP_P_COUNT = 21
+  /* gravity.hpp:189-215 (gravity.prtcl:26-33): a = -|g| xc_o / |xc_o| with
+   * xc_o the part of x - gravity_center orthogonal to g (0 on the axis) */
+  PRTCL_HIP_P_GRAVITY_INITIALIZE_AXIAL_ACCELERATION = 21,
+  /* gravity.hpp:157-186 (gravity.prtcl:21-25): a += g */
+  PRTCL_HIP_P_GRAVITY_ACCUMULATE_ACCELERATION = 22,
+  PRTCL_HIP_P_COUNT = 23
 };
 
 /* Scene description: counts, arithmetic type and the model's parameter
@@ -167,6 +172,12 @@ typedef struct prtcl_hip_scene_desc {
   double pt16_velocity_maximum_error;
   int32_t pt16_vorticity_maximum_iterations;
   int32_t pt16_velocity_maximum_iterations;
+  /* global gravity_center (gravity.prtcl:13; example.lua sets it to the
+   * domain centre) and the step's gravity procedure: 0 runs
+   * gravity.initialize_acceleration, 1 initialize_axial_acceleration (the
+   * call example.lua:391 makes) */
+  double gravity_center[3];
+  int32_t axial_gravity;
 } prtcl_hip_scene_desc;
 
 /* Fills a descriptor with the reference script defaults. */
@@ -235,7 +246,12 @@ int prtcl_hip_iisph_solve(prtcl_hip_ctx *ctx, int32_t scene, double tol, int32_t
 /* k whole steps (example.lua:363-462 order) of each listed scene. Scenes are
  * advanced together: every kernel of the step sweeps all listed scenes, so a
  * batch of uneven scenes costs one launch per stage rather than one per
- * scene. Per-scene results are independent of which other scenes are listed. */
+ * scene. Per-scene results are independent of which other scenes are listed.
+ * Returns once the steps have finished on the device (as the reference's
+ * procedures return when done): PRTCL_HIP_NONFINITE when a step of a listed
+ * scene produced a NaN or infinite position or speed (the particle state is
+ * left as computed), else PRTCL_HIP_NOT_CONVERGED when an IISPH solve stopped
+ * at its iteration cap, else PRTCL_HIP_OK. */
 int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
                    int32_t k_steps);
 /* Per-scene statistics of the last step: IISPH iterations, relative density

@@ -593,6 +593,42 @@ struct Scene {
         a[3 * i + k] = prm.gravity[k];
   }
 
+  // ---- gravity.accumulate_acceleration (gravity.hpp:157-186): a += g
+  void accumulate_acceleration() {
+    real *a = grp[kFluid].ptr("acceleration");
+    const size_t n = grp[kFluid].n;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k)
+        a[3 * i + k] += prm.gravity[k];
+  }
+
+  // ---- gravity.initialize_axial_acceleration (gravity.hpp:189-215;
+  //      gravity.prtcl:26-33): the acceleration of magnitude |g| towards the
+  //      axis through gravity_center along g, zero on the axis
+  void initialize_axial_acceleration() {
+    real *x = grp[kFluid].ptr("position"), *a = grp[kFluid].ptr("acceleration");
+    const size_t n = grp[kFluid].n;
+    const real *g = prm.gravity, *c = prm.gravity_center;
+    const real eps = static_cast<real>(1e-9);
+    auto rcp0 = [&](real v) { return std::abs(v) > eps ? static_cast<real>(1) / v : real(0); };
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i) {
+      real xc[3], gn[3], xo[3];
+      sub3(x + 3 * i, c, xc);
+      const real ng = norm3(g);
+      const real rg = rcp0(ng);
+      for (int k = 0; k < 3; ++k)
+        gn[k] = g[k] * rg;
+      const real d = dot3(xc, gn);
+      for (int k = 0; k < 3; ++k)
+        xo[k] = xc[k] - d * gn[k];
+      const real ro = rcp0(norm3(xo));
+      for (int k = 0; k < 3; ++k)
+        a[3 * i + k] = ((-ng) * xo[k]) * ro;
+    }
+  }
+
   // ---- viscosity.accumulate_acceleration (viscosity.hpp:171-240)
   void viscosity_accumulate() {
     auto &F = grp[kFluid];
@@ -1284,6 +1320,10 @@ struct Scene {
       compute_density();
     else if (proc == "gravity.initialize_acceleration")
       initialize_acceleration();
+    else if (proc == "gravity.initialize_axial_acceleration")
+      initialize_axial_acceleration();
+    else if (proc == "gravity.accumulate_acceleration")
+      accumulate_acceleration();
     else if (proc == "viscosity.accumulate_acceleration")
       viscosity_accumulate();
     else if (proc == "symplectic_euler.integrate_velocity_with_hard_fade")
@@ -1337,7 +1377,10 @@ struct Scene {
     compute_density();
     if (prm.implicit_viscosity == 2)  // prtcl-lua.bunny.lua:279-281
       correction_compute();
-    initialize_acceleration();
+    if (prm.axial_gravity)  // example.lua:390-391
+      initialize_axial_acceleration();
+    else
+      initialize_acceleration();
     if (prm.explicit_viscosity)
       viscosity_accumulate();
     integrate_velocity_with_hard_fade();

@@ -51,6 +51,11 @@ typedef struct oracle_params {
   float pt16_velocity_maximum_error;
   int32_t pt16_vorticity_maximum_iterations;
   int32_t pt16_velocity_maximum_iterations;
+  /* global gravity_center (gravity.prtcl:13) and the step's choice between
+   * gravity.initialize_acceleration (0) and initialize_axial_acceleration (1,
+   * example.lua:390-391) */
+  float gravity_center[3];
+  int32_t axial_gravity;
 } oracle_params;
 
 typedef struct oracle_step_info {

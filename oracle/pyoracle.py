@@ -42,6 +42,8 @@ class Params(C.Structure):
         ("pt16_velocity_maximum_error", C.c_float),
         ("pt16_vorticity_maximum_iterations", C.c_int32),
         ("pt16_velocity_maximum_iterations", C.c_int32),
+        ("gravity_center", C.c_float * 3),
+        ("axial_gravity", C.c_int32),
     ]
 
 
@@ -185,6 +187,9 @@ class OracleScene:
         p.pt16_velocity_maximum_error = scene.pt16_velocity_maximum_error
         p.pt16_vorticity_maximum_iterations = scene.pt16_vorticity_maximum_iterations
         p.pt16_velocity_maximum_iterations = scene.pt16_velocity_maximum_iterations
+        for k in range(3):
+            p.gravity_center[k] = scene.gravity_center[k]
+        p.axial_gravity = int(scene.axial_gravity)
         self.params = p
         self.L.oracle_set_params(self.h, C.byref(p))
         self.set("fluid", "position", scene.fluid_x)

@@ -563,8 +563,10 @@ prtcl_hip_scene_desc desc_of(Model &m, double radius) {
   read_uniform(g, "time_step", &d.time_step);
   read_uniform(g, "iisph_relaxation", &d.iisph_relaxation);
   read_uniform(g, "fade_duration", &d.fade_duration);
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 3; ++k) {
     read_uniform(g, "gravity", &d.gravity[k], k);
+    read_uniform(g, "gravity_center", &d.gravity_center[k], k);
+  }
   if (f) {
     read_uniform(f->GetUniform(), "rest_density", &d.rest_density);
     read_uniform(f->GetUniform(), "dynamic_viscosity", &d.fluid_viscosity);
@@ -1050,9 +1052,14 @@ void RegisterSchemes() {
     R.RegisterFactory(name_of("gravity"), [] {
       return new HipScheme(name_of("gravity"),
                            {{"initialize_acceleration",
-                             PRTCL_HIP_P_GRAVITY_INITIALIZE_ACCELERATION}},
+                             PRTCL_HIP_P_GRAVITY_INITIALIZE_ACCELERATION},
+                            {"accumulate_acceleration",
+                             PRTCL_HIP_P_GRAVITY_ACCUMULATE_ACCELERATION},
+                            {"initialize_axial_acceleration",
+                             PRTCL_HIP_P_GRAVITY_INITIALIZE_AXIAL_ACCELERATION}},
                            [](Model &m) {
                              m.AddGlobalFieldImpl<float, 3>("gravity");
+                             m.AddGlobalFieldImpl<float, 3>("gravity_center");
                              each_dynamic(m, [](Group &g) {
                                g.AddVaryingFieldImpl<float, 3>("position");
                                g.AddVaryingFieldImpl<float, 3>("acceleration");

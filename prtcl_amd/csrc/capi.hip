@@ -130,6 +130,8 @@ const Named kProcs[] = {
     {"pt16.solve_vorticity_diffusion", PRTCL_HIP_P_PT16_SOLVE_VORTICITY_DIFFUSION},
     {"pt16.vorticity_preservation", PRTCL_HIP_P_PT16_VORTICITY_PRESERVATION},
     {"pt16.solve_velocity_reconstruction", PRTCL_HIP_P_PT16_SOLVE_VELOCITY_RECONSTRUCTION},
+    {"gravity.initialize_axial_acceleration", PRTCL_HIP_P_GRAVITY_INITIALIZE_AXIAL_ACCELERATION},
+    {"gravity.accumulate_acceleration", PRTCL_HIP_P_GRAVITY_ACCUMULATE_ACCELERATION},
 };
 
 }  // namespace

@@ -553,8 +553,25 @@ __global__ __launch_bounds__(kBlock) void k_viscosity_list(SceneView<T> S, NbrLi
     }
   });
   const T H = fade_step(S, vi.w);
-  vout[i] = make4<vec4_t<T>>(vi.x + S.dt * (S.g0 + a0) * H, vi.y + S.dt * (S.g1 + a1) * H,
-                             vi.z + S.dt * (S.g2 + a2) * H, vi.w);
+  T g0, g1, g2;
+  step_gravity(S, xi.x, xi.y, xi.z, g0, g1, g2);
+  vout[i] = make4<vec4_t<T>>(vi.x + S.dt * (g0 + a0) * H, vi.y + S.dt * (g1 + a1) * H,
+                             vi.z + S.dt * (g2 + a2) * H, vi.w);
+}
+
+// v* = v + dt a_axial(x) H (gravity.hpp:189-215, symplectic_euler.hpp:172-200)
+// for steps with axial gravity and no explicit viscosity
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gravity_v(SceneView<T> S, vec4_t<T> *vout) {
+  const int i = static_cast<int>(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= S.nf)
+    return;
+  const vec4_t<T> xi = S.xm[i], vi = S.v[i];
+  const T H = fade_step(S, vi.w);
+  T g0, g1, g2;
+  step_gravity(S, xi.x, xi.y, xi.z, g0, g1, g2);
+  vout[i] = make4<vec4_t<T>>(vi.x + S.dt * g0 * H, vi.y + S.dt * g1 * H,
+                             vi.z + S.dt * g2 * H, vi.w);
 }
 
 // source term s from the integrated velocities (explicit-viscosity path)
@@ -815,8 +832,10 @@ __device__ __forceinline__ void k_speed_finalize_body(Ctl *c, const double *part
   for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
     m = partials[k] > m ? partials[k] : m;
   m = block_max<kBlock>(m);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     max_speed_atomic<T>(c, static_cast<T>(m));
+    publish_step_end(c);
+  }
 }
 
 template <class T>
@@ -880,7 +899,10 @@ void FusedOps<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
   if (S.nf <= 0)
     return;
   const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
-  if (!explicit_viscosity) {
+  // the one-pass setup folds a uniform g into every neighbour's velocity;
+  // explicit viscosity or a position-dependent (axial) gravity need the
+  // integrated velocities first
+  if (!explicit_viscosity && !S.axial) {
     if (lds)
       hipLaunchKernelGGL((k_setup<T, true, true>), gr, bl, 0, st, S, L, vtmp);
     else
@@ -891,7 +913,10 @@ void FusedOps<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
     hipLaunchKernelGGL((k_setup<T, false, true>), gr, bl, 0, st, S, L, vtmp);
   else
     hipLaunchKernelGGL((k_setup<T, false, false>), gr, bl, 0, st, S, L, vtmp);
-  hipLaunchKernelGGL(k_viscosity_list<T>, gr, bl, 0, st, S, L, vtmp);
+  if (explicit_viscosity)
+    hipLaunchKernelGGL(k_viscosity_list<T>, gr, bl, 0, st, S, L, vtmp);
+  else
+    hipLaunchKernelGGL(k_gravity_v<T>, gr, bl, 0, st, S, vtmp);
   // the integrated velocities replace v once every thread has read the old ones
   SceneView<T> S2 = S;
   S2.v = vtmp;

@@ -58,7 +58,31 @@ struct Ctl {
   int cg_iter, cg_done, cg_max_k;
   int cg_iter_next;  // cg_iter + 1, set beside alpha (read while cg_iter is written)
   double cg_tol;
+  // end-of-step report: steps finished (integrations of the position) and
+  // how many of them produced a non-finite position or speed; published to
+  // host_mirror[3] as (step_count << 32) | nonfinite_steps
+  unsigned step_count;
+  unsigned nonfinite_steps;
 };
+
+// host_mirror words: [0] IISPH progress, [1] its aprde / nprde, [2] CG
+// progress, [3] end-of-step report
+constexpr int kMirrorStep = 3;
+
+// Counts the finished step (one thread; after the integration kernel of the
+// step): folds the non-finite flag of the step into the running count,
+// clears it and publishes the report with one system-scope store.
+__device__ __forceinline__ void publish_step_end(Ctl *c) {
+  c->step_count += 1u;
+  c->nonfinite_steps += c->nonfinite ? 1u : 0u;
+  c->nonfinite = 0;
+  if (c->host_mirror) {
+    const unsigned long long w = (static_cast<unsigned long long>(c->step_count) << 32) |
+                                 static_cast<unsigned long long>(c->nonfinite_steps);
+    __hip_atomic_store(c->host_mirror + kMirrorStep, w, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
 
 // device -> host progress of a solve (one 64-bit store at system scope)
 __device__ __forceinline__ void publish_progress(Ctl *c) {
@@ -88,6 +112,8 @@ struct SceneView {
   T thr2;  // neighbour iff squared distance < thr2 (exactly the reference's test)
   T dt, rho0, omega, mu_f, mu_b, fade, t, hh001;  // hh001 = (0.01*h)*h
   T g0, g1, g2;
+  T gc0, gc1, gc2;  // gravity_center
+  int axial;        // the step's gravity is initialize_axial_acceleration
   GridDims grid;
   const uint32_t *fstart;
   const uint32_t *bstart;
@@ -130,6 +156,41 @@ struct SceneView {
   // neighbour come from one cache line
   V4 *cx, *cr, *cxp, *cq, *cy, *cb;
 };
+
+// gravity.initialize_axial_acceleration (gravity.hpp:189-215,
+// gravity.prtcl:26-33) at position x, in the reference's operation order
+// without contraction: xc = x - c, g_n = g rcp0(|g|), xc_o = xc - (xc.g_n) g_n,
+// a = ((-|g|) xc_o) rcp0(|xc_o|), rcp0(v) = 1/v if |v| > 1e-9 else 0
+template <class T>
+__device__ __forceinline__ vec4_t<T> axial_acceleration(const SceneView<T> &S, T x, T y, T z) {
+#pragma clang fp contract(off)
+  const T eps = static_cast<T>(1e-9);
+  const T xc0 = x - S.gc0, xc1 = y - S.gc1, xc2 = z - S.gc2;
+  const T ng = sqrt(sqdist_exact(S.g0, S.g1, S.g2));
+  const T rg = fabs(ng) > eps ? T(1) / ng : T(0);
+  const T n0 = S.g0 * rg, n1 = S.g1 * rg, n2 = S.g2 * rg;
+  const T d = dot3_exact(xc0, xc1, xc2, n0, n1, n2);
+  const T o0 = xc0 - d * n0, o1 = xc1 - d * n1, o2 = xc2 - d * n2;
+  const T no = sqrt(sqdist_exact(o0, o1, o2));
+  const T ro = fabs(no) > eps ? T(1) / no : T(0);
+  return make4<vec4_t<T>>(((-ng) * o0) * ro, ((-ng) * o1) * ro, ((-ng) * o2) * ro, T(0));
+}
+
+// the step's gravity at position x: g, or the axial acceleration
+template <class T>
+__device__ __forceinline__ void step_gravity(const SceneView<T> &S, T x, T y, T z, T &a0, T &a1,
+                                             T &a2) {
+  if (S.axial) {
+    const vec4_t<T> a = axial_acceleration(S, x, y, z);
+    a0 = a.x;
+    a1 = a.y;
+    a2 = a.z;
+  } else {
+    a0 = S.g0;
+    a1 = S.g1;
+    a2 = S.g2;
+  }
+}
 
 struct BlockRanges;
 

@@ -118,7 +118,7 @@ const char *stage_name(int s) {
       "nsearch_sort",      "density",       "gravity_integrate_v", "viscosity",
       "iisph_setup",       "iisph_pressure_acceleration", "iisph_pressure",
       "integrate_position", "wkbb18_diagonal", "pcg_spmv", "pcg_vector",
-      "gradient_correction"};
+      "gradient_correction", "wkbb18_diagonal_setup", "pt16_setup"};
   return (s >= 0 && s < kStageCount) ? names[s] : "?";
 }
 
@@ -777,6 +777,10 @@ class SceneImpl final : public SceneBase {
     S.g0 = static_cast<T>(desc.gravity[0]);
     S.g1 = static_cast<T>(desc.gravity[1]);
     S.g2 = static_cast<T>(desc.gravity[2]);
+    S.gc0 = static_cast<T>(desc.gravity_center[0]);
+    S.gc1 = static_cast<T>(desc.gravity_center[1]);
+    S.gc2 = static_cast<T>(desc.gravity_center[2]);
+    S.axial = desc.axial_gravity ? 1 : 0;
     S.grid = gd_;
     S.fstart = G_.fstart;
     S.bstart = G_.bstart;
@@ -846,9 +850,18 @@ class SceneImpl final : public SceneBase {
         break;
       case PRTCL_HIP_P_DENSITY_COMPUTE_DENSITY: require_grid(); SphOps<T>::density(st, view()); break;
       case PRTCL_HIP_P_GRAVITY_INITIALIZE_ACCELERATION: SphOps<T>::gravity(st, view()); break;
+      case PRTCL_HIP_P_GRAVITY_INITIALIZE_AXIAL_ACCELERATION:
+        SphOps<T>::gravity_axial(st, view());
+        break;
+      case PRTCL_HIP_P_GRAVITY_ACCUMULATE_ACCELERATION:
+        SphOps<T>::gravity_accumulate(st, view());
+        break;
       case PRTCL_HIP_P_VISCOSITY_ACCUMULATE_ACCELERATION: require_grid(); SphOps<T>::viscosity(st, view()); break;
       case PRTCL_HIP_P_SE_INTEGRATE_VELOCITY_WITH_HARD_FADE: SphOps<T>::integrate_v(st, view()); break;
-      case PRTCL_HIP_P_SE_INTEGRATE_POSITION: SphOps<T>::integrate_x(st, view()); break;
+      case PRTCL_HIP_P_SE_INTEGRATE_POSITION:
+        SphOps<T>::integrate_x(st, view());
+        steps_enqueued_ += nf_ > 0 ? 1u : 0u;
+        break;
       case PRTCL_HIP_P_IISPH_SETUP:
         require_grid();
         SphOps<T>::iisph_setup(st, view());
@@ -918,7 +931,7 @@ class SceneImpl final : public SceneBase {
     const bool split = tuner_.current().cg_split;
     int r;
     if (with_diagonal && pl) {
-      r = pf.begin(st, kStageWkbb18Diag);
+      r = pf.begin(st, kStageWkbb18DiagSetup);
       Wkbb18Ops<T>::diagonal_setup(st, S, *pl, gc, P_.cg_part, desc.wkbb18_maximum_error, max_k,
                                    seq);
     } else {
@@ -1016,7 +1029,7 @@ class SceneImpl final : public SceneBase {
         pt16_iters_ = p.iters;
       }
     };
-    int r = pf.begin(st, kStageWkbb18Diag);
+    int r = pf.begin(st, kStagePt16Setup);
     switch (proc) {
       case PRTCL_HIP_P_PT16_SETUP:
         Pt16Ops<T>::setup(st, S, pl, xi);
@@ -1241,7 +1254,10 @@ class SceneImpl final : public SceneBase {
         SphOps<T>::density(st, S);
         pf.end(st, r);
         r = pf.begin(st, kStageGravityIntegrateV);
-        SphOps<T>::gravity(st, S);
+        if (desc.axial_gravity)  // example.lua:390-391
+          SphOps<T>::gravity_axial(st, S);
+        else
+          SphOps<T>::gravity(st, S);
         pf.end(st, r);
         if (desc.explicit_viscosity) {
           r = pf.begin(st, kStageViscosity);
@@ -1267,6 +1283,7 @@ class SceneImpl final : public SceneBase {
       if (fused && !desc.implicit_viscosity) {
         r = pf.begin(st, kStageIntegrateX);
         FusedOps<T>::integrate(st, S);
+        steps_enqueued_ += nf_ > 0 ? 1u : 0u;
         pf.end(st, r);
       } else {
         if (fused)
@@ -1317,6 +1334,7 @@ class SceneImpl final : public SceneBase {
         }
         r = pf.begin(st, kStageIntegrateX);
         SphOps<T>::integrate_x(st, S);
+        steps_enqueued_ += nf_ > 0 ? 1u : 0u;
         pf.end(st, r);
       }
       clock_ += static_cast<double>(static_cast<T>(desc.time_step));
@@ -1325,10 +1343,39 @@ class SceneImpl final : public SceneBase {
     return status;
   }
 
+  int finish_steps() override {
+    if (!arena_)
+      return PRTCL_HIP_OK;
+    const unsigned want = steps_enqueued_;
+    unsigned nonfinite = nonfinite_seen_;
+    auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; ++spin) {
+      const uint64_t w = __atomic_load_n(&mirror_host_[kMirrorStep], __ATOMIC_ACQUIRE);
+      if (static_cast<unsigned>(w >> 32) == want) {
+        nonfinite = static_cast<unsigned>(w);
+        break;
+      }
+      if ((spin & 1023u) == 1023u) {
+        // the stream drained: the control block itself holds the report
+        if (hipStreamQuery(res_->stream) == hipSuccess) {
+          const Ctl c = read_ctl();
+          nonfinite = c.nonfinite_steps;
+          break;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300))
+          throw HipError{hipErrorLaunchTimeOut, "step made no progress for 300 s"};
+      }
+    }
+    const bool bad = nonfinite != nonfinite_seen_;
+    nonfinite_seen_ = nonfinite;
+    return bad ? PRTCL_HIP_NONFINITE : PRTCL_HIP_OK;
+  }
+
   // ------------------------------------------------------- batched stepping
   // a scene the batched step can take: fused passes, no viscosity stage
   bool batchable() const {
     return desc.step_mode == 0 && !desc.explicit_viscosity && !desc.implicit_viscosity &&
+           !desc.axial_gravity &&
            nf_ > 0 && desc.iisph_max_iterations > 0 &&
            grid_blocks(nf_) <= kBatchLocalMask;
   }
@@ -1536,6 +1583,8 @@ class SceneImpl final : public SceneBase {
     tn.finish_step();
     r = pf.begin(st, kStageIntegrateX);
     FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
+    for (SceneImpl *sc : v)
+      sc->steps_enqueued_ += sc->nf_ > 0 ? 1u : 0u;
     pf.end(st, r);
     if (pf.on) {
       PRTCL_HIP_CHECK(hipStreamSynchronize(st));
@@ -1840,7 +1889,7 @@ class SceneImpl final : public SceneBase {
     // pinned, mapped, coherent progress words of the IISPH solve
     PRTCL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&mirror_host_), 64,
                                   hipHostMallocMapped | hipHostMallocCoherent));
-    mirror_host_[0] = mirror_host_[1] = 0;
+    mirror_host_[0] = mirror_host_[1] = mirror_host_[2] = mirror_host_[kMirrorStep] = 0;
     PRTCL_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&mirror_dev_),
                                             mirror_host_, 0));
     ctl_op(kCtlInit);
@@ -1875,6 +1924,9 @@ class SceneImpl final : public SceneBase {
   int pt16_max_k_ = 500, pt16_iters_ = 0;
   int predicted_pt16_[2] = {4, 4};
   unsigned seq_ = 0;
+  // integrations of the position enqueued (each ends with the end-of-step
+  // report) and the non-finite count of the last report read
+  unsigned steps_enqueued_ = 0, nonfinite_seen_ = 0;
   uint64_t *mirror_host_ = nullptr;
   unsigned long long *mirror_dev_ = nullptr;
   int last_iisph_iters_ = 0;
@@ -1936,6 +1988,11 @@ int step_scenes(const std::vector<SceneBase *> &scenes, int k, CtxResources *res
     for (SceneBase *sc : rest)
       keep(sc->step(1));
   }
+  // NaN / Inf in a scene's state is an error and outranks the warning of a
+  // solve that stopped at its cap
+  for (SceneBase *sc : scenes)
+    if (sc->finish_steps() == PRTCL_HIP_NONFINITE)
+      status = PRTCL_HIP_NONFINITE;
   return status;
 }
 

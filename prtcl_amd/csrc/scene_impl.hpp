@@ -22,7 +22,7 @@ namespace prtcl_amd {
 enum Stage {
   kStageSort = 0, kStageDensity, kStageGravityIntegrateV, kStageViscosity, kStageSetup,
   kStagePressureAccel, kStagePressure, kStageIntegrateX, kStageWkbb18Diag, kStagePcgSpmv,
-  kStagePcgVector, kStageCorrection, kStageCount
+  kStagePcgVector, kStageCorrection, kStageWkbb18DiagSetup, kStagePt16Setup, kStageCount
 };
 const char *stage_name(int s);
 
@@ -123,6 +123,10 @@ class SceneBase {
   virtual int iisph_solve(double tol, int min_it, int max_it, int *iters, double *rel,
                           bool wait) = 0;
   virtual int step(int k) = 0;
+  // Waits until every step enqueued so far has finished on the device and
+  // returns PRTCL_HIP_NONFINITE when one of them, since the last call,
+  // produced a non-finite position or speed (else PRTCL_HIP_OK).
+  virtual int finish_steps() = 0;
   virtual void copy_neighbors(int group, int ngroup, uint32_t *offsets, uint32_t *indices,
                               uint64_t *total) = 0;
   virtual void last_permutation(int group, uint32_t *perm) = 0;

@@ -100,6 +100,29 @@ __global__ __launch_bounds__(kBlock) void k_gravity(SceneView<T> S) {
     S.a[i] = make4<vec4_t<T>>(S.g0, S.g1, S.g2, T(0));
 }
 
+// ---- gravity.accumulate_acceleration (gravity.hpp:157-186): a += g
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gravity_accumulate(SceneView<T> S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < S.nf) {
+    vec4_t<T> a = S.a[i];
+    a.x += S.g0;
+    a.y += S.g1;
+    a.z += S.g2;
+    S.a[i] = a;
+  }
+}
+
+// ---- gravity.initialize_axial_acceleration (gravity.hpp:189-215)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gravity_axial(SceneView<T> S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < S.nf) {
+    const vec4_t<T> x = S.xm[i];
+    S.a[i] = axial_acceleration(S, x.x, x.y, x.z);
+  }
+}
+
 // ---- viscosity.accumulate_acceleration (viscosity.hpp:171-240)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_viscosity(SceneView<T> S) {
@@ -374,8 +397,10 @@ __global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double 
   for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
     m = partials[k] > m ? partials[k] : m;
   m = block_max<kBlock>(m);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     atomic_max_speed<T>(c, static_cast<T>(m));
+    publish_step_end(c);
+  }
 }
 
 // ---- neighbour export (Neighborhood::CopyNeighbors, neighborhood.cpp:156-167)
@@ -430,6 +455,14 @@ void SphOps<T>::density(hipStream_t st, const SceneView<T> &S) {
 template <class T>
 void SphOps<T>::gravity(hipStream_t st, const SceneView<T> &S) {
   PRTCL_FLUID_LAUNCH(k_gravity<T>);
+}
+template <class T>
+void SphOps<T>::gravity_accumulate(hipStream_t st, const SceneView<T> &S) {
+  PRTCL_FLUID_LAUNCH(k_gravity_accumulate<T>);
+}
+template <class T>
+void SphOps<T>::gravity_axial(hipStream_t st, const SceneView<T> &S) {
+  PRTCL_FLUID_LAUNCH(k_gravity_axial<T>);
 }
 template <class T>
 void SphOps<T>::viscosity(hipStream_t st, const SceneView<T> &S) {

@@ -11,6 +11,8 @@ struct SphOps {
   static void volume(hipStream_t st, const SceneView<T> &S);
   static void density(hipStream_t st, const SceneView<T> &S);
   static void gravity(hipStream_t st, const SceneView<T> &S);
+  static void gravity_axial(hipStream_t st, const SceneView<T> &S);
+  static void gravity_accumulate(hipStream_t st, const SceneView<T> &S);
   static void viscosity(hipStream_t st, const SceneView<T> &S);
   static void integrate_v(hipStream_t st, const SceneView<T> &S);
   static void integrate_x(hipStream_t st, const SceneView<T> &S);

@@ -50,6 +50,8 @@ PROCS = {
     "wkbb18_gc.compute_diagonal": 15, "wkbb18_gc.accumulate_acceleration": 16,
     "pt16.setup": 17, "pt16.solve_vorticity_diffusion": 18, "pt16.vorticity_preservation": 19,
     "pt16.solve_velocity_reconstruction": 20,
+    "gravity.initialize_axial_acceleration": 21,
+    "gravity.accumulate_acceleration": 22,
 }
 
 # exported symbols declared in include/prtcl_hip.h
@@ -115,6 +117,8 @@ class SceneDesc(C.Structure):
         ("pt16_velocity_maximum_error", C.c_double),
         ("pt16_vorticity_maximum_iterations", C.c_int32),
         ("pt16_velocity_maximum_iterations", C.c_int32),
+        ("gravity_center", C.c_double * 3),
+        ("axial_gravity", C.c_int32),
     ]
 
 
@@ -218,6 +222,9 @@ def desc_for(scene, real_type: int = 0, step_mode: int = 0) -> SceneDesc:
     d.pt16_velocity_maximum_error = float(np.float32(scene.pt16_velocity_maximum_error))
     d.pt16_vorticity_maximum_iterations = int(scene.pt16_vorticity_maximum_iterations)
     d.pt16_velocity_maximum_iterations = int(scene.pt16_velocity_maximum_iterations)
+    for k in range(3):
+        d.gravity_center[k] = float(np.float32(scene.gravity_center[k]))
+    d.axial_gravity = int(scene.axial_gravity)
     return d
 
 

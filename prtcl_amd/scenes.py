@@ -81,6 +81,10 @@ class Scene:
     wkbb18_maximum_error: float = 1e-4
     wkbb18_maximum_iterations: int = 200
     permute_every: int = 8
+    # gravity.initialize_axial_acceleration instead of initialize_acceleration
+    # (example.lua:390-391) about the axis through gravity_center along g
+    axial_gravity: bool = False
+    gravity_center: tuple = (0.0, 0.0, 0.0)
     steps: int = 100
 
     @property

@@ -160,7 +160,9 @@ static SceneFile read_scene(const char *path) {
   return s;
 }
 
-static int gpu_check(const char *path, int steps) {
+// axial: the loop runs gravity.initialize_axial_acceleration (example.lua:391)
+// about the axis through gravity_center = (0.2, 0.15, 0) along g
+static int gpu_check(const char *path, int steps, bool axial) {
   const SceneFile sf = read_scene(path);
   hip::RegisterSchemes();
   Model model;
@@ -187,6 +189,10 @@ static int gpu_check(const char *path, int steps) {
   auto grav = model.AddGlobalFieldImpl<float, 3>("gravity");
   for (int k = 0; k < 3; ++k)
     grav[k] = static_cast<float>(sf.g[k]);
+  const float center[3] = {0.2f, 0.15f, 0.0f};
+  auto gcen = model.AddGlobalFieldImpl<float, 3>("gravity_center");
+  for (int k = 0; k < 3; ++k)
+    gcen[k] = center[k];
   auto t_now = model.AddGlobalFieldImpl<float>("current_time");
   *f.AddUniformFieldImpl<float>("rest_density") = static_cast<float>(sf.rho0);
   std::memcpy(f.GetVarying().FieldSpan<float, 3>("position").data(), sf.fx.data(), 4 * sf.fx.size());
@@ -210,7 +216,8 @@ static int gpu_check(const char *path, int steps) {
     *t_now = static_cast<float>(t);
     nh.Update();
     density->RunProcedure("compute_density", nh);
-    gravity->RunProcedure("initialize_acceleration", nh);
+    gravity->RunProcedure(axial ? "initialize_axial_acceleration" : "initialize_acceleration",
+                          nh);
     se->RunProcedure("integrate_velocity_with_hard_fade", nh);
     iisph->RunProcedure("setup", nh);
     int it = 0;
@@ -240,8 +247,11 @@ static int gpu_check(const char *path, int steps) {
   d.rest_density = static_cast<float>(sf.rho0);
   d.iisph_relaxation = static_cast<float>(sf.omega);
   d.fade_duration = static_cast<float>(sf.fade);
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 3; ++k) {
     d.gravity[k] = static_cast<float>(sf.g[k]);
+    d.gravity_center[k] = center[k];
+  }
+  d.axial_gravity = axial ? 1 : 0;
   d.iisph_tolerance = sf.tol;
   d.iisph_max_iterations = static_cast<int32_t>(sf.max_it);
   d.permute_every = 0;
@@ -595,7 +605,8 @@ int main(int argc, char **argv) {
     if (argc >= 2 && std::string(argv[1]) == "cpu")
       return cpu_checks();
     if (argc >= 4 && std::string(argv[1]) == "gpu")
-      return gpu_check(argv[2], std::atoi(argv[3]));
+      return gpu_check(argv[2], std::atoi(argv[3]),
+                       argc >= 5 && std::string(argv[4]) == "axial");
     if (argc >= 3 && std::string(argv[1]) == "archive")
       return archive_checks(argv[2]);
     if (argc >= 4 && std::string(argv[1]) == "vtk")
@@ -607,6 +618,6 @@ int main(int argc, char **argv) {
     std::printf("FAIL exception: %s\n", s);
     return 1;
   }
-  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS | archive DIR | vtk IN OUT\n");
+  std::printf("usage: compat_check cpu | gpu SCENE_FILE STEPS [axial] | archive DIR | vtk IN OUT\n");
   return 2;
 }

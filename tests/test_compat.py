@@ -79,7 +79,8 @@ def test_compat_archive_matches_restatement(tmp_path):
 
 
 @pytest.mark.gpu
-def test_compat_step_loop_matches_c_abi(tmp_path):
+@pytest.mark.parametrize("gravity", ["uniform", "axial"])
+def test_compat_step_loop_matches_c_abi(tmp_path, gravity):
     from prtcl_amd import scenes
     sc = scenes.tiny(8, steps=6)
     path = str(tmp_path / "scene.bin")
@@ -91,6 +92,7 @@ def test_compat_step_loop_matches_c_abi(tmp_path):
                   *sc.gravity, sc.iisph_tolerance, sc.iisph_max_iterations,
                   sc.fade_duration], np.float64).tofile(fh)
     exe = build_check(tmp_path)
-    r = subprocess.run([exe, "gpu", path, "6"], capture_output=True, text=True, timeout=120)
+    args = [exe, "gpu", path, "6"] + (["axial"] if gravity == "axial" else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
