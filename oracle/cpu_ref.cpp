@@ -67,6 +67,21 @@ inline void sub3(const real *a, const real *b, real *out) {
   out[2] = a[2] - b[2];
 }
 
+// Planted mistakes for the invariant tests (tests/test_oracle_invariants.py).
+// Built with -DORACLE_MUTANT=k, the hook numbered k returns its planted value
+// (a wrong factor, sign or index) instead of the reference's, so the tests can
+// show that each invariant catches such a mistake. The default build
+// (ORACLE_MUTANT 0) is the restatement itself: every hook returns its first
+// argument and folds away at compile time.
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+template <class V>
+constexpr V mutant(int k, V reference, V planted) {
+  return ORACLE_MUTANT == k ? planted : reference;
+}
+const real kZero3[3] = {0, 0, 0};
+
 // reciprocal_or_zero (eigen.hpp:292-300)
 inline real reciprocal_or_zero(real x, real eps) {
   if (std::abs(x) > eps)
@@ -552,6 +567,8 @@ struct Scene {
         copy_neighbors(kBoundary, i, nb);
         V[i] = 0;
         for (size_t j : nb[kBoundary]) {
+          if (mutant<bool>(9, false, j == i))
+            continue;
           real d[3];
           sub3(x + 3 * i, x + 3 * j, d);
           V[i] += kernel_w(d, h);
@@ -642,9 +659,10 @@ struct Scene {
       for (size_t j : nb[kFluid]) {
         real d[3], dv[3], gw[3];
         sub3(x + 3 * i, x + 3 * j, d);
-        sub3(v + 3 * i, v + 3 * j, dv);
+        sub3(v + 3 * i, mutant<const real *>(6, v + 3 * j, kZero3), dv);
         kernel_grad(d, h, gw);
-        real s = (((static_cast<real>(10) * (mu / rho[i])) * (m[j] / rho[j])) *
+        real s = (((static_cast<real>(10) * (mu / rho[i])) *
+                   (m[j] / rho[mutant<size_t>(5, j, i)])) *
                   dot3(dv, d)) /
                  (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
         for (int k = 0; k < 3; ++k)
@@ -720,7 +738,9 @@ struct Scene {
         real d[3], gw[3];
         sub3(x + 3 * i, bx + 3 * j, d);
         kernel_grad(d, h, gw);
-        real f = ((static_cast<real>(0.7) * static_cast<real>(2)) * rho0) *
+        real f = (mutant<real>(1, static_cast<real>(0.7) * static_cast<real>(2),
+                               static_cast<real>(0.7)) *
+                  rho0) *
                  (V[j] / (rho[i] * rho[i]));
         for (int k = 0; k < 3; ++k)
           ci[k] -= f * gw[k];
@@ -735,11 +755,11 @@ struct Scene {
       for (size_t j : nb[kFluid]) {
         real d[3], dv[3], gw[3];
         sub3(x + 3 * i, x + 3 * j, d);
-        sub3(v + 3 * i, v + 3 * j, dv);
+        sub3(v + 3 * i, mutant<const real *>(8, v + 3 * j, kZero3), dv);
         kernel_grad(d, h, gw);
         s[i] += m[j] * dot3(dv, gw);
         AA[i] += m[j] * dot3(c + 3 * i, gw);
-        AA[i] -= (m[j] * (m[i] / (rho[i] * rho[i]))) * sqnorm3(gw);
+        AA[i] -= mutant<real>(2, 1, -1) * ((m[j] * (m[i] / (rho[i] * rho[i]))) * sqnorm3(gw));
       }
       for (size_t j : nb[kBoundary]) {
         real d[3], gw[3];
@@ -770,7 +790,8 @@ struct Scene {
         real d[3], gw[3];
         sub3(x + 3 * i, x + 3 * j, d);
         kernel_grad(d, h, gw);
-        real f = m[j] * ((p[i] / (rho[i] * rho[i])) + (p[j] / (rho[j] * rho[j])));
+        const size_t jr = mutant<size_t>(7, j, i);
+        real f = m[j] * ((p[i] / (rho[i] * rho[i])) + (p[j] / (rho[jr] * rho[jr])));
         for (int k = 0; k < 3; ++k)
           ai[k] -= f * gw[k];
       }
@@ -779,7 +800,7 @@ struct Scene {
         sub3(x + 3 * i, bx + 3 * j, d);
         kernel_grad(d, h, gw);
         real f = ((static_cast<real>(0.7) * rho0) * V[j]) *
-                 ((static_cast<real>(2) * p[i]) / (rho[i] * rho[i]));
+                 ((mutant<real>(3, 2, 1) * p[i]) / (rho[i] * rho[i]));
         for (int k = 0; k < 3; ++k)
           ai[k] -= f * gw[k];
       }
@@ -810,7 +831,7 @@ struct Scene {
         sub3(x + 3 * i, bx + 3 * j, d);
         kernel_grad(d, h, gw);
         real ai[3] = {a[3 * i] - 0, a[3 * i + 1] - 0, a[3 * i + 2] - 0};
-        Ap[i] += (rho0 * V[j]) * dot3(ai, gw);
+        Ap[i] += mutant<real>(4, 1, -1) * ((rho0 * V[j]) * dot3(ai, gw));
       }
       Ap[i] *= (dt * dt) * (rho0 / rho[i]);
       p[i] = std::max(static_cast<real>(0),
@@ -1099,9 +1120,10 @@ struct Scene {
         grad(d, gw);
         real f = ((static_cast<real>(10) * (mu / rho[i])) * (m[j] / rho[j])) /
                  (sqnorm3(d) + ((static_cast<real>(0.01) * h) * h));
+        const real *dc = mutant<const real *>(11, d, x + 3 * i);
         for (int col = 0; col < 3; ++col)
           for (int row = 0; row < 3; ++row)
-            D[row + 3 * col] += f * (gw[row] * d[col]);
+            D[row + 3 * col] += f * (gw[row] * dc[col]);
       }
       for (size_t j : nb[kBoundary]) {
         real d[3], gw[3];
@@ -1167,7 +1189,7 @@ struct Scene {
         for (size_t j : nb[kFluid]) {
           real d[3], dw[3], gw[3];
           sub3(x + 3 * i, x + 3 * j, d);
-          sub3(w + 3 * i, w + 3 * j, dw);
+          sub3(w + 3 * i, mutant<const real *>(10, w + 3 * j, kZero3), dw);
           kernel_grad(d, h, gw);
           real f = (((static_cast<real>(10) * (mu / rho[i])) * (m[j] / rho[j])) *
                     dot3(dw, d)) /

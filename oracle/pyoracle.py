@@ -92,15 +92,17 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(path: str = None):
+    """The oracle library (default build), or the build at `path` (the
+    invariant tests load builds with a planted mistake)."""
+    path = LIB_PATH if path is None else path
+    if path not in _libs:
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
         L.oracle_scene_create.restype = vp
         L.oracle_scene_create.argtypes = [i64, i64]
@@ -134,8 +136,8 @@ def lib():
         L.oracle_sphere_trace.argtypes = [vp, i64, C.c_double, C.POINTER(Camera), C.c_double,
                                           i64, vp, vp]
         L.oracle_sphere_trace.restype = C.c_int
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a: np.ndarray):
@@ -158,8 +160,9 @@ BOUNDARY_FIELDS = {"position": 3, "volume": 1}
 class OracleScene:
     """One scene in the CPU restatement."""
 
-    def __init__(self, scene, fp64_reductions: bool = False, permute_every=None):
-        self.L = lib()
+    def __init__(self, scene, fp64_reductions: bool = False, permute_every=None,
+                 lib_path: str = None):
+        self.L = lib(lib_path)
         self.nf, self.nb = scene.n_fluid, scene.n_boundary
         self.h = self.L.oracle_scene_create(self.nf, self.nb)
         p = Params()
