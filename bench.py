@@ -92,6 +92,12 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the batched workload (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-viscous", action="store_true",
+                    help="skip the viscous-column sub-run of the default workload")
+    ap.add_argument("--no-served", action="store_true",
+                    help="skip the served-stream sub-run of the default workload")
+    ap.add_argument("--serve-duration", type=float, default=20.0,
+                    help="trace length (s) of the served-stream sub-run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -102,7 +108,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the step "
-              "path of this build uses one GPU", file=sys.stderr)
+              "path of this library uses one GPU", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
@@ -129,9 +135,56 @@ def main():
         sc.name = "viscous_gc_column_250k"
     sc.permute_every = 8
     ctx = hip.Context(0)
+    out, g = bench_scene(ctx, sc, args.steps, args.warmup, schedules=True,
+                         traffic=args.workload == "dambreak_1m")
+    out = {
+        "metric": "IISPH particle-steps/s at 1M particles; batched-scene p50/p99 step latency",
+        "value": out.pop("value"), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic", **out,
+    }
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc, g, args.cpu_seconds)
+    g.destroy()
+
+    if args.workload == "dambreak_1m":
+        if args.batch_scenes > 0:
+            out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
+        if not args.no_viscous:
+            # config 2: the 250k viscous column (WKBB18 PCG every step)
+            vsc = scenes.config2()
+            vsc.permute_every = 8
+            v, gv = bench_scene(ctx, vsc, min(args.steps, 20), min(args.warmup, 3))
+            gv.destroy()
+            out["viscous"] = v
+        ctx.close()
+        if not args.no_served:
+            out["served"] = bench_serve(args, sub=True)
+    print(json.dumps(out), flush=True)
+
+
+def step_bytes(nf, nb_active, k, kv=0.0, viscous=False):
+    """Algorithmic HBM bytes of one whole step: 332 + 88 k per fluid particle
+    and 16 (4 + 2 k) per active boundary particle for IISPH; a WKBB18 step
+    adds 376 + 200 k_v per fluid particle and 16 (2 + k_v) per active
+    boundary particle (diagonal, the set-up product and one product per PCG
+    iteration)."""
+    b = (332 + 88 * k) * nf + 16 * (4 + 2 * k) * nb_active
+    if viscous:
+        b += (376 + 200 * kv) * nf + 16 * (2 + kv) * nb_active
+    return b
+
+
+def bench_scene(ctx, sc, steps, warmup, schedules=False, traffic=False):
+    """Whole steps of one device-resident scene: the timed region (K steps,
+    no instrumentation), a pass with per-stage HIP events, the roofline of
+    the dominant stage and of the whole step. Returns (dict, scene)."""
+    from prtcl_amd import hip
     g = hip.HipScene(ctx, sc)
     g.prepare()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         g.step(1)
     ctx.synchronize()
 
@@ -144,7 +197,7 @@ def main():
     ctx.synchronize()
     t_start = time.perf_counter()
     prev = t_start
-    for _ in range(args.steps):
+    for _ in range(steps):
         g.step(1)
         info = g.step_info()
         iters.append(info["iisph_iterations"])
@@ -159,7 +212,7 @@ def main():
     # the library's stream around every stage (the events add small gaps, so
     # this pass is kept out of the timed region above)
     ctx.profile(True)
-    prof_steps = min(args.steps, 20)
+    prof_steps = min(steps, 20)
     for _ in range(prof_steps):
         g.step(1)
     ctx.synchronize()
@@ -167,27 +220,26 @@ def main():
     ctx.profile(False)
     tuned = g.schedule()
 
-    # per-launch stage times of both schedules of the fused passes (global
-    # gathers vs LDS-staged slabs), each forced for a few further steps
-    schedules = {}
-    for name, mask in (("global", 0), ("lds", 7)):
-        g.set_schedule(mask)
-        g.step(1)
-        ctx.profile(True)
-        for _ in range(min(prof_steps, 10)):
+    sched = {}
+    if schedules:
+        # per-launch stage times of both schedules of the fused passes (global
+        # gathers vs LDS-staged slabs), each forced for a few further steps
+        for name, mask in (("global", 8), ("lds", 7 | 8)):
+            g.set_schedule(mask)
             g.step(1)
-        ctx.synchronize()
-        pr = ctx.profile_read()
-        ctx.profile(False)
-        schedules[name] = {k: round(v[0] / v[1], 5) for k, v in pr.items()
+            ctx.profile(True)
+            for _ in range(min(prof_steps, 10)):
+                g.step(1)
+            ctx.synchronize()
+            pr = ctx.profile_read()
+            ctx.profile(False)
+            sched[name] = {k: round(v[0] / v[1], 5) for k, v in pr.items()
                            if v[1] > 0 and k in ("iisph_setup", "iisph_pressure_acceleration",
                                                  "iisph_pressure")}
-    g.set_schedule(tuned)
+        g.set_schedule(tuned)
 
     nf = sc.n_fluid
-    value = nf * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-
+    viscous = sc.implicit_viscosity == 1
     # dominant kernel by total time (pcg_vector is not one kernel: the vector
     # kernels of every CG iteration plus the set-up and apply of the solve)
     stages = {k: v for k, v in prof.items() if v[1] > 0}
@@ -201,29 +253,22 @@ def main():
     roofline = {
         "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": profiled_traffic(dom) if args.workload == "dambreak_1m" else None,
+        "traffic": profiled_traffic(dom) if traffic else None,
         "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5),
         "launches": dom_n,
     }
-    # whole-step roofline: algorithmic bytes per step (332 + 88 k per fluid
-    # particle + 16 (4 + 2k) per active boundary particle)
     k_mean = float(np.mean(iters))
-    step_bytes = (332 + 88 * k_mean) * nf + 16 * (4 + 2 * k_mean) * nb_active
-    step_frac = step_bytes / (elapsed / args.steps) / (HBM_PEAK_GBS * 1e9)
-
+    kv_mean = float(np.mean(wk_iters))
+    step_frac = None
+    if sc.implicit_viscosity in (0, 1):
+        sb = step_bytes(nf, nb_active, k_mean, kv_mean, viscous)
+        step_frac = round(sb / (elapsed / steps) / (HBM_PEAK_GBS * 1e9), 5)
     out = {
-        "metric": "IISPH particle-steps/s at 1M particles; batched-scene p50/p99 step latency",
-        "value": round(value, 1),
+        "workload": sc.name,
+        "value": round(nf * steps / elapsed, 1),
         "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
+        "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 4),
         "config": {
             "workload": sc.name, "fluid_particles": nf, "boundary_particles": sc.n_boundary,
             "active_boundary_particles": nb_active, "spacing_m": sc.spacing,
@@ -231,23 +276,20 @@ def main():
             "iisph_max_iterations": sc.iisph_max_iterations, "reductions": "f64",
         },
         "iisph_iterations_mean": round(k_mean, 3),
-        "wkbb18_iterations_mean": round(float(np.mean(wk_iters)), 3),
+        "wkbb18_iterations_mean": round(kv_mean, 3),
         "p50_step_ms": round(1e3 * float(np.percentile(lat, 50)), 4),
         "p99_step_ms": round(1e3 * float(np.percentile(lat, 99)), 4),
-        "step_roofline_frac": round(step_frac, 5),
+        "step_roofline_frac": step_frac,
         "roofline": roofline,
         "stages_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in stages.items()},
-        "schedule": {"tuned_mask": tuned, "ms_per_launch": schedules},
     }
-
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sc, g, args.cpu_seconds)
-
-    if args.batch_scenes > 0 and args.workload == "dambreak_1m":
-        g.destroy()
-        out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
-
-    print(json.dumps(out), flush=True)
+    if viscous:
+        out["config"].update(dynamic_viscosity=sc.fluid_viscosity,
+                             wkbb18_tolerance=sc.wkbb18_maximum_error,
+                             wkbb18_max_iterations=sc.wkbb18_maximum_iterations)
+    if schedules:
+        out["schedule"] = {"tuned_mask": tuned, "ms_per_launch": sched}
+    return out, g
 
 
 def bench_checkpoint(args):
@@ -369,14 +411,19 @@ def bench_checkpoint(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args):
+def bench_serve(args, sub=False):
     """Served stream (config 4): Poisson arrivals of "advance scene s by
     k = 10 steps" over resident scenes with N_f log-uniform in [5k, 500k], 30%
-    with implicit viscosity; scenes admitted against the HBM budget on their
-    first request. Reports request latency percentiles and the aggregate
-    particle-steps/s of the run (scene creation included in the latency of a
-    scene's first request)."""
+    with implicit viscosity; scenes admitted against the HBM budget, each
+    warmed by a few steps before the trace. Reports request latency
+    percentiles, the aggregate particle-steps/s of the run and its HBM
+    roofline fraction (algorithmic bytes of the fluid-particle terms of every
+    step served, with each request's iteration counts taken as those of its
+    last step; boundary terms left out, so a lower bound), over the wall time
+    and over the time the request loop spent stepping. With sub=True the
+    result is returned as a sub-object of the default workload's line."""
     from prtcl_amd import hip, scenes, serving
+    duration = args.serve_duration if sub else args.duration
     t0 = time.perf_counter()
     scs = scenes.config4_scenes(args.serve_scenes)
     gen_s = time.perf_counter() - t0
@@ -390,13 +437,20 @@ def bench_serve(args):
         ss.get(i)
     ctx.synchronize()
     admit_s = time.perf_counter() - t0
-    arr = scenes.poisson_arrivals(4, args.rate, args.duration, len(scs))
+    arr = scenes.poisson_arrivals(4, args.rate, duration, len(scs))
     out = serving.replay(ctx, ss, arr, k=10)
     lat = out["latencies"]
     visc = np.array([scs[i].implicit_viscosity for i in out["latency_scene"]], bool)
     tk = out["tick_ms"]
     nf = sum(s.n_fluid for s in scs)
     nvisc = sum(1 for s in scs if s.implicit_viscosity)
+    # algorithmic bytes of the steps served
+    its = out["last_step_iterations"]
+    req_bytes = 0.0
+    for (k, kv), idx in zip(its, out["latency_scene"]):
+        n = scs[idx].n_fluid
+        req_bytes += 10 * step_bytes(n, 0, float(k), float(kv), bool(scs[idx].implicit_viscosity))
+    busy_s = float(tk.sum()) / 1e3
     res = {
         "metric": "served stream: request latency p50/p99 and particle-steps/s",
         "value": round(out["particle_steps"] / out["wall"], 1),
@@ -406,7 +460,7 @@ def bench_serve(args):
         "data": "synthetic",
         "config": {"workload": "served_stream", "scenes": len(scs), "viscous_scenes": nvisc,
                    "fluid_particles_total": nf, "rate_per_s": args.rate, "k_steps": 10,
-                   "trace_s": args.duration, "hbm_budget_gb": args.budget_gb},
+                   "trace_s": duration, "hbm_budget_gb": args.budget_gb},
         "requests": out["requests"],
         "p50_request_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
         "p99_request_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
@@ -418,10 +472,20 @@ def bench_serve(args):
         "tick_ms_p99": round(float(np.percentile(tk, 99)), 3),
         "tick_ms_max": round(float(tk.max()), 3),
         "scenes_per_tick_mean": round(float(out["active"].mean()), 2),
+        "roofline_frac_wall": round(req_bytes / out["wall"] / (HBM_PEAK_GBS * 1e9), 5),
+        "roofline_frac_busy": round(req_bytes / max(busy_s, 1e-9) / (HBM_PEAK_GBS * 1e9), 5),
+        "busy_s": round(busy_s, 3),
         "wall_s": round(out["wall"], 3), "scene_generation_s": round(gen_s, 2),
         "admission_s": round(admit_s, 2), "warm_steps_per_scene": args.serve_warm_steps,
         "device_gb": round(ctx.bytes_in_use() / 1e9, 3),
     }
+    ctx.close()
+    if sub:
+        for key in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype",
+                    "data", "warmup"):
+            res.pop(key)
+        res["workload"] = "served_stream"
+        return res
     print(json.dumps(res), flush=True)
 
 

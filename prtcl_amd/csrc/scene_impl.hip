@@ -36,7 +36,8 @@ void CtxResources::ensure_batch(size_t bytes) {
   // kernels of earlier batches may still read the old tables, and the
   // captured batch iterations point at them
   PRTCL_HIP_CHECK(hipStreamSynchronize(stream));
-  batch_graphs.clear();
+  for (GraphCache &g : batch_graphs)
+    g.clear();
   if (batch_host)
     PRTCL_HIP_CHECK(hipHostFree(batch_host));
   if (batch_dev)
@@ -50,7 +51,8 @@ void CtxResources::ensure_batch(size_t bytes) {
 }
 
 void CtxResources::release() {
-  batch_graphs.clear();
+  for (GraphCache &g : batch_graphs)
+    g.clear();
   if (batch_host)
     (void)hipHostFree(batch_host);
   if (batch_dev)
@@ -112,6 +114,10 @@ void GraphCache::launch(hipStream_t st, const std::string &key,
   hit->used = ++clock_;
   PRTCL_HIP_CHECK(hipGraphLaunch(hit->exec, st));
 }
+
+// tags of the captured launch sequences (or-ed with the element size)
+constexpr uint32_t kGraphIteration = 0x100u;       // one scene's IISPH iteration
+constexpr uint32_t kGraphBatchIteration = 0x200u;  // a batch's IISPH iteration
 
 const char *stage_name(int s) {
   static const char *names[kStageCount] = {
@@ -313,6 +319,8 @@ class SceneImpl final : public SceneBase {
   ~SceneImpl() override {
     if (mirror_host_)
       (void)hipHostFree(mirror_host_);
+    if (perm_host_)
+      (void)hipHostFree(perm_host_);
     if (arena_)
       (void)hipFree(arena_);
     if (grid_mem_)
@@ -737,23 +745,42 @@ class SceneImpl final : public SceneBase {
   void nsearch_permute() {
     if (!grid_built_)
       nsearch_update(true);
-    hipStream_t st = res_->stream;
-    last_perm_f_.resize(nf_);
-    last_perm_b_.resize(nb_);
+    host_permute(res_->stream);
+    PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
+  }
+
+  // The host order becomes the slot order (id[s] = s), after the gather
+  // permutation (new host index s held old host index id[s]) is copied to
+  // page-locked memory, where last_permutation() finds it. Used by the
+  // procedure and by the permutes inside steps; allocates nothing.
+  void host_permute(hipStream_t st) {
     if (nf_ > 0) {
-      PRTCL_HIP_CHECK(hipMemcpyAsync(last_perm_f_.data(), P_.id, 4 * nf_, hipMemcpyDeviceToHost, st));
+      PRTCL_HIP_CHECK(hipMemcpyAsync(perm_host_, P_.id, 4 * size_t(nf_), hipMemcpyDeviceToHost, st));
       hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nf_)), dim3(kBlock), 0, st, P_.id, nf_);
     }
     if (nb_ > 0) {
-      PRTCL_HIP_CHECK(hipMemcpyAsync(last_perm_b_.data(), P_.bid, 4 * nb_, hipMemcpyDeviceToHost, st));
+      PRTCL_HIP_CHECK(hipMemcpyAsync(perm_host_ + nf_, P_.bid, 4 * size_t(nb_),
+                                     hipMemcpyDeviceToHost, st));
       hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nb_)), dim3(kBlock), 0, st, P_.bid, nb_);
     }
-    PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+    permuted_ = true;
+    perm_in_flight_ = true;
   }
 
   void last_permutation(int group, uint32_t *perm) override {
-    const auto &v = group == PRTCL_HIP_FLUID ? last_perm_f_ : last_perm_b_;
-    std::copy(v.begin(), v.end(), perm);
+    const int n = group == PRTCL_HIP_FLUID ? nf_ : nb_;
+    if (!permuted_) {  // no permute yet: the identity
+      for (int i = 0; i < n; ++i)
+        perm[i] = static_cast<uint32_t>(i);
+      return;
+    }
+    if (perm_in_flight_) {
+      PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
+      perm_in_flight_ = false;
+    }
+    const int32_t *src = perm_host_ + (group == PRTCL_HIP_FLUID ? 0 : nf_);
+    for (int i = 0; i < n; ++i)
+      perm[i] = static_cast<uint32_t>(src[i]);
   }
 
   // ------------------------------------------------------------- procedures
@@ -1136,7 +1163,8 @@ class SceneImpl final : public SceneBase {
     // one graph per buffer arrangement (two, alternating step by step)
     SceneView<T> Sg = S;
     Sg.t = T(0);
-    const std::string key = graph ? graph_key(Sg, L, sc) : std::string();
+    const std::string key =
+        graph ? graph_key(kGraphIteration | sizeof(T), Sg, L, sc) : std::string();
     auto enqueue = [&](int n) {
       if (graph) {
         for (int k = 0; k < n; ++k)
@@ -1228,10 +1256,7 @@ class SceneImpl final : public SceneBase {
       build_fluid_grid(false);
       pf.end(st, r);
       if (desc.permute_every > 0 && steps_since_permute_ >= desc.permute_every) {
-        if (nf_ > 0)
-          hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nf_)), dim3(kBlock), 0, st, P_.id, nf_);
-        if (nb_ > 0)
-          hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nb_)), dim3(kBlock), 0, st, P_.bid, nb_);
+        host_permute(st);
         steps_since_permute_ = 0;
       }
       const bool fused = desc.step_mode == 0;
@@ -1492,13 +1517,8 @@ class SceneImpl final : public SceneBase {
     GridOps<T>::sort_batch(st, d_sort, d_pmap, np, d_tmap, nt, ns);
     pf.end(st, r);
     for (unsigned k = 0; k < ns; ++k)
-      if (permute[k]) {
-        SceneImpl &s = *v[k];
-        hipLaunchKernelGGL(k_iota, dim3(grid_blocks(s.nf_)), dim3(kBlock), 0, st, s.P_.id, s.nf_);
-        if (s.nb_ > 0)
-          hipLaunchKernelGGL(k_iota, dim3(grid_blocks(s.nb_)), dim3(kBlock), 0, st, s.P_.bid,
-                             s.nb_);
-      }
+      if (permute[k])
+        v[k]->host_permute(st);
     ScheduleTuner &tn = res->batch_tuner[sizeof(T) == 8 ? 1 : 0];
     const bool tune = tn.trial();
     const FusedSchedule sc = tn.current();
@@ -1522,11 +1542,13 @@ class SceneImpl final : public SceneBase {
     }
     int enq = 0;
     const bool graph = !sc.eager && !pf.on && !tune;
-    const std::string key = graph ? graph_key(d_solve, d_pmap, np, ns, sc) : std::string();
+    const std::string key =
+        graph ? graph_key(kGraphBatchIteration | sizeof(T), d_solve, d_pmap, np, ns, sc)
+              : std::string();
     auto enqueue = [&](int n) {
       if (graph) {
         for (int j = 0; j < n; ++j)
-          res->batch_graphs.launch(st, key, [&] {
+          res->batch_graphs[sizeof(T) == 8 ? 1 : 0].launch(st, key, [&] {
             FusedOps<T>::accel_batch(st, d_solve, d_pmap, np, sc.accel_lds);
             FusedOps<T>::pressure_batch(st, d_solve, d_pmap, np, ns, sc.pressure_lds);
           });
@@ -1889,6 +1911,11 @@ class SceneImpl final : public SceneBase {
     // pinned, mapped, coherent progress words of the IISPH solve
     PRTCL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&mirror_host_), 64,
                                   hipHostMallocMapped | hipHostMallocCoherent));
+    // page-locked home of the last host-visible permutation (fluid, then
+    // boundary), so the permutes inside steps allocate nothing
+    PRTCL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&perm_host_),
+                                  4 * std::max<size_t>(1, size_t(nf_) + size_t(nb_)),
+                                  hipHostMallocDefault));
     mirror_host_[0] = mirror_host_[1] = mirror_host_[2] = mirror_host_[kMirrorStep] = 0;
     PRTCL_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&mirror_dev_),
                                             mirror_host_, 0));
@@ -1932,7 +1959,8 @@ class SceneImpl final : public SceneBase {
   int last_iisph_iters_ = 0;
   double last_rel_ = 0;
   int last_wkbb_ = 0;
-  std::vector<uint32_t> last_perm_f_, last_perm_b_;
+  int32_t *perm_host_ = nullptr;  // last host-visible permutation (page-locked)
+  bool permuted_ = false, perm_in_flight_ = false;
 };
 
 }  // namespace

@@ -50,7 +50,10 @@ struct Profiler {
 // arguments and replayed with one hipGraphLaunch each. The sequences must not
 // contain memsets (graph memset nodes are avoided on purpose) and must read
 // any per-launch data from device memory, not from captured host values that
-// change between replays.
+// change between replays. A key holds only argument bytes, so one cache must
+// serve one launch sequence of one element type: the FP32 and the FP64
+// batched iterations take the same kinds of arguments (table and block-map
+// pointers, counts) and their keys can coincide.
 class GraphCache {
  public:
   GraphCache() = default;
@@ -74,9 +77,11 @@ class GraphCache {
   static constexpr size_t kMaxEntries = 8;
 };
 
+// the key of a launch sequence: a tag naming the sequence and its element
+// type, then the bytes of its arguments
 template <class... A>
-std::string graph_key(const A &...a) {
-  std::string k;
+std::string graph_key(uint32_t tag, const A &...a) {
+  std::string k(reinterpret_cast<const char *>(&tag), sizeof(tag));
   (k.append(reinterpret_cast<const char *>(&a), sizeof(A)), ...);
   return k;
 }
@@ -101,7 +106,8 @@ struct CtxResources {
   // schedule of the batched fused passes (FP32, FP64) and its trial timing
   ScheduleTuner batch_tuner[2];
   Profiler batch_prof;
-  GraphCache batch_graphs;
+  // captured batched IISPH iterations (FP32, FP64): one cache per type
+  GraphCache batch_graphs[2];
   void ensure_batch(size_t bytes);
   void release();
 };

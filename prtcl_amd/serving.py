@@ -69,6 +69,7 @@ def replay(ctx: hip.Context, scenes: SceneSet, arrivals: Sequence[Tuple[float, i
     lat_scene: List[int] = []
     tick_ms: List[float] = []
     active: List[int] = []
+    iters: List[Tuple[int, int]] = []
     psteps = 0
     ticks = 0
     i = 0
@@ -94,9 +95,12 @@ def replay(ctx: hip.Context, scenes: SceneSet, arrivals: Sequence[Tuple[float, i
                 lat.append(t_done - t_arr)
                 lat_scene.append(idx)
                 psteps += k * scenes.nf[idx]
+                info = scenes.live[idx].step_info()  # the request's last step
+                iters.append((info["iisph_iterations"], info["wkbb18_iterations"]))
         elif i < n:
             time.sleep(max(0.0, min(arrivals[i][0] - (time.perf_counter() - t0), 0.01)))
     wall = time.perf_counter() - t0
     return {"latencies": np.asarray(lat), "latency_scene": np.asarray(lat_scene),
             "particle_steps": psteps, "wall": wall, "ticks": ticks, "requests": len(lat),
-            "tick_ms": np.asarray(tick_ms), "active": np.asarray(active)}
+            "tick_ms": np.asarray(tick_ms), "active": np.asarray(active),
+            "last_step_iterations": np.asarray(iters, np.int64).reshape(-1, 2)}

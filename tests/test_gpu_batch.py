@@ -124,6 +124,41 @@ def test_schedules_bit_identical(hip_ctx, real_type):
                 assert np.array_equal(a[f], b[f]), (mask, f)
 
 
+def test_fp32_and_fp64_batch_graphs_stay_apart(hip_ctx):
+    """Batched IISPH iterations replayed from captured graphs, FP32 scenes
+    first, then FP64 scenes of the same mix: equal block counts and, for three
+    scenes, equal 256-byte-aligned table offsets, so the launch arguments of
+    the two batches are the same bytes. Each type must replay its own graphs:
+    the FP64 batch equals the FP64 scenes stepped with eager launches (an FP32
+    graph replayed on FP64 tables reads them as FP32 entries and dereferences
+    garbage pointers)."""
+    from prtcl_amd import hip
+    mix = small_mix(3, seed=7)
+
+    def go(real_type, mask):
+        hs = [hip.HipScene(hip_ctx, sc, real_type=real_type) for sc in mix]
+        for h in hs:
+            h.prepare()
+        hip_ctx.set_batch_schedule(mask)
+        for _ in range(3):
+            hip_ctx.step(hs, 1)
+        out = [{f: h.download("fluid", f) for f in FIELDS} for h in hs]
+        for h in hs:
+            h.destroy()
+        return out
+
+    print("fp32 batch from graphs, then fp64 batch from graphs and eager", flush=True)
+    try:
+        go(0, 0)
+        from_graphs = go(1, 0)
+        eager = go(1, 8)
+    finally:
+        hip_ctx.set_batch_schedule(-1)
+    for j, (a, b) in enumerate(zip(eager, from_graphs)):
+        for f in FIELDS:
+            assert np.array_equal(a[f], b[f]), (j, f)
+
+
 def test_graph_replay_equals_eager(hip_ctx):
     """The IISPH iteration replayed from captured hipGraphs (default) and
     launched kernel by kernel (schedule bit 3) give identical fields, alone

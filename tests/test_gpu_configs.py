@@ -1,0 +1,275 @@
+"""Parity at the sizes of the benchmark configurations, through the C ABI.
+
+* config 1 (1,000,000 fluid + 400,002 boundary particles): neighbour sets of
+  all four group pairs bit-exact in original-index space at step 0, and again
+  after 100 device steps with the oracle's grid built from the device's
+  positions (grouped_uniform_grid.hpp:407-422, neighborhood.cpp:156-177); and
+  5 whole steps against the oracle's steps, FP32 and the FP64 kernels.
+* config 0 (8,000 fluid): all 200 fused steps against the oracle, FP32 and FP64
+  kernels, IISPH iterations within one per step.
+* config 2 (250,000-particle viscous column): 5 fused steps against the
+  oracle, IISPH and WKBB18 iterations within one per step.
+* config 3 (64 scenes): the batched step equals every scene stepped alone, bit
+  for bit, over 9 steps (a host-visible permute inside).
+* config 4 (the first 64 scenes of the served stream, 30 % viscous): every
+  scene's state after an interleaved request stream under a budget that
+  forces evictions equals the scene stepped alone.
+
+The achieved errors are printed (normwise relative ||x - y|| / ||y|| per
+field, and the largest componentwise difference over the field's largest
+magnitude).
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+from prtcl_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+PAIRS = [(1, 1), (1, 0), (0, 1), (0, 0)]
+
+
+def _errs(x, y):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    nrm = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    mx = float(np.max(np.abs(x - y)) / max(float(np.max(np.abs(y))), 1e-300)) if y.size else 0.0
+    return nrm, mx
+
+
+def _report(what, errs):
+    print(f"  {what}: " + ", ".join(f"{f} {n:.2e} (max {m:.2e})" for f, (n, m) in errs.items()),
+          flush=True)
+
+
+def _oracle_state(o, f):
+    ids = o.get("fluid", "id")
+    inv = np.empty_like(ids)
+    inv[ids] = np.arange(ids.size)
+    return o.get("fluid", f)[inv]
+
+
+def _same_sets(g, o, what):
+    for qg, ng in PAIRS:
+        a, b = g.neighbors(qg, ng), o.neighbors(qg, ng)
+        assert np.array_equal(a[0], b[0]), (what, qg, ng, "offsets")
+        assert np.array_equal(a[1], b[1]), (what, qg, ng, "indices")
+        print(f"  {what} pair {qg}->{ng}: {len(b[1])} neighbour entries identical", flush=True)
+
+
+def test_config1_neighbour_sets_at_steps_0_and_100(hip_ctx, oracle_lib):
+    from prtcl_amd import hip
+    sc = scenes.config1()
+    sc.permute_every = 0  # the host order stays the original index order
+    print(f"config1 neighbour sets: {sc.n_fluid} fluid, {sc.n_boundary} boundary", flush=True)
+    g = hip.HipScene(hip_ctx, sc)
+    g.prepare()
+    o = oracle_lib.OracleScene(sc, permute_every=0)
+    o.update()
+    _same_sets(g, o, "step 0")
+    t = time.time()
+    for _ in range(100):
+        g.step(1)
+    print(f"  100 device steps in {time.time() - t:.2f} s", flush=True)
+    x = g.download("fluid", "position")
+    assert np.all(np.isfinite(x))
+    assert float(np.max(np.abs(x - sc.fluid_x))) > 1e-4  # the particles moved
+    o.set("fluid", "position", x)
+    o.update()
+    _same_sets(g, o, "step 100")
+    g.destroy()
+
+
+@pytest.mark.parametrize("real_type", [0, 1], ids=["fp32", "fp64"])
+def test_config1_steps_against_oracle(hip_ctx, oracle_lib, real_type):
+    from prtcl_amd import hip
+    sc = scenes.config1()
+    sc.permute_every = 0
+    print(f"config1 5 steps real_type={real_type}", flush=True)
+    g = hip.HipScene(hip_ctx, sc, real_type=real_type)
+    o = oracle_lib.OracleScene(sc, fp64_reductions=True, permute_every=8)
+    g.prepare()
+    o.prepare()
+    for s in range(5):
+        info = o.step()
+        g.step(1)
+        gi = g.step_info()["iisph_iterations"]
+        assert abs(gi - info["iisph_iterations"]) <= 1, (s, gi, info)
+    errs = {f: _errs(g.download("fluid", f), _oracle_state(o, f))
+            for f in ("position", "velocity", "density", "pressure")}
+    _report(f"config1 after 5 steps vs oracle ({['fp32', 'fp64'][real_type]})", errs)
+    bars = {"position": 1e-6, "velocity": 1e-4, "density": 1e-4, "pressure": 1e-2}
+    for f, (n, _) in errs.items():
+        assert n < bars[f], (f, n)
+    g.destroy()
+
+
+def test_config0_all_200_steps(hip_ctx, oracle_lib):
+    from prtcl_amd import hip
+    sc = scenes.config0()
+    sc.permute_every = 0
+    print("config0 200 fused steps", flush=True)
+    o = oracle_lib.OracleScene(sc, fp64_reductions=True, permute_every=8)
+    g32 = hip.HipScene(hip_ctx, sc, real_type=0)
+    g64 = hip.HipScene(hip_ctx, sc, real_type=1)
+    for h in (o, g32, g64):
+        h.prepare()
+    worst = 0
+    for s in range(200):
+        info = o.step()
+        g32.step(1)
+        g64.step(1)
+        d = abs(g32.step_info()["iisph_iterations"] - info["iisph_iterations"])
+        worst = max(worst, d)
+        assert d <= 1, (s, g32.step_info(), info)
+    fields = ("position", "velocity", "density", "pressure")
+    e_o = {f: _errs(g32.download("fluid", f), _oracle_state(o, f)) for f in fields}
+    e_64 = {f: _errs(g32.download("fluid", f), g64.download("fluid", f)) for f in fields}
+    e_64o = {f: _errs(g64.download("fluid", f), _oracle_state(o, f)) for f in fields}
+    _report("config0 after 200 steps, fp32 vs oracle", e_o)
+    _report("config0 after 200 steps, fp32 vs fp64 kernels", e_64)
+    _report("config0 after 200 steps, fp64 kernels vs oracle", e_64o)
+    print(f"  largest IISPH iteration difference: {worst}", flush=True)
+    bars = {"position": 1e-4, "velocity": 1e-2, "density": 1e-3, "pressure": 5e-2}
+    for f in fields:
+        assert e_o[f][0] < bars[f], (f, e_o[f])
+        assert e_64[f][0] < bars[f], (f, e_64[f])
+    for h in (g32, g64):
+        h.destroy()
+
+
+def test_config2_viscous_column(hip_ctx, oracle_lib):
+    from prtcl_amd import hip
+    sc = scenes.config2()
+    sc.permute_every = 0
+    print(f"config2 5 fused steps: {sc.n_fluid} fluid, {sc.n_boundary} boundary", flush=True)
+    o = oracle_lib.OracleScene(sc, fp64_reductions=True, permute_every=8)
+    g = hip.HipScene(hip_ctx, sc)
+    for h in (o, g):
+        h.prepare()
+    for s in range(5):
+        info = o.step()
+        g.step(1)
+        gi = g.step_info()
+        print(f"  step {s}: iisph {gi['iisph_iterations']} / {info['iisph_iterations']}, "
+              f"wkbb18 {gi['wkbb18_iterations']} / {info['wkbb18_iterations']}", flush=True)
+        assert abs(gi["iisph_iterations"] - info["iisph_iterations"]) <= 1
+        assert abs(gi["wkbb18_iterations"] - info["wkbb18_iterations"]) <= 1
+    fields = ("position", "velocity", "density", "pressure")
+    e_o = {f: _errs(g.download("fluid", f), _oracle_state(o, f)) for f in fields}
+    _report("config2 after 5 steps, fp32 vs oracle", e_o)
+    bars = {"position": 1e-6, "velocity": 1e-3, "density": 1e-4, "pressure": 1e-2}
+    for f in fields:
+        assert e_o[f][0] < bars[f], (f, e_o[f])
+    g.destroy()
+
+
+def test_config3_batch_equals_alone_all_64(hip_ctx):
+    from prtcl_amd import hip
+    scs = scenes.config3(steps=9)
+    print(f"config3: {len(scs)} scenes, {sum(s.n_fluid for s in scs)} fluid particles", flush=True)
+    fields = ("position", "velocity", "density", "pressure")
+
+    def run(together):
+        hs = [hip.HipScene(hip_ctx, sc) for sc in scs]
+        for h in hs:
+            h.prepare()
+        its = []
+        for _ in range(9):
+            if together:
+                hip_ctx.step(hs, 1)
+            else:
+                for h in hs:
+                    h.step(1)
+            its.append([h.step_info()["iisph_iterations"] for h in hs])
+        out = [{f: h.download("fluid", f) for f in fields} for h in hs]
+        for h in hs:
+            h.destroy()
+        return out, its
+
+    alone, it_a = run(False)
+    batch, it_b = run(True)
+    assert it_a == it_b
+    for j, (a, b) in enumerate(zip(alone, batch)):
+        for f in fields:
+            assert np.all(np.isfinite(b[f])), (j, f)
+            assert np.array_equal(a[f], b[f]), (j, f)
+    print(f"  64 scenes identical after 9 steps; mean IISPH iterations "
+          f"{np.mean(it_b):.2f}", flush=True)
+
+
+def test_config4_served_equals_alone_64_scenes():
+    from prtcl_amd import hip
+    scs = scenes.config4_scenes(64)
+    nvisc = sum(1 for s in scs if s.implicit_viscosity)
+    print(f"config4: 64 scenes ({nvisc} viscous), {sum(s.n_fluid for s in scs)} fluid",
+          flush=True)
+    fields = ("position", "velocity", "density", "pressure")
+    # two requests of 3 steps per scene, interleaved in a fixed shuffled order
+    rng = np.random.default_rng(4)
+    requests = [(i, 3) for i in range(64)] * 2
+    order = rng.permutation(len(requests))
+    requests = [requests[k] for k in order]
+    steps = [6] * 64
+
+    t = time.time()
+    ctx = hip.Context(0)
+    ref = []
+    for sc, n in zip(scs, steps):
+        h = hip.HipScene(ctx, sc)
+        h.prepare()
+        h.step(n)
+        ref.append({f: h.download("fluid", f) for f in fields})
+        h.destroy()
+    total = 0
+    for sc in scs:
+        d = hip.desc_for(sc)
+        total += int(ctx.L.prtcl_hip_scene_footprint(hip.C.byref(d)))
+    ctx.close()
+    print(f"  alone: {time.time() - t:.1f} s; footprint {total / 1e9:.2f} GB", flush=True)
+
+    t = time.time()
+    budget = int(total * 0.5)
+    ctx = hip.Context(0, budget=budget)
+    live = {}
+
+    def get(i):
+        if i not in live:
+            d = hip.desc_for(scs[i])
+            need = int(ctx.L.prtcl_hip_scene_footprint(hip.C.byref(d))) * 2
+            while True:
+                try:
+                    ctx.make_room(need)
+                    break
+                except hip.HipError:
+                    assert ctx.pending()[0] > 0
+                    ctx.tick()
+            h = hip.HipScene(ctx, scs[i])
+            h.prepare()
+            live[i] = h
+        return live[i]
+
+    tickets = {}
+    evicted = 0
+    for n, (s, k) in enumerate(requests):
+        tickets[ctx.submit(get(s), k)] = s
+        if n % 4 == 3:
+            ctx.tick()
+            evicted = max(evicted, sum(1 for h in live.values() if not h.resident()))
+    while ctx.pending()[0] > 0:
+        ctx.tick()
+        evicted = max(evicted, sum(1 for h in live.values() if not h.resident()))
+    assert sorted(ctx.poll()) == sorted(tickets)
+    assert ctx.bytes_in_use() <= budget
+    assert evicted > 0
+    print(f"  served: {time.time() - t:.1f} s; at most {evicted} scenes evicted at once",
+          flush=True)
+    for i, h in live.items():
+        st = {f: h.download("fluid", f) for f in fields}
+        h.evict()
+        for f in fields:
+            assert np.array_equal(st[f], ref[i][f]), (i, f)
+    ctx.close()
