@@ -254,6 +254,22 @@ int prtcl_hip_iisph_solve(prtcl_hip_ctx *ctx, int32_t scene, double tol, int32_t
  * at its iteration cap, else PRTCL_HIP_OK. */
 int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
                    int32_t k_steps);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). prtcl_hip_step's batched step of the listed scenes with
+ * its per-step tables in a fixed 4 MiB device array of the library (one per
+ * GPU, set up with the context) instead of context buffers grown on demand:
+ * it calls no hipMalloc / hipFree, captures no graph and times nothing, and
+ * its results equal prtcl_hip_step's bit for bit. Every listed scene must be
+ * resident, take the batched step (fused IISPH without viscosity or axial
+ * gravity) and have its grid built (by a step or neighborhood.update); all
+ * FP32 or all FP64. Returns PRTCL_HIP_INVALID_ARGUMENT otherwise, and when
+ * the tables exceed 4 MiB (about 2,000 scenes of 10,000 particles). Calls on
+ * one GPU share the array: a call copies its tables in only after the last
+ * kernel of the call before it (any context) that read them, but calls that
+ * overlap in time from different host threads race. */
+int prtcl_hip_step_noalloc(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
+                           int32_t k_steps);
 /* Per-scene statistics of the last step: IISPH iterations, relative density
  * error, WKBB18 iterations. */
 int prtcl_hip_last_step_info(prtcl_hip_ctx *ctx, int32_t scene, int32_t *iisph_iters,

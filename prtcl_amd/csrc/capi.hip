@@ -4,6 +4,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -18,11 +19,50 @@ using prtcl_amd::CtxResources;
 using prtcl_amd::HipError;
 using prtcl_amd::SceneBase;
 
+// Home of the batched tables of prtcl_hip_step_noalloc: a fixed array in
+// device memory (one copy per GPU, held until the process ends), so the entry
+// point allocates nothing. Every call on one GPU shares it and leaves the last
+// call's tables in it; each call writes all of the tables it reads, so nothing
+// in it must start at zero.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+// Per GPU: the device address of `scratch`, looked up once with the GPU
+// current (the array's name in host code is only a stand-in for the symbol),
+// and the event after the last kernel that read it, so that a call on
+// another stream (another context) copies its tables in only after that.
+struct ScratchHome {
+  void *dev = nullptr;
+  hipEvent_t released = nullptr;
+};
+std::mutex g_scratch_mu;
+std::map<int, ScratchHome> g_scratch;
+
+ScratchHome scratch_home(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  if (it != g_scratch.end())
+    return it->second;
+  ScratchHome h;
+  PRTCL_HIP_CHECK(hipGetSymbolAddress(&h.dev, HIP_SYMBOL(scratch)));
+  PRTCL_HIP_CHECK(hipEventCreateWithFlags(&h.released, hipEventDisableTiming));
+  g_scratch[device] = h;
+  return h;
+}
+}  // namespace
+
 struct prtcl_hip_ctx {
   int device = 0;
   uint64_t budget = 0;
   uint64_t in_use = 0;
   CtxResources res;
+  // prtcl_hip_step_noalloc: this GPU's `scratch` (and its release event),
+  // the page-locked host copy of the tables (set up with the context) and the
+  // end of its last copy
+  ScratchHome scratch;
+  void *noalloc_host = nullptr;
+  hipEvent_t noalloc_ev = nullptr;
   std::map<int32_t, std::unique_ptr<SceneBase>> scenes;
   int32_t next_id = 0;
   std::string last_error;
@@ -201,6 +241,9 @@ int prtcl_hip_ctx_create(int device, uint64_t budget, prtcl_hip_ctx **out) {
     std::memset(ctx->res.pinned, 0, 64 * sizeof(int32_t));
     for (auto &e : ctx->res.ev)
       PRTCL_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->scratch = scratch_home(device);
+    PRTCL_HIP_CHECK(hipHostMalloc(&ctx->noalloc_host, kNoAllocScratchBytes, hipHostMallocDefault));
+    PRTCL_HIP_CHECK(hipEventCreateWithFlags(&ctx->noalloc_ev, hipEventDisableTiming));
     return int(PRTCL_HIP_OK);
   });
   if (rc != PRTCL_HIP_OK) {
@@ -226,6 +269,10 @@ int prtcl_hip_ctx_destroy(prtcl_hip_ctx *ctx) {
   for (auto &e : ctx->res.ev)
     if (e)
       (void)hipEventDestroy(e);
+  if (ctx->noalloc_host)
+    (void)hipHostFree(ctx->noalloc_host);
+  if (ctx->noalloc_ev)
+    (void)hipEventDestroy(ctx->noalloc_ev);
   if (ctx->res.stream)
     (void)hipStreamDestroy(ctx->res.stream);
   delete ctx;
@@ -378,6 +425,31 @@ int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
     for (int i = 0; i < n_scenes; ++i)
       list.push_back(find(ctx, scenes[i]));
     return step_scenes(list, k_steps, &ctx->res);
+  });
+}
+
+int prtcl_hip_step_noalloc(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
+                           int32_t k_steps) {
+  return guarded(ctx, [&]() -> int {
+    if (!scenes || n_scenes < 0 || k_steps < 0)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "invalid scene list");
+    std::vector<SceneBase *> list;
+    for (int i = 0; i < n_scenes; ++i)
+      list.push_back(find(ctx, scenes[i]));
+    prtcl_amd::BatchTables t;
+    t.host = ctx->noalloc_host;
+    t.dev = ctx->scratch.dev;
+    t.bytes = kNoAllocScratchBytes;
+    t.copied = ctx->noalloc_ev;
+    t.released = ctx->scratch.released;
+    try {
+      return prtcl_amd::step_scenes_noalloc(list, k_steps, &ctx->res, t);
+    } catch (const prtcl_amd::TablesTooLarge &e) {
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT,
+                  "the batch's tables take " + std::to_string(e.bytes) +
+                      " bytes; prtcl_hip_step_noalloc holds at most " +
+                      std::to_string(kNoAllocScratchBytes));
+    }
   });
 }
 

@@ -1405,16 +1405,24 @@ class SceneImpl final : public SceneBase {
            grid_blocks(nf_) <= kBatchLocalMask;
   }
 
+  // the grid exists, so a batched step allocates nothing for this scene
+  bool stepped_ready() const { return arena_ != nullptr && grid_mem_ != nullptr; }
+
   // One step of every scene in `v` (all batchable, at most 1 << 12), with one
   // launch per stage over the blocks of all scenes. Per scene the kernels,
   // their inputs and the order of every reduction are those of step(), so
   // each scene's results do not depend on the rest of the batch.
-  static int step_batch(const std::vector<SceneImpl *> &v, CtxResources *res) {
+  // With `fixed`, the tables go to the caller's buffers instead of the
+  // context's grown ones: no allocation, no graph capture, no timing (the
+  // launches are eager and the tuner keeps its current choice).
+  static int step_batch(const std::vector<SceneImpl *> &v, CtxResources *res,
+                        const BatchTables *fixed = nullptr) {
     const unsigned ns = static_cast<unsigned>(v.size());
     if (ns == 0)
       return PRTCL_HIP_OK;
     hipStream_t st = res->stream;
-    Profiler &pf = res->prof;
+    Profiler idle;  // never enabled: no events
+    Profiler &pf = fixed ? idle : res->prof;
     // boundary grids and volumes (first step after an upload only)
     for (SceneImpl *sc : v) {
       sc->ensure_grid();
@@ -1443,11 +1451,15 @@ class SceneImpl final : public SceneBase {
     const size_t off_pmap = 3 * tab_bytes;
     const size_t off_tmap = off_pmap + align(sizeof(uint32_t) * np);
     const size_t total = off_tmap + align(sizeof(uint32_t) * nt);
-    res->ensure_batch(total);
+    if (fixed && total > fixed->bytes)
+      throw TablesTooLarge{total};
+    if (!fixed)
+      res->ensure_batch(total);
+    hipEvent_t copied = fixed ? fixed->copied : res->batch_ev;
     // the previous batch's copy out of the pinned tables must have finished
-    PRTCL_HIP_CHECK(hipEventSynchronize(res->batch_ev));
-    char *hb = static_cast<char *>(res->batch_host);
-    char *db = static_cast<char *>(res->batch_dev);
+    PRTCL_HIP_CHECK(hipEventSynchronize(copied));
+    char *hb = static_cast<char *>(fixed ? fixed->host : res->batch_host);
+    char *db = static_cast<char *>(fixed ? fixed->dev : res->batch_dev);
     auto *t_sort = reinterpret_cast<BatchEntry<T> *>(hb);
     auto *t_setup = reinterpret_cast<BatchEntry<T> *>(hb + tab_bytes);
     auto *t_solve = reinterpret_cast<BatchEntry<T> *>(hb + 2 * tab_bytes);
@@ -1505,8 +1517,12 @@ class SceneImpl final : public SceneBase {
       E.seq = seqs[k];
       t_solve[k] = E;
     }
+    // a fixed device buffer may be shared with other streams: the copy into
+    // it waits for the kernels of its last user
+    if (fixed)
+      PRTCL_HIP_CHECK(hipStreamWaitEvent(st, fixed->released, 0));
     PRTCL_HIP_CHECK(hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, st));
-    PRTCL_HIP_CHECK(hipEventRecord(res->batch_ev, st));
+    PRTCL_HIP_CHECK(hipEventRecord(copied, st));
     const auto *d_sort = reinterpret_cast<const BatchEntry<T> *>(db);
     const auto *d_setup = reinterpret_cast<const BatchEntry<T> *>(db + tab_bytes);
     const auto *d_solve = reinterpret_cast<const BatchEntry<T> *>(db + 2 * tab_bytes);
@@ -1520,9 +1536,11 @@ class SceneImpl final : public SceneBase {
       if (permute[k])
         v[k]->host_permute(st);
     ScheduleTuner &tn = res->batch_tuner[sizeof(T) == 8 ? 1 : 0];
-    const bool tune = tn.trial();
-    const FusedSchedule sc = tn.current();
-    Profiler &tp = res->batch_prof;
+    const bool tune = !fixed && tn.trial();
+    FusedSchedule sc = tn.current();
+    if (fixed)
+      sc.eager = true;
+    Profiler &tp = fixed ? idle : res->batch_prof;
     tp.on = tune;
     r = pf.begin(st, kStageSetup);
     int q = tp.begin(st, kStageSetup);
@@ -1602,9 +1620,12 @@ class SceneImpl final : public SceneBase {
       tp.reset();
       tp.on = false;
     }
-    tn.finish_step();
+    if (!fixed)
+      tn.finish_step();
     r = pf.begin(st, kStageIntegrateX);
     FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
+    if (fixed)  // the last kernel reading the tables is enqueued
+      PRTCL_HIP_CHECK(hipEventRecord(fixed->released, st));
     for (SceneImpl *sc : v)
       sc->steps_enqueued_ += sc->nf_ > 0 ? 1u : 0u;
     pf.end(st, r);
@@ -2018,6 +2039,46 @@ int step_scenes(const std::vector<SceneBase *> &scenes, int k, CtxResources *res
   }
   // NaN / Inf in a scene's state is an error and outranks the warning of a
   // solve that stopped at its cap
+  for (SceneBase *sc : scenes)
+    if (sc->finish_steps() == PRTCL_HIP_NONFINITE)
+      status = PRTCL_HIP_NONFINITE;
+  return status;
+}
+
+int step_scenes_noalloc(const std::vector<SceneBase *> &scenes, int k, CtxResources *res,
+                        const BatchTables &tables) {
+  std::vector<SceneImpl<float> *> bf;
+  std::vector<SceneImpl<double> *> bd;
+  for (size_t i = 0; i < scenes.size(); ++i) {
+    for (size_t j = 0; j < i; ++j)
+      if (scenes[j] == scenes[i])
+        throw std::invalid_argument("a scene is listed twice");
+    SceneBase *sc = scenes[i];
+    if (!sc->resident())
+      throw std::invalid_argument("a listed scene is evicted (bringing it back allocates)");
+    if (auto *f = dynamic_cast<SceneImpl<float> *>(sc); f && f->batchable() && f->stepped_ready())
+      bf.push_back(f);
+    else if (auto *d = dynamic_cast<SceneImpl<double> *>(sc);
+             d && d->batchable() && d->stepped_ready())
+      bd.push_back(d);
+    else
+      throw std::invalid_argument(
+          "a listed scene cannot be stepped without allocating: it needs the fused IISPH "
+          "step without viscosity or axial gravity, and its grid built (by a step, or by "
+          "neighborhood.update)");
+  }
+  if (!bf.empty() && !bd.empty())
+    throw std::invalid_argument("FP32 and FP64 scenes in one call share the one table buffer");
+  int status = PRTCL_HIP_OK;
+  for (int s = 0; s < k; ++s) {
+    int rc = PRTCL_HIP_OK;
+    if (!bf.empty())
+      rc = SceneImpl<float>::step_batch(bf, res, &tables);
+    if (!bd.empty())
+      rc = SceneImpl<double>::step_batch(bd, res, &tables);
+    if (rc != PRTCL_HIP_OK && status == PRTCL_HIP_OK)
+      status = rc;
+  }
   for (SceneBase *sc : scenes)
     if (sc->finish_steps() == PRTCL_HIP_NONFINITE)
       status = PRTCL_HIP_NONFINITE;

@@ -174,12 +174,36 @@ class SceneBase {
   virtual void set_steps_since_permute(int k) = 0;
 };
 
+// Caller-provided home of a batched step's tables: `host` (page-locked) is
+// filled and copied to `dev` (device memory of at least `bytes`). `copied`
+// marks the end of the last copy out of `host`; `released` the end of the
+// last kernel that read `dev` (on whichever stream used it last): the next
+// copy into `dev` waits for it.
+struct BatchTables {
+  void *host = nullptr;
+  void *dev = nullptr;
+  size_t bytes = 0;
+  hipEvent_t copied = nullptr;
+  hipEvent_t released = nullptr;
+};
+// thrown when a batch's tables exceed BatchTables::bytes
+struct TablesTooLarge {
+  size_t bytes;
+};
+
 SceneBase *make_scene(const prtcl_hip_scene_desc &desc, CtxResources *res);
 // k whole steps of every listed scene; scenes of the fused path without
 // explicit or implicit viscosity are advanced together (one launch per stage
 // for all of them), the others one after the other. Returns the first
 // non-OK status of a scene.
 int step_scenes(const std::vector<SceneBase *> &scenes, int k, CtxResources *res);
+// The same steps with every batched table in `tables` and nothing allocated:
+// every scene must be resident, take the batched step (fused IISPH without
+// viscosity or axial gravity) and have its grid; all FP32 or all FP64.
+// Throws std::invalid_argument otherwise, TablesTooLarge when the tables do
+// not fit.
+int step_scenes_noalloc(const std::vector<SceneBase *> &scenes, int k, CtxResources *res,
+                        const BatchTables &tables);
 uint64_t scene_particle_bytes(const prtcl_hip_scene_desc &desc);
 
 }  // namespace prtcl_amd

@@ -63,7 +63,8 @@ EXPORTS = [
     "prtcl_hip_scene_save_vtk", "prtcl_hip_scene_trace",
     "prtcl_hip_upload_field", "prtcl_hip_download_field", "prtcl_hip_field_id",
     "prtcl_hip_get_global", "prtcl_hip_set_global", "prtcl_hip_run", "prtcl_hip_proc_id",
-    "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_last_step_info",
+    "prtcl_hip_iisph_solve", "prtcl_hip_step", "prtcl_hip_step_noalloc",
+    "prtcl_hip_last_step_info",
     "prtcl_hip_set_schedule", "prtcl_hip_get_schedule",
     "prtcl_hip_scene_evict", "prtcl_hip_scene_resident", "prtcl_hip_serve_make_room",
     "prtcl_hip_serve_submit", "prtcl_hip_serve_tick", "prtcl_hip_serve_poll",
@@ -163,6 +164,7 @@ def lib():
         L.prtcl_hip_iisph_solve.argtypes = [vp, i32, C.c_double, i32, i32, C.POINTER(i32),
                                             C.POINTER(C.c_double)]
         L.prtcl_hip_step.argtypes = [vp, C.POINTER(i32), i32, i32]
+        L.prtcl_hip_step_noalloc.argtypes = [vp, C.POINTER(i32), i32, i32]
         L.prtcl_hip_set_schedule.argtypes = [vp, i32, i32]
         L.prtcl_hip_scene_describe.argtypes = [vp, i32, C.POINTER(SceneDesc)]
         L.prtcl_hip_scene_save_archive.argtypes = [vp, i32, C.c_char_p, C.POINTER(C.c_uint64)]
