@@ -397,8 +397,11 @@ int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *step
  * every CG iteration (WKBB18 and pt16 solves) one-block kernels of their own
  * (they are otherwise folded into the vector kernels of scenes of up to
  * 262,144 fluid particles); bit 5 walks the neighbour list twice for the
- * WKBB18 diagonal and the CG set-up of a fused step instead of once; -1
- * restarts the tuner. scene < 0 addresses the schedule of batched steps. */
+ * WKBB18 diagonal and the CG set-up of a fused step instead of once; bit 6
+ * runs IISPH iteration 1 as its own pressure-acceleration and pressure
+ * passes instead of folding it into the setup pass (with every pressure 0 at
+ * the start of the solve it needs no neighbours); -1 restarts the tuner.
+ * scene < 0 addresses the schedule of batched steps. */
 int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask);
 int prtcl_hip_get_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t *mask);
 

@@ -637,8 +637,8 @@ int prtcl_hip_serve_pending(prtcl_hip_ctx *ctx, int64_t *requests, int64_t *step
 
 int prtcl_hip_set_schedule(prtcl_hip_ctx *ctx, int32_t scene, int32_t mask) {
   return guarded(ctx, [&]() -> int {
-    if (mask < -1 || mask > 63)
-      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "schedule mask must be -1 or 0..63");
+    if (mask < -1 || mask > 127)
+      return fail(ctx, PRTCL_HIP_INVALID_ARGUMENT, "schedule mask must be -1 or 0..127");
     if (scene < 0) {
       for (auto &t : ctx->res.batch_tuner)
         t.force(mask);

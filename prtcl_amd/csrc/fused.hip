@@ -246,6 +246,29 @@ __device__ __forceinline__ void block_spans(const GridDims &g, const uint32_t *f
   __syncthreads();
 }
 
+// IISPH iteration 1 of a solve without its neighbour passes (the loop of
+// example.lua:409-421 over iisph.hpp:358-516): every pressure is 0 when the
+// solve starts (iisph.hpp:282), so the pressure acceleration of iteration 1
+// is zero, Ap = 0, and the relaxed Jacobi update and the aprde term need only
+// the particle's own s and AA. The expressions are those of
+// k_pressure_list_body with p_old = Ap = 0, so p and the aprde term are the
+// ones the two passes give, bit for bit. Returns the particle's aprde term.
+template <class T>
+__device__ __forceinline__ double first_iteration(const SceneView<T> &S, int i,
+                                                  const vec4_t<T> &xi, T rhoi, T si, T AA) {
+  const T p_old = T(0), Ap = T(0);
+  const T rcp = (fabs(AA) > T(1e-9)) ? T(1) / AA : T(0);
+  const T upd = p_old + S.omega * (si - Ap) * rcp;
+  const T pn = ((T(0) < upd) ? upd : T(0)) * ((T(1e-9) < fabs(AA)) ? T(1) : T(0));
+  S.Ap[i] = Ap;
+  S.p[i] = pn;
+  S.pr[i] = pn / (rhoi * rhoi);
+  S.xq[i] = make4<vec4_t<T>>(xi.x, xi.y, xi.z, xi.w * pn / (rhoi * rhoi));
+  // the pressure acceleration of iteration 1 (read if the solve stops there)
+  S.xa[S.xa_a(static_cast<size_t>(i))] = make4<vec4_t<T>>(T(0), T(0), T(0), T(0));
+  return static_cast<double>((Ap - si) / S.rho0 * ((T(1e-9) < pn) ? T(1) : T(0)));
+}
+
 // ---------------------------------------------------------------------------
 // setup pass: density + neighbour list + IISPH c, AA (+ s when the velocity
 // of every neighbour is known without another pass, i.e. no explicit
@@ -424,8 +447,9 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
       }
     }
   }
-  if (!valid)
-    return;
+  // the rest is per particle; with S.first_iter the block then sums the
+  // aprde terms of IISPH iteration 1 (every thread takes part)
+  auto finish = [&]() -> double {
   const size_t nn = static_cast<size_t>(S.nf);
   uint32_t info;
   if (over) {
@@ -498,8 +522,18 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
                   S.dt * rho0 * irho * (dsum + rho0 * (vs0 * gb0 + vs1 * gb1 + vs2 * gb2));
     S.s[i] = src;
     vout[i] = make4<vec4_t<T>>(vs0, vs1, vs2, vi.w);
+    if (S.first_iter)
+      return first_iteration(S, i, xi, rho, src, AA);
   } else {
     S.vd[i] = make4<vec4_t<T>>(gb0, gb1, gb2, T(0));  // G_b for the source pass
+  }
+  return 0.0;
+  };
+  const double term = valid ? finish() : 0.0;
+  if (kWithSource && S.first_iter) {
+    const double bs = block_sum<kBlock>(term);
+    if (t == 0)
+      S.partials[lb] = bs;
   }
 }
 
@@ -579,23 +613,32 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList L) {
   const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
-  if (i >= S.nf)
-    return;
-  const FastSpline<T> K = fast_spline(S);
-  const vec4_t<T> xi = S.xm[i], vi = S.v[i], gb = S.vd[i];
-  const T rhoi = S.rho[i];
-  T dsum = 0;
-  auto pair = [&](uint32_t j) {
-    const vec4_t<T> pj = S.xm[j];
-    const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
-    const T f = K.gradf(d0 * d0 + d1 * d1 + d2 * d2);
-    const vec4_t<T> vj = S.v[j];
-    dsum += pj.w * f * ((vi.x - vj.x) * d0 + (vi.y - vj.y) * d1 + (vi.z - vj.z) * d2);
-  };
-  walk(S, L, i, xi, pair);
-  const T rho0 = S.rho0;
-  S.s[i] = (rho0 - rhoi) -
-           S.dt * rho0 / rhoi * (dsum + rho0 * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z));
+  double term = 0.0;
+  if (i < S.nf) {
+    const FastSpline<T> K = fast_spline(S);
+    const vec4_t<T> xi = S.xm[i], vi = S.v[i], gb = S.vd[i];
+    const T rhoi = S.rho[i];
+    T dsum = 0;
+    auto pair = [&](uint32_t j) {
+      const vec4_t<T> pj = S.xm[j];
+      const T d0 = xi.x - pj.x, d1 = xi.y - pj.y, d2 = xi.z - pj.z;
+      const T f = K.gradf(d0 * d0 + d1 * d1 + d2 * d2);
+      const vec4_t<T> vj = S.v[j];
+      dsum += pj.w * f * ((vi.x - vj.x) * d0 + (vi.y - vj.y) * d1 + (vi.z - vj.z) * d2);
+    };
+    walk(S, L, i, xi, pair);
+    const T rho0 = S.rho0;
+    const T src = (rho0 - rhoi) -
+                  S.dt * rho0 / rhoi * (dsum + rho0 * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z));
+    S.s[i] = src;
+    if (S.first_iter)
+      term = first_iteration(S, i, xi, rhoi, src, S.AA[i]);
+  }
+  if (S.first_iter) {  // IISPH iteration 1: the block's aprde partial
+    const double bs = block_sum<kBlock>(term);
+    if (threadIdx.x == 0)
+      S.partials[lb] = bs;
+  }
 }
 
 // Walk of the list entries of particle i (all threads of the block call it).
@@ -952,6 +995,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_accel(SceneView<T> S) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < S.nf)
     S.a[i] = S.xa[S.xa_a(static_cast<size_t>(i))];
+}
+
+template <class T>
+void FusedOps<T>::finish_first_iteration(hipStream_t st, const SceneView<T> &S) {
+  if (S.nf > 0)
+    hipLaunchKernelGGL(k_pressure_finalize, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials,
+                       grid_blocks(S.nf));
+}
+
+template <class T>
+void FusedOps<T>::finish_first_iteration_batch(hipStream_t st, const BatchEntry<T> *tab,
+                                               unsigned nscenes) {
+  if (nscenes > 0)
+    hipLaunchKernelGGL(k_pressure_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
 }
 
 template <class T>

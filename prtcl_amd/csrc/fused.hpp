@@ -21,9 +21,12 @@ struct FusedSchedule {
   // the WKBB18 diagonal and the CG set-up of a fused step in two list walks
   // instead of one
   bool wk_split = false;
+  // IISPH iteration 1 in its own two neighbour passes instead of folded into
+  // the setup pass (the fold needs no neighbours: every pressure starts at 0)
+  bool first_split = false;
   int mask() const {
     return (setup_lds ? 1 : 0) | (accel_lds ? 2 : 0) | (pressure_lds ? 4 : 0) | (eager ? 8 : 0) |
-           (cg_split ? 16 : 0) | (wk_split ? 32 : 0);
+           (cg_split ? 16 : 0) | (wk_split ? 32 : 0) | (first_split ? 64 : 0);
   }
   static FusedSchedule from_mask(int m) {
     FusedSchedule s;
@@ -33,6 +36,7 @@ struct FusedSchedule {
     s.eager = (m & 8) != 0;
     s.cg_split = (m & 16) != 0;
     s.wk_split = (m & 32) != 0;
+    s.first_split = (m & 64) != 0;
     return s;
   }
 };
@@ -101,6 +105,11 @@ struct FusedOps {
   // the pressure acceleration goes to the second half of S.xa only
   static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
   static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
+  // the convergence step of IISPH iteration 1, whose pressures and aprde
+  // partials the setup (with S.first_iter) has written
+  static void finish_first_iteration(hipStream_t st, const SceneView<T> &S);
+  static void finish_first_iteration_batch(hipStream_t st, const BatchEntry<T> *tab,
+                                           unsigned nscenes);
   // v += dt a H, x += dt v, with a from S.xa, also stored to S.a
   static void integrate(hipStream_t st, const SceneView<T> &S);
   // S.a from S.xa (before a procedure kernel reads the acceleration)

@@ -114,6 +114,9 @@ struct SceneView {
   T g0, g1, g2;
   T gc0, gc1, gc2;  // gravity_center
   int axial;        // the step's gravity is initialize_axial_acceleration
+  // the fused setup (or source pass) also takes IISPH iteration 1, which
+  // needs no neighbours when every pressure starts at 0
+  int first_iter;
   GridDims grid;
   const uint32_t *fstart;
   const uint32_t *bstart;
@@ -219,7 +222,7 @@ struct BatchEntry {
 constexpr uint32_t kBatchLocalBits = 20;
 constexpr uint32_t kBatchLocalMask = (1u << kBatchLocalBits) - 1u;
 
-inline unsigned grid_blocks(long n, int bs = kBlock) {
+__host__ __device__ inline unsigned grid_blocks(long n, int bs = kBlock) {
   return static_cast<unsigned>((n + bs - 1) / bs);
 }
 

@@ -1144,9 +1144,11 @@ class SceneImpl final : public SceneBase {
   // IISPH iterations until the device reports convergence or the cap. With
   // `tune`, the fused passes are also timed for the schedule tuner.
   // nprde >= 0 also sets iisph_nprde (one control launch for both)
+  // first_done: the fused setup has taken iteration 1 (its pressures and
+  // aprde partials are written); its convergence step is the first launch
   int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused,
                  const FusedSchedule &sc = FusedSchedule{}, bool tune = false,
-                 int nprde = -1) {
+                 int nprde = -1, bool first_done = false) {
     require_grid();
     hipStream_t st = res_->stream;
     const unsigned seq = ++seq_;
@@ -1155,7 +1157,12 @@ class SceneImpl final : public SceneBase {
     const NbrList L = nbr_list();
     Profiler &pf = res_->prof;
     tprof_.on = tune;
+    // iterations enqueued so far (the iteration taken by the setup counts)
     int enq = 0;
+    if (first_done && nf_ > 0 && max_it > 0) {
+      FusedOps<T>::finish_first_iteration(st, S);
+      enq = 1;
+    }
     // one IISPH iteration (pressure acceleration, pressure, finalize) replayed
     // from a captured graph unless timed or eager
     const bool graph = fused && !sc.eager && !pf.on && !tune;
@@ -1197,7 +1204,7 @@ class SceneImpl final : public SceneBase {
     };
     Progress p;
     if (nf_ > 0 && max_it > 0) {
-      enqueue(std::min(std::max(min_it, std::min(predicted_iters_, 8)), max_it));
+      enqueue(std::max(0, std::min(std::max(min_it, std::min(predicted_iters_, 8)), max_it) - enq));
       for (;;) {
         const int ahead = enq < max_it ? 1 : 0;
         if (ahead)
@@ -1262,13 +1269,18 @@ class SceneImpl final : public SceneBase {
       const bool fused = desc.step_mode == 0;
       const bool tune = fused && tuner_.trial();
       const FusedSchedule sc = tuner_.current();
+      // the fused setup takes IISPH iteration 1 (no neighbours needed at p = 0)
+      const bool first_iter =
+          fused && !sc.first_split && nf_ > 0 && desc.iisph_max_iterations > 0;
       if (fused) {
         // one neighbour pass for density + list + IISPH setup (three with
         // explicit viscosity); the integrated velocity lands in tmpV4
         r = pf.begin(st, kStageSetup);
         tprof_.on = tune;
         const int q = tprof_.begin(st, kStageSetup);
-        FusedOps<T>::setup(st, view(), nbr_list(), desc.explicit_viscosity != 0, P_.tmpV4,
+        SceneView<T> Sv = view();
+        Sv.first_iter = first_iter ? 1 : 0;
+        FusedOps<T>::setup(st, Sv, nbr_list(), desc.explicit_viscosity != 0, P_.tmpV4,
                            sc.setup_lds);
         tprof_.end(st, q);
         pf.end(st, r);
@@ -1299,7 +1311,8 @@ class SceneImpl final : public SceneBase {
       int it = 0;
       double rel = 0;
       int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
-                          desc.iisph_max_iterations, &it, &rel, fused, sc, tune, nf_);
+                          desc.iisph_max_iterations, &it, &rel, fused, sc, tune, nf_,
+                          first_iter);
       if (fused)
         tuner_.finish_step();
       if (rc != PRTCL_HIP_OK)
@@ -1473,6 +1486,8 @@ class SceneImpl final : public SceneBase {
     // on the host while the three tables are filled
     std::vector<char> permute(ns, 0);
     std::vector<unsigned> seqs(ns);
+    // the setup takes IISPH iteration 1 (batchable scenes have max_it > 0)
+    const bool fold_first = !res->batch_tuner[sizeof(T) == 8 ? 1 : 0].current().first_split;
     for (unsigned k = 0; k < ns; ++k) {
       SceneImpl &s = *v[k];
       BatchEntry<T> E{};
@@ -1504,6 +1519,7 @@ class SceneImpl final : public SceneBase {
         s.steps_since_permute_ = 0;
       }
       E.S = s.view();
+      E.S.first_iter = fold_first ? 1 : 0;
       E.vtmp = s.P_.tmpV4;
       t_setup[k] = E;
       std::swap(s.P_.v, s.P_.tmpV4);
@@ -1549,6 +1565,9 @@ class SceneImpl final : public SceneBase {
     pf.end(st, r);
     hipLaunchKernelGGL(k_begin_solve_b<T>, dim3(grid_blocks(ns, 64)), dim3(64), 0, st, d_solve,
                        ns);
+    // the convergence step of iteration 1, which the setup has taken
+    if (fold_first)
+      FusedOps<T>::finish_first_iteration_batch(st, d_solve, ns);
 
     // IISPH iterations of all scenes; a scene whose solve is done exits its
     // blocks at its flag while the others go on
@@ -1558,7 +1577,7 @@ class SceneImpl final : public SceneBase {
                                      std::min(sc->predicted_iters_, 8)));
       max_it = std::max(max_it, static_cast<int>(sc->desc.iisph_max_iterations));
     }
-    int enq = 0;
+    int enq = fold_first ? 1 : 0;  // iterations enqueued (iteration 1 by the setup)
     const bool graph = !sc.eager && !pf.on && !tune;
     const std::string key =
         graph ? graph_key(kGraphBatchIteration | sizeof(T), d_solve, d_pmap, np, ns, sc)
@@ -1589,7 +1608,7 @@ class SceneImpl final : public SceneBase {
     };
     std::vector<Progress> prog(ns);
     std::vector<char> finished(ns, 0);
-    enqueue(std::min(pred, max_it));
+    enqueue(std::max(0, std::min(pred, max_it) - enq));
     for (;;) {
       const int ahead = enq < max_it ? 1 : 0;
       if (ahead)

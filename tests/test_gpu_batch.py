@@ -159,6 +159,55 @@ def test_fp32_and_fp64_batch_graphs_stay_apart(hip_ctx):
             assert np.array_equal(a[f], b[f]), (j, f)
 
 
+@pytest.mark.parametrize("real_type", [0, 1], ids=["fp32", "fp64"])
+def test_first_iteration_fold_bit_identical(hip_ctx, real_type):
+    """IISPH iteration 1 folded into the setup pass (every pressure is 0 at
+    the start of the solve, so Ap = 0 and the Jacobi update needs no
+    neighbours) equals iteration 1 run as its own two passes (schedule bit 6),
+    bit for bit in every field and in the iteration counts and density
+    errors: batched scenes, a scene with explicit viscosity and one with
+    axial gravity (the separate source pass takes the iteration there), and a
+    scene whose solve stops after iteration 1 (min 1 iteration, loose
+    tolerance)."""
+    from prtcl_amd import hip
+    mix = small_mix(3, seed=11)
+    visc = scenes.random_scene(11, 3, 2000, 6000, 4)
+    visc.explicit_viscosity = True
+    visc.fluid_viscosity = visc.boundary_viscosity = 0.05
+    axial = scenes.random_scene(11, 4, 2000, 6000, 4)
+    axial.axial_gravity = True
+    axial.gravity_center = tuple(float(c) for c in axial.fluid_x.mean(axis=0))
+    one = scenes.random_scene(11, 5, 2000, 6000, 4)
+    one.iisph_min_iterations = 1
+    one.iisph_tolerance = 0.5
+    scs = mix + [visc, axial, one]
+    out = {}
+    for mask in (8, 8 | 64):
+        print(f"fold check real_type={real_type} mask={mask}", flush=True)
+        hs = [hip.HipScene(hip_ctx, sc, real_type=real_type) for sc in scs]
+        for h in hs:
+            h.prepare()
+            h.set_schedule(mask)
+        hip_ctx.set_batch_schedule(mask)
+        info = []
+        try:
+            for _ in range(4):
+                hip_ctx.step(hs, 1)
+                info.append([h.step_info() for h in hs])
+        finally:
+            hip_ctx.set_batch_schedule(-1)
+        out[mask] = ([{f: h.download("fluid", f) for f in FIELDS + ("acceleration",)}
+                      for h in hs], info)
+        for h in hs:
+            h.destroy()
+    (a, ia), (b, ib) = out[8], out[8 | 64]
+    assert ia == ib
+    assert all(row[-1]["iisph_iterations"] == 1 for row in ia)  # the scene `one`
+    for j, (x, y) in enumerate(zip(a, b)):
+        for f in x:
+            assert np.array_equal(x[f], y[f]), (j, f)
+
+
 def test_graph_replay_equals_eager(hip_ctx):
     """The IISPH iteration replayed from captured hipGraphs (default) and
     launched kernel by kernel (schedule bit 3) give identical fields, alone
