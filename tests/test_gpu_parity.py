@@ -176,6 +176,46 @@ def test_dense_block_list_overflow(hip_ctx, oracle_lib):
     _close(g.download("fluid", "position"), o.get("fluid", "position"), 1e-5, "position")
 
 
+@pytest.mark.parametrize("iv", [1, 2, 3], ids=["wkbb18", "wkbb18_gc", "pt16"])
+def test_dense_block_list_overflow_viscous(hip_ctx, oracle_lib, iv):
+    """The dense block of test_dense_block_list_overflow with implicit
+    viscosity in fused steps: for the particles whose list overflowed, the
+    viscous list kernels take their grid-walk fallbacks (the WKBB18 diagonal
+    and CG set-up walk, the pt16 pair sums and products and their skipped
+    coefficient slots). One step against the oracle (the compressed block
+    explodes: a second step reaches 10^3 m/s, and pt16's CG diverges in the
+    oracle too), and the schedules with the diagonal and the CG set-up in one
+    list walk and in two (bit 5) bit-identical."""
+    from prtcl_amd import hip
+    s = 0.025
+    bx = scenes.box_boundary(17, 17, 17, s)
+    fx = scenes.fluid_block((10, 10, 10), (8, 8, 8), 0.6 * s, 1e-4, scenes.stream_base(9, 0))
+    rng = np.random.default_rng(5)
+    v = rng.uniform(-0.2, 0.2, fx.shape).astype(np.float32)
+    sc = scenes._finish("dense_viscous", s, fx, bx, 1, v=v, implicit_viscosity=iv,
+                        fluid_viscosity=10.0, boundary_viscosity=10.0,
+                        wkbb18_maximum_error=1e-4, pt16_vorticity_maximum_error=1e-4,
+                        pt16_velocity_maximum_error=1e-4)
+    sc.permute_every = 0
+    sc.iisph_max_iterations = 20
+    print(f"case: dense block, implicit_viscosity {iv}, fused", flush=True)
+    o = oracle_lib.OracleScene(sc, fp64_reductions=True, permute_every=0)
+    o.prepare()
+    out = []
+    for mask in (8, 8 | 32):
+        g = hip.HipScene(hip_ctx, sc)
+        g.set_schedule(mask)
+        g.prepare()
+        g.step(1)
+        out.append({f: g.download("fluid", f) for f in ("position", "velocity", "density")})
+        g.destroy()
+    o.step()
+    for f, tol in (("position", 1e-5), ("velocity", 5e-3), ("density", 1e-3)):
+        _close(out[0][f], o.get("fluid", f), tol, f)
+    for f in out[0]:
+        np.testing.assert_array_equal(out[0][f], out[1][f], err_msg=f)
+
+
 def test_fp32_against_fp64_kernels(hip_ctx):
     """Error statement of the FP32 path against the FP64 instantiation of the
     same fused kernels after 10 steps of config 0 (printed; bounded here)."""
