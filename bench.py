@@ -35,9 +35,10 @@ STAGE_BYTES = {
     "density": (20, 16),                      # x, m -> rho
     # the fused setup pass does the work of density (20), a = g (12),
     # integrate v (40), setup c (32) and setup s/AA (56), three boundary
-    # passes, and IISPH iteration 1 without neighbours (p, p/rho^2, Ap, the
-    # (x, m p/rho^2) record and a = 0 written: 44)
-    "iisph_setup": (204, 48),
+    # passes, and IISPH iteration 1 without neighbours (p, p/rho^2 and the
+    # (x, m p/rho^2) record take the place of the zeros the setup writes; Ap
+    # and a = 0 are new: 20)
+    "iisph_setup": (180, 48),
     # the fused integration: integrate v (40) + integrate x (36)
     "integrate_position": (76, 0),
     "nsearch_sort": (96, 0),
@@ -168,14 +169,15 @@ def main():
 
 
 def step_bytes(nf, nb_active, k, kv=0.0, viscous=False):
-    """Algorithmic HBM bytes of one whole step: 376 + 88 (k - 1) per fluid
+    """Algorithmic HBM bytes of one whole step: 352 + 88 (k - 1) per fluid
     particle and 16 (4 + 2 (k - 1)) per active boundary particle for IISPH
     (iteration 1 needs no neighbour pass: every pressure starts at 0, so it
-    costs the 44 bytes it writes); a WKBB18 step adds 376 + 200 k_v per fluid
-    particle and 16 (2 + k_v) per active boundary particle (diagonal, the
-    set-up product and one product per PCG iteration)."""
+    costs the 20 bytes of Ap and a it adds to the setup's writes); a WKBB18
+    step adds 376 + 200 k_v per fluid particle and 16 (2 + k_v) per active
+    boundary particle (diagonal, the set-up product and one product per PCG
+    iteration)."""
     kp = max(k - 1.0, 0.0)
-    b = (376 + 88 * kp) * nf + 16 * (4 + 2 * kp) * nb_active
+    b = (352 + 88 * kp) * nf + 16 * (4 + 2 * kp) * nb_active
     if viscous:
         b += (376 + 200 * kv) * nf + 16 * (2 + kv) * nb_active
     return b
