@@ -830,9 +830,12 @@ __device__ __forceinline__ void max_speed_atomic<double>(Ctl *ctl, double s) {
   atomicMax(&ctl->max_speed_bits_d, static_cast<unsigned long long>(__double_as_longlong(s)));
 }
 
-// v += dt a H (pressure acceleration), x += dt v, running max |v|
+// v += dt a H (pressure acceleration), x += dt v, running max |v|; seq > 0:
+// only once solve seq is done (integrate_now)
 template <class T>
-__device__ __forceinline__ void k_integrate_body(SceneView<T> S, unsigned lb) {
+__device__ __forceinline__ void k_integrate_body(SceneView<T> S, unsigned lb, unsigned seq) {
+  if (seq != 0u && !integrate_now(S.ctl, seq))
+    return;
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   T speed = T(0);
   bool bad = false;
@@ -863,28 +866,33 @@ __device__ __forceinline__ void k_integrate_body(SceneView<T> S, unsigned lb) {
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_integrate(SceneView<T> S) {
-  k_integrate_body<T>(S, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_integrate(SceneView<T> S, unsigned seq) {
+  k_integrate_body<T>(S, blockIdx.x, seq);
 }
 
-// running maximum speed (symplectic_euler.hpp:250-268) from the block maxima
+// running maximum speed (symplectic_euler.hpp:250-268) from the block maxima;
+// seq > 0: after the integration of solve seq's step, which it marks taken
 template <class T>
 __device__ __forceinline__ void k_speed_finalize_body(Ctl *c, const double *partials,
-                                                      unsigned nparts) {
+                                                      unsigned nparts, unsigned seq) {
+  if (seq != 0u && !integrate_now(c, seq))
+    return;
   double m = 0.0;
   for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
     m = partials[k] > m ? partials[k] : m;
   m = block_max<kBlock>(m);
   if (threadIdx.x == 0) {
     max_speed_atomic<T>(c, static_cast<T>(m));
+    if (seq != 0u)
+      c->integrated = seq;
     publish_step_end(c);
   }
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double *partials,
-                                                           unsigned nparts) {
-  k_speed_finalize_body<T>(c, partials, nparts);
+                                                           unsigned nparts, unsigned seq) {
+  k_speed_finalize_body<T>(c, partials, nparts, seq);
 }
 
 // ---- batched wrappers (prtcl_hip_step over several scenes)
@@ -920,18 +928,20 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize_b(const BatchEntry
   k_pressure_finalize_body(E.S.ctl, E.S.partials, E.nblocks);
 }
 
+// guarded: each scene's integration only once its solve (E.seq) is done
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_integrate_b(const BatchEntry<T> *tab,
-                                                        const uint32_t *pmap) {
+                                                        const uint32_t *pmap, int guarded) {
   const uint32_t u = pmap[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  k_integrate_body<T>(E.S, u & kBatchLocalMask);
+  k_integrate_body<T>(E.S, u & kBatchLocalMask, guarded ? E.seq : 0u);
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_speed_finalize_b(const BatchEntry<T> *tab) {
+__global__ __launch_bounds__(kBlock) void k_speed_finalize_b(const BatchEntry<T> *tab,
+                                                             int guarded) {
   const BatchEntry<T> &E = tab[blockIdx.x];
-  k_speed_finalize_body<T>(E.S.ctl, E.S.partials, E.nblocks);
+  k_speed_finalize_body<T>(E.S.ctl, E.S.partials, E.nblocks, guarded ? E.seq : 0u);
 }
 
 }  // namespace
@@ -1018,12 +1028,13 @@ void FusedOps<T>::copy_accel(hipStream_t st, const SceneView<T> &S) {
 }
 
 template <class T>
-void FusedOps<T>::integrate(hipStream_t st, const SceneView<T> &S) {
+void FusedOps<T>::integrate(hipStream_t st, const SceneView<T> &S, unsigned seq) {
   if (S.nf <= 0)
     return;
   const unsigned nb = grid_blocks(S.nf);
-  hipLaunchKernelGGL(k_integrate<T>, dim3(nb), dim3(kBlock), 0, st, S);
-  hipLaunchKernelGGL(k_speed_finalize<T>, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials, nb);
+  hipLaunchKernelGGL(k_integrate<T>, dim3(nb), dim3(kBlock), 0, st, S, seq);
+  hipLaunchKernelGGL(k_speed_finalize<T>, dim3(1), dim3(kBlock), 0, st, S.ctl, S.partials, nb,
+                     seq);
 }
 
 template <class T>
@@ -1062,11 +1073,12 @@ void FusedOps<T>::pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const
 
 template <class T>
 void FusedOps<T>::integrate_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                                  unsigned npblocks, unsigned nscenes) {
+                                  unsigned npblocks, unsigned nscenes, bool guarded) {
   if (npblocks == 0)
     return;
-  hipLaunchKernelGGL(k_integrate_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
-  hipLaunchKernelGGL(k_speed_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
+  const int g = guarded ? 1 : 0;
+  hipLaunchKernelGGL(k_integrate_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap, g);
+  hipLaunchKernelGGL(k_speed_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab, g);
 }
 
 template struct FusedOps<float>;

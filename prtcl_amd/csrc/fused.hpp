@@ -110,8 +110,10 @@ struct FusedOps {
   static void finish_first_iteration(hipStream_t st, const SceneView<T> &S);
   static void finish_first_iteration_batch(hipStream_t st, const BatchEntry<T> *tab,
                                            unsigned nscenes);
-  // v += dt a H, x += dt v, with a from S.xa, also stored to S.a
-  static void integrate(hipStream_t st, const SceneView<T> &S);
+  // v += dt a H, x += dt v, with a from S.xa, also stored to S.a; seq > 0:
+  // enqueued ahead of the host's knowledge that IISPH solve seq is done, it
+  // integrates only once that solve is done, and only once (integrate_now)
+  static void integrate(hipStream_t st, const SceneView<T> &S, unsigned seq = 0);
   // S.a from S.xa (before a procedure kernel reads the acceleration)
   static void copy_accel(hipStream_t st, const SceneView<T> &S);
   // the same passes over every scene of a batch (pmap: block map)
@@ -121,8 +123,9 @@ struct FusedOps {
                           unsigned npblocks, bool lds);
   static void pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
                              unsigned npblocks, unsigned nscenes, bool lds);
+  // guarded: each scene as integrate() with seq = its solve's sequence
   static void integrate_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
-                              unsigned npblocks, unsigned nscenes);
+                              unsigned npblocks, unsigned nscenes, bool guarded = false);
 };
 
 }  // namespace prtcl_amd

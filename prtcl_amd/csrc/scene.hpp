@@ -63,7 +63,18 @@ struct Ctl {
   // host_mirror[3] as (step_count << 32) | nonfinite_steps
   unsigned step_count;
   unsigned nonfinite_steps;
+  // the IISPH solve (seq) whose step the integration kernels have taken
+  unsigned integrated;
 };
+
+// An integration enqueued before the host knows that IISPH solve `seq` is
+// done (right after the predicted iterations) does its work only once that
+// solve is done, and only once: a launch that finds the solve still running
+// or its step already integrated exits at its first instruction. Every block
+// of the launch reads the same words, written by earlier kernels.
+__device__ __forceinline__ bool integrate_now(const Ctl *c, unsigned seq) {
+  return c->seq == seq && c->done != 0 && c->integrated != seq;
+}
 
 // host_mirror words: [0] IISPH progress, [1] its aprde / nprde, [2] CG
 // progress, [3] end-of-step report

@@ -1145,10 +1145,14 @@ class SceneImpl final : public SceneBase {
   // `tune`, the fused passes are also timed for the schedule tuner.
   // nprde >= 0 also sets iisph_nprde (one control launch for both)
   // first_done: the fused setup has taken iteration 1 (its pressures and
-  // aprde partials are written); its convergence step is the first launch
+  // aprde partials are written); its convergence step is the first launch.
+  // integrate: the step's fused integration is enqueued right behind the
+  // predicted iterations (and again behind any further ones) in its guarded
+  // form, so it runs as soon as the device decides convergence instead of
+  // after the host has seen the decision; on return it is enqueued.
   int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused,
                  const FusedSchedule &sc = FusedSchedule{}, bool tune = false,
-                 int nprde = -1, bool first_done = false) {
+                 int nprde = -1, bool first_done = false, bool integrate = false) {
     require_grid();
     hipStream_t st = res_->stream;
     const unsigned seq = ++seq_;
@@ -1203,7 +1207,21 @@ class SceneImpl final : public SceneBase {
       enq += n;
     };
     Progress p;
-    if (nf_ > 0 && max_it > 0) {
+    // the guarded integration behind what is enqueued (the caller passes
+    // integrate only for fused inviscid steps without per-stage timing)
+    if (nf_ > 0 && max_it > 0 && integrate) {
+      enqueue(std::max(0, std::min(std::max(min_it, std::min(predicted_iters_, 8)), max_it) - enq));
+      FusedOps<T>::integrate(st, S, seq);
+      for (;;) {
+        p = wait_progress(seq, enq);
+        if (p.done)
+          break;
+        // more iterations than predicted: a growing chunk, each followed by
+        // the guarded integration again
+        enqueue(std::min(std::max(1, enq / 2), max_it - enq));
+        FusedOps<T>::integrate(st, S, seq);
+      }
+    } else if (nf_ > 0 && max_it > 0) {
       enqueue(std::max(0, std::min(std::max(min_it, std::min(predicted_iters_, 8)), max_it) - enq));
       for (;;) {
         const int ahead = enq < max_it ? 1 : 0;
@@ -1235,7 +1253,9 @@ class SceneImpl final : public SceneBase {
       tprof_.reset();
       tprof_.on = false;
     }
-    predicted_iters_ = std::max(1, p.iters);
+    // an iteration predicted too many costs a no-op launch, one too few a
+    // host round trip: predict the larger of the last two counts
+    predicted_iters_ = std::max(1, std::max(p.iters, last_iisph_iters_));
     last_iisph_iters_ = p.iters;
     last_rel_ = p.rel;
     if (iters)
@@ -1310,19 +1330,25 @@ class SceneImpl final : public SceneBase {
       }
       int it = 0;
       double rel = 0;
+      // the integration rides behind the solve's iterations unless stages
+      // are timed
+      const bool spec_integrate = fused && !desc.implicit_viscosity && !pf.on && !tune &&
+                                  nf_ > 0 && desc.iisph_max_iterations > 0;
       int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
                           desc.iisph_max_iterations, &it, &rel, fused, sc, tune, nf_,
-                          first_iter);
+                          first_iter, spec_integrate);
       if (fused)
         tuner_.finish_step();
       if (rc != PRTCL_HIP_OK)
         status = rc;
       const SceneView<T> S = view();
       if (fused && !desc.implicit_viscosity) {
-        r = pf.begin(st, kStageIntegrateX);
-        FusedOps<T>::integrate(st, S);
+        if (!spec_integrate) {
+          r = pf.begin(st, kStageIntegrateX);
+          FusedOps<T>::integrate(st, S);
+          pf.end(st, r);
+        }
         steps_enqueued_ += nf_ > 0 ? 1u : 0u;
-        pf.end(st, r);
       } else {
         if (fused)
           FusedOps<T>::copy_accel(st, S);
@@ -1609,7 +1635,29 @@ class SceneImpl final : public SceneBase {
     std::vector<Progress> prog(ns);
     std::vector<char> finished(ns, 0);
     enqueue(std::max(0, std::min(pred, max_it) - enq));
-    for (;;) {
+    // the integration rides behind the iterations in its guarded form (each
+    // scene integrates once its own solve is done) unless stages are timed
+    const bool spec = !tune && !pf.on;
+    if (spec) {
+      FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns, true);
+      for (;;) {
+        bool all = true;
+        for (unsigned k = 0; k < ns; ++k) {
+          if (finished[k])
+            continue;
+          prog[k] = v[k]->wait_progress(seqs[k], enq);
+          if (prog[k].done)
+            finished[k] = 1;
+          else
+            all = false;
+        }
+        if (all)
+          break;
+        enqueue(std::min(std::max(1, enq / 2), max_it - enq));
+        FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns, true);
+      }
+    }
+    for (; !spec;) {
       const int ahead = enq < max_it ? 1 : 0;
       if (ahead)
         enqueue(1);
@@ -1642,7 +1690,8 @@ class SceneImpl final : public SceneBase {
     if (!fixed)
       tn.finish_step();
     r = pf.begin(st, kStageIntegrateX);
-    FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
+    if (!spec)
+      FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
     if (fixed)  // the last kernel reading the tables is enqueued
       PRTCL_HIP_CHECK(hipEventRecord(fixed->released, st));
     for (SceneImpl *sc : v)
@@ -1659,7 +1708,7 @@ class SceneImpl final : public SceneBase {
     for (unsigned k = 0; k < ns; ++k) {
       SceneImpl &s = *v[k];
       const Progress &p = prog[k];
-      s.predicted_iters_ = std::max(1, p.iters);
+      s.predicted_iters_ = std::max(1, std::max(p.iters, s.last_iisph_iters_));
       s.last_iisph_iters_ = p.iters;
       s.last_rel_ = p.rel;
       const bool converged = p.iters >= s.desc.iisph_min_iterations &&
