@@ -90,11 +90,15 @@ def parse():
     ap.add_argument("--serve-warm-steps", type=int, default=3)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="force the schedule mask of the main scene (-1: autotune)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the batched workload (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-viscous-scenes", type=int, default=32,
+                    help="scenes of the batched mix with WKBB18 scenes (0 skips it)")
     ap.add_argument("--no-viscous", action="store_true",
                     help="skip the viscous-column sub-run of the default workload")
     ap.add_argument("--no-served", action="store_true",
@@ -138,8 +142,8 @@ def main():
         sc.name = "viscous_gc_column_250k"
     sc.permute_every = 8
     ctx = hip.Context(0)
-    out, g = bench_scene(ctx, sc, args.steps, args.warmup, schedules=True,
-                         traffic=args.workload == "dambreak_1m")
+    out, g = bench_scene(ctx, sc, args.steps, args.warmup, schedules=args.schedule < 0,
+                         traffic=args.workload == "dambreak_1m", schedule=args.schedule)
     out = {
         "metric": "IISPH particle-steps/s at 1M particles; batched-scene p50/p99 step latency",
         "value": out.pop("value"), "unit": "particle-steps/s", "n_gpus": 1,
@@ -155,6 +159,12 @@ def main():
     if args.workload == "dambreak_1m":
         if args.batch_scenes > 0:
             out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
+        if args.batch_viscous_scenes > 0:
+            # the served stream's scene mix: 30 % with WKBB18, stepped batched
+            vs = scenes.config4_scenes(args.batch_viscous_scenes)
+            out["batched_viscous"] = bench_batch(
+                ctx, len(vs), min(args.batch_steps, 10), min(args.warmup, 3), scs=vs,
+                name=f"served_mix{len(vs)}")
         if not args.no_viscous:
             # config 2: the 250k viscous column (WKBB18 PCG every step)
             vsc = scenes.config2()
@@ -183,13 +193,17 @@ def step_bytes(nf, nb_active, k, kv=0.0, viscous=False):
     return b
 
 
-def bench_scene(ctx, sc, steps, warmup, schedules=False, traffic=False):
+def bench_scene(ctx, sc, steps, warmup, schedules=False, traffic=False, schedule=-1):
     """Whole steps of one device-resident scene: the timed region (K steps,
     no instrumentation), a pass with per-stage HIP events, the roofline of
-    the dominant stage and of the whole step. Returns (dict, scene)."""
+    the dominant stage and of the whole step. schedule >= 0 forces the
+    scene's schedule mask (bit-identical variants) instead of the tuner's
+    choice. Returns (dict, scene)."""
     from prtcl_amd import hip
     g = hip.HipScene(ctx, sc)
     g.prepare()
+    if schedule >= 0:
+        g.set_schedule(schedule)
     for _ in range(warmup):
         g.step(1)
     ctx.synchronize()
@@ -513,17 +527,19 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
           flush=True)
 
 
-def bench_batch(ctx, count, steps, warmup):
+def bench_batch(ctx, count, steps, warmup, scs=None, name=None):
     """The batched workload: `count` independent dam-break scenes (N_f
     log-uniform in [2k, 200k], uneven aspect ratios, random initial
-    velocities, each with its own IISPH convergence at 0.1%). Two identical
+    velocities, each with its own IISPH convergence at 0.1%), or the given
+    scenes (`scs`, e.g. the served mix with WKBB18 scenes). Two identical
     copies are stepped from the same initial state: one with one
     prtcl_hip_step call per scene (the plain per-scene path), one with a
     single call over all scenes (one launch per stage for the whole batch).
     Both do the same work (their results are bit-identical)."""
     from prtcl_amd import hip, scenes
-    scs = scenes.config3(count=count)
+    scs = scs if scs is not None else scenes.config3(count=count)
     nf_total = sum(s.n_fluid for s in scs)
+    nvisc = sum(1 for s in scs if s.implicit_viscosity)
 
     def make():
         hs = [hip.HipScene(ctx, s) for s in scs]
@@ -539,7 +555,7 @@ def bench_batch(ctx, count, steps, warmup):
                 for h in hs:
                     h.step(1)
         ctx.synchronize()
-        lat, its = [], []
+        lat, its, wks = [], [], []
         t0 = prev = time.perf_counter()
         for _ in range(steps):
             if together:
@@ -550,23 +566,25 @@ def bench_batch(ctx, count, steps, warmup):
             now = time.perf_counter()
             lat.append(now - prev)
             prev = now
-            its.append(max(h.step_info()["iisph_iterations"] for h in hs))
+            info = [h.step_info() for h in hs]
+            its.append(max(d["iisph_iterations"] for d in info))
+            wks.append(max(d["wkbb18_iterations"] for d in info))
         ctx.synchronize()
         el = time.perf_counter() - t0
-        return el, lat, its
+        return el, lat, its, wks
 
     per = make()
-    el_p, lat_p, _ = timed(per, False)
+    el_p, lat_p, _, _ = timed(per, False)
     for h in per:
         h.destroy()
     bat = make()
-    el_b, lat_b, its_b = timed(bat, True)
+    el_b, lat_b, its_b, wks_b = timed(bat, True)
     for h in bat:
         h.destroy()
     v_b = nf_total * steps / el_b
     v_p = nf_total * steps / el_p
-    return {
-        "workload": f"batch{count}", "scenes": count, "fluid_particles_total": nf_total,
+    out = {
+        "workload": name or f"batch{count}", "scenes": len(scs), "fluid_particles_total": nf_total,
         "steps": steps, "warmup": warmup,
         "value": round(v_b, 1), "unit": "particle-steps/s",
         "per_scene_value": round(v_p, 1), "speedup_vs_per_scene": round(v_b / v_p, 3),
@@ -576,6 +594,10 @@ def bench_batch(ctx, count, steps, warmup):
         "per_scene_p99_step_ms": round(1e3 * float(np.percentile(lat_p, 99)), 4),
         "max_iisph_iterations_mean": round(float(np.mean(its_b)), 2),
     }
+    if nvisc:
+        out["viscous_scenes"] = nvisc
+        out["max_wkbb18_iterations_mean"] = round(float(np.mean(wks_b)), 2)
+    return out
 
 
 def cpu_baseline(sc, g, seconds):

@@ -62,9 +62,8 @@ __device__ __forceinline__ void publish_cg(Ctl *c) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *part,
-                                                         unsigned nb, double tol, int max_k,
-                                                         unsigned seq) {
+__device__ __forceinline__ void cg_setup_fin_body(Ctl *c, const double *part, unsigned nb,
+                                                  double tol, int max_k, unsigned seq) {
   const double bb = sum_partials(part, nb);
   const double rr = sum_partials(part + nb, nb);
   const double ry = sum_partials(part + 2 * nb, nb);
@@ -84,8 +83,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *p
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_cg_setup_fin(Ctl *c, const double *part,
+                                                         unsigned nb, double tol, int max_k,
+                                                         unsigned seq) {
+  cg_setup_fin_body(c, part, nb, tol, max_k, seq);
+}
+
 // |p.q| < tol breaks (cg_openmp.hpp:147); alpha = (r.y) / (p.q)
-__global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl *c, const double *part, unsigned nb) {
+__device__ __forceinline__ void cg_alpha_body(Ctl *c, const double *part, unsigned nb) {
   if (c->cg_done)
     return;
   const double pq = sum_partials(part, nb);
@@ -101,8 +106,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl *c, const double *part,
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl *c, const double *part, unsigned nb) {
+  cg_alpha_body(c, part, nb);
+}
+
 // |r.y|_prev < tol breaks (:187); beta; ++k; ok(k, |r|^2) (:100-111)
-__global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, unsigned nb) {
+__device__ __forceinline__ void cg_beta_body(Ctl *c, const double *part, unsigned nb) {
   if (c->cg_done)
     return;
   const double rr = sum_partials(part, nb);
@@ -123,6 +132,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, 
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, unsigned nb) {
+  cg_beta_body(c, part, nb);
+}
+
 // The same decisions inside the vector kernels of an iteration, for
 // launches of at most kFusedCgBlocks blocks: every block sums all partials of
 // the kernel before it (sum_partials_all: the order of the one-block
@@ -131,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl *c, const double *part, 
 // needs is staged in cg_iter_next beside alpha, so no value a block reads is
 // written in the same kernel except cg_done, which a block reads once
 // (cg_done_for_block): a block that sees it set would have decided the same.
-constexpr unsigned kFusedCgBlocks = 1024;
+// (kFusedCgBlocks: scene.hpp)
 
 // cg_done as one value for the whole block. Block 0 of a folded kernel may
 // set it while the wavefronts of another block start, so each wavefront
@@ -149,12 +162,13 @@ __device__ __forceinline__ bool cg_done_for_block(const Ctl *c) {
 
 // k_cg_alpha in every block of the update kernel: false when the solve
 // breaks (|p.q| < tol), else alpha
+// (lb: the block's logical index; block 0 writes the control block)
 __device__ __forceinline__ bool cg_alpha_in_block(Ctl *c, const double *pq_part, unsigned nb,
-                                                  double *alpha) {
+                                                  double *alpha, unsigned lb = blockIdx.x) {
   const double pq = sum_partials_all(pq_part, nb);
   const double ry_prev = c->cg_ry;
   const bool brk = fabs(pq) < c->cg_tol;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (lb == 0 && threadIdx.x == 0) {
     c->cg_ry_prev = ry_prev;
     c->cg_pq = pq;
     c->cg_iter_next = c->cg_iter + 1;
@@ -172,7 +186,7 @@ __device__ __forceinline__ bool cg_alpha_in_block(Ctl *c, const double *pq_part,
 // k_cg_beta in every block of the direction kernel (r.r partials at part,
 // r.y at part + nb): false when the solve is done, else beta
 __device__ __forceinline__ bool cg_beta_in_block(Ctl *c, const double *part, unsigned nb,
-                                                 double *beta) {
+                                                 double *beta, unsigned lb = blockIdx.x) {
   const double rr = sum_partials_all(part, nb);
   const double ry = sum_partials_all(part + nb, nb);
   const double ry_prev = c->cg_ry_prev;
@@ -180,7 +194,7 @@ __device__ __forceinline__ bool cg_beta_in_block(Ctl *c, const double *part, uns
   const double b = brk ? 0.0 : ry / ry_prev;
   const int k = c->cg_iter_next;
   const bool done = brk || k > c->cg_max_k || rr < c->cg_thr;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (lb == 0 && threadIdx.x == 0) {
     c->cg_r_nsq = rr;
     c->cg_ry = ry;
     if (!brk) {

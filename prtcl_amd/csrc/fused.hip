@@ -928,12 +928,15 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize_b(const BatchEntry
   k_pressure_finalize_body(E.S.ctl, E.S.partials, E.nblocks);
 }
 
-// guarded: each scene's integration only once its solve (E.seq) is done
+// guarded: each scene's integration only once its solve (E.seq) is done;
+// scenes whose step ends with the WKBB18 path are left to it
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_integrate_b(const BatchEntry<T> *tab,
                                                         const uint32_t *pmap, int guarded) {
   const uint32_t u = pmap[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  if (E.viscous)
+    return;
   k_integrate_body<T>(E.S, u & kBatchLocalMask, guarded ? E.seq : 0u);
 }
 
@@ -941,6 +944,8 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_speed_finalize_b(const BatchEntry<T> *tab,
                                                              int guarded) {
   const BatchEntry<T> &E = tab[blockIdx.x];
+  if (E.viscous)
+    return;
   k_speed_finalize_body<T>(E.S.ctl, E.S.partials, E.nblocks, guarded ? E.seq : 0u);
 }
 

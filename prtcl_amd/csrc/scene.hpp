@@ -229,7 +229,21 @@ struct BatchEntry {
   double tol;
   int min_it, max_it, nprde;
   unsigned seq;
+  // the step ends with the WKBB18 path (implicit viscosity), not with the
+  // fused integration
+  int viscous;
+  // the scene's WKBB18 solve in this step: FP64 CG partials, tolerance, cap
+  // and sequence number; cg_fold: alpha and beta derived in the vector
+  // kernels (at most kFusedCgBlocks blocks)
+  double *cg_part;
+  double cg_tol;
+  int cg_max_k;
+  unsigned cg_seq;
+  int cg_fold;
 };
+// the largest CG launch whose vector kernels derive alpha and beta
+// themselves: every block sums all nb partials of the kernel before it
+constexpr unsigned kFusedCgBlocks = 1024;
 constexpr uint32_t kBatchLocalBits = 20;
 constexpr uint32_t kBatchLocalMask = (1u << kBatchLocalBits) - 1u;
 

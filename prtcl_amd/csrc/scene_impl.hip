@@ -1436,11 +1436,12 @@ class SceneImpl final : public SceneBase {
   }
 
   // ------------------------------------------------------- batched stepping
-  // a scene the batched step can take: fused passes, no viscosity stage
+  // a scene the batched step can take: fused passes, no explicit viscosity,
+  // implicit viscosity by WKBB18 or none
   bool batchable() const {
-    return desc.step_mode == 0 && !desc.explicit_viscosity && !desc.implicit_viscosity &&
-           !desc.axial_gravity &&
-           nf_ > 0 && desc.iisph_max_iterations > 0 &&
+    return desc.step_mode == 0 && !desc.explicit_viscosity &&
+           (desc.implicit_viscosity == 0 || desc.implicit_viscosity == 1) &&
+           !desc.axial_gravity && nf_ > 0 && desc.iisph_max_iterations > 0 &&
            grid_blocks(nf_) <= kBatchLocalMask;
   }
 
@@ -1482,14 +1483,31 @@ class SceneImpl final : public SceneBase {
       for (unsigned b = 0; b < nt; ++b)
         tlist.push_back((k << kBatchLocalBits) | b);
     }
+    // the scenes whose step ends with WKBB18 (their own table and block map)
+    std::vector<unsigned> vsc;
+    std::vector<uint32_t> vlist;
+    for (unsigned k = 0; k < ns; ++k) {
+      if (!v[k]->desc.implicit_viscosity)
+        continue;
+      const unsigned kv = static_cast<unsigned>(vsc.size());
+      vsc.push_back(k);
+      const unsigned nb = grid_blocks(v[k]->nf_);
+      for (unsigned b = 0; b < nb; ++b)
+        vlist.push_back((kv << kBatchLocalBits) | b);
+    }
     const unsigned np = static_cast<unsigned>(plist.size());
     const unsigned nt = static_cast<unsigned>(tlist.size());
-    // pinned / device table layout: [sort | setup | solve] entries, pmap, tmap
+    const unsigned nvs = static_cast<unsigned>(vsc.size());
+    const unsigned nvb = static_cast<unsigned>(vlist.size());
+    // pinned / device table layout: [sort | setup | solve] entries, the
+    // WKBB18 entries, pmap, tmap, vmap
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t tab_bytes = align(sizeof(BatchEntry<T>) * ns);
-    const size_t off_pmap = 3 * tab_bytes;
+    const size_t off_visc = 3 * tab_bytes;
+    const size_t off_pmap = off_visc + align(sizeof(BatchEntry<T>) * nvs);
     const size_t off_tmap = off_pmap + align(sizeof(uint32_t) * np);
-    const size_t total = off_tmap + align(sizeof(uint32_t) * nt);
+    const size_t off_vmap = off_tmap + align(sizeof(uint32_t) * nt);
+    const size_t total = off_vmap + align(sizeof(uint32_t) * nvb);
     if (fixed && total > fixed->bytes)
       throw TablesTooLarge{total};
     if (!fixed)
@@ -1502,12 +1520,16 @@ class SceneImpl final : public SceneBase {
     auto *t_sort = reinterpret_cast<BatchEntry<T> *>(hb);
     auto *t_setup = reinterpret_cast<BatchEntry<T> *>(hb + tab_bytes);
     auto *t_solve = reinterpret_cast<BatchEntry<T> *>(hb + 2 * tab_bytes);
+    auto *t_visc = reinterpret_cast<BatchEntry<T> *>(hb + off_visc);
     uint32_t *pmap = reinterpret_cast<uint32_t *>(hb + off_pmap);
     uint32_t *tmap = reinterpret_cast<uint32_t *>(hb + off_tmap);
+    uint32_t *vmap = reinterpret_cast<uint32_t *>(hb + off_vmap);
     for (unsigned b = 0; b < np; ++b)
       pmap[b] = plist[xcd_of(b, np)];
     for (unsigned b = 0; b < nt; ++b)
       tmap[b] = tlist[b];
+    for (unsigned b = 0; b < nvb; ++b)
+      vmap[b] = vlist[xcd_of(b, nvb)];
     // the pointer swaps of build_fluid_grid() and of the setup pass are done
     // on the host while the three tables are filled
     std::vector<char> permute(ns, 0);
@@ -1557,7 +1579,25 @@ class SceneImpl final : public SceneBase {
       E.max_it = s.desc.iisph_max_iterations;
       E.nprde = s.nf_;
       E.seq = seqs[k];
+      E.viscous = s.desc.implicit_viscosity ? 1 : 0;
       t_solve[k] = E;
+    }
+    // WKBB18 after the IISPH solve (the order of run_wkbb18: the CG's
+    // sequence number follows the solve's)
+    const bool cg_split = res->batch_tuner[sizeof(T) == 8 ? 1 : 0].current().cg_split;
+    std::vector<unsigned> cg_seqs(nvs);
+    bool any_split = false;
+    for (unsigned kv = 0; kv < nvs; ++kv) {
+      SceneImpl &s = *v[vsc[kv]];
+      BatchEntry<T> E = t_solve[vsc[kv]];
+      E.cg_part = s.P_.cg_part;
+      E.cg_tol = s.desc.wkbb18_maximum_error;
+      E.cg_max_k = std::max(0, static_cast<int>(s.desc.wkbb18_maximum_iterations));
+      cg_seqs[kv] = ++s.seq_;
+      E.cg_seq = cg_seqs[kv];
+      E.cg_fold = !cg_split && E.nblocks <= kFusedCgBlocks ? 1 : 0;
+      any_split = any_split || !E.cg_fold;
+      t_visc[kv] = E;
     }
     // a fixed device buffer may be shared with other streams: the copy into
     // it waits for the kernels of its last user
@@ -1570,6 +1610,8 @@ class SceneImpl final : public SceneBase {
     const auto *d_solve = reinterpret_cast<const BatchEntry<T> *>(db + 2 * tab_bytes);
     const auto *d_pmap = reinterpret_cast<const uint32_t *>(db + off_pmap);
     const auto *d_tmap = reinterpret_cast<const uint32_t *>(db + off_tmap);
+    const auto *d_visc = reinterpret_cast<const BatchEntry<T> *>(db + off_visc);
+    const auto *d_vmap = reinterpret_cast<const uint32_t *>(db + off_vmap);
 
     int r = pf.begin(st, kStageSort);
     GridOps<T>::sort_batch(st, d_sort, d_pmap, np, d_tmap, nt, ns);
@@ -1692,10 +1734,6 @@ class SceneImpl final : public SceneBase {
     r = pf.begin(st, kStageIntegrateX);
     if (!spec)
       FusedOps<T>::integrate_batch(st, d_solve, d_pmap, np, ns);
-    if (fixed)  // the last kernel reading the tables is enqueued
-      PRTCL_HIP_CHECK(hipEventRecord(fixed->released, st));
-    for (SceneImpl *sc : v)
-      sc->steps_enqueued_ += sc->nf_ > 0 ? 1u : 0u;
     pf.end(st, r);
     if (pf.on) {
       PRTCL_HIP_CHECK(hipStreamSynchronize(st));
@@ -1703,6 +1741,84 @@ class SceneImpl final : public SceneBase {
       for (const Progress &p : prog)
         it = std::max(it, p.iters);
       pf.resolve(it);
+    }
+    // the WKBB18 scenes: a = the pressure acceleration, integrate v, the
+    // diagonal and the CG set-up in one list walk, the PCG of every scene
+    // until each reports done, apply, integrate v and x (the fused step's
+    // order for implicit viscosity)
+    std::vector<Progress> cgp(nvs);
+    if (nvs > 0) {
+      r = pf.begin(st, kStageGravityIntegrateV);
+      SphOps<T>::integrate_v_batch(st, d_visc, d_vmap, nvb, true);
+      pf.end(st, r);
+      r = pf.begin(st, kStageWkbb18DiagSetup);
+      Wkbb18Ops<T>::diagonal_setup_batch(st, d_visc, d_vmap, nvb, nvs);
+      pf.end(st, r);
+      int cap = 1, pred = 1;
+      for (unsigned kv = 0; kv < nvs; ++kv) {
+        const SceneImpl &s = *v[vsc[kv]];
+        cap = std::max(cap, std::max(0, static_cast<int>(s.desc.wkbb18_maximum_iterations)) + 1);
+        pred = std::max(pred, std::min(std::max(1, s.predicted_cg_), 16));
+      }
+      int cenq = 0;
+      auto cg_enqueue = [&](int n) {
+        for (int j = 0; j < n; ++j) {
+          const int q1 = pf.begin(st, kStagePcgSpmv, cenq + j);
+          Wkbb18Ops<T>::iteration_batch(st, d_visc, d_vmap, nvb, nvs, any_split);
+          pf.end(st, q1);
+        }
+        cenq += n;
+      };
+      // the predicted iterations at once, then one queued ahead while the
+      // host polls each scene's progress word
+      cg_enqueue(std::min(pred, cap));
+      std::vector<char> cg_fin(nvs, 0);
+      for (;;) {
+        const int ahead = cenq < cap ? 1 : 0;
+        if (ahead)
+          cg_enqueue(1);
+        bool all = true;
+        for (unsigned kv = 0; kv < nvs; ++kv) {
+          if (cg_fin[kv])
+            continue;
+          cgp[kv] = v[vsc[kv]]->wait_progress(cg_seqs[kv], cenq - ahead, 2);
+          if (cgp[kv].done) {
+            cg_fin[kv] = 1;
+          } else if (!ahead) {
+            cgp[kv] = v[vsc[kv]]->wait_progress(cg_seqs[kv], cenq, 2);
+            cg_fin[kv] = 1;
+          } else {
+            all = false;
+          }
+        }
+        if (all)
+          break;
+      }
+      r = pf.begin(st, kStagePcgVector);
+      Wkbb18Ops<T>::apply_batch(st, d_visc, d_vmap, nvb);
+      pf.end(st, r);
+      r = pf.begin(st, kStageGravityIntegrateV);
+      SphOps<T>::integrate_v_batch(st, d_visc, d_vmap, nvb, false);
+      pf.end(st, r);
+      r = pf.begin(st, kStageIntegrateX);
+      SphOps<T>::integrate_x_batch(st, d_visc, d_vmap, nvb, nvs);
+      pf.end(st, r);
+      if (pf.on) {
+        PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+        int it = 0;
+        for (const Progress &p : cgp)
+          it = std::max(it, p.iters);
+        pf.resolve(it);
+      }
+    }
+    if (fixed)  // the last kernel reading the tables is enqueued
+      PRTCL_HIP_CHECK(hipEventRecord(fixed->released, st));
+    for (SceneImpl *sc : v)
+      sc->steps_enqueued_ += sc->nf_ > 0 ? 1u : 0u;
+    for (unsigned kv = 0; kv < nvs; ++kv) {
+      SceneImpl &s = *v[vsc[kv]];
+      s.predicted_cg_ = std::max(1, cgp[kv].iters);
+      s.last_wkbb_ = cgp[kv].iters;
     }
     int status = PRTCL_HIP_OK;
     for (unsigned k = 0; k < ns; ++k) {

@@ -163,18 +163,23 @@ __global__ __launch_bounds__(kBlock) void k_viscosity(SceneView<T> S) {
 // ---- symplectic_euler.integrate_velocity_with_hard_fade
 //      (symplectic_euler.hpp:172-200): v += (dt * a) * [0 < (t - t_birth) - fade]
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_integrate_v(SceneView<T> S) {
+__device__ __forceinline__ void integrate_v_body(const SceneView<T> &S, int i,
+                                                 const vec4_t<T> &a) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= S.nf)
-    return;
   vec4_t<T> v = S.v[i];
-  const vec4_t<T> a = S.a[i];
   const T H = step_l(T(0), (S.t - v.w) - S.fade);
   v.x += (S.dt * a.x) * H;
   v.y += (S.dt * a.y) * H;
   v.z += (S.dt * a.z) * H;
   S.v[i] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_integrate_v(SceneView<T> S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nf)
+    return;
+  integrate_v_body(S, i, S.a[i]);
 }
 
 template <class T>
@@ -193,9 +198,9 @@ __device__ __forceinline__ void atomic_max_speed<double>(Ctl *ctl, double s) {
 //      x += dt * v, running max of |v| (bit patterns of non-negative IEEE
 //      values order like the values, so an integer atomic max is exact)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_integrate_x(SceneView<T> S) {
+__device__ __forceinline__ void integrate_x_body(const SceneView<T> &S, unsigned lb) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   T speed = T(0);
   bool bad = false;
   if (i < S.nf) {
@@ -212,9 +217,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_x(SceneView<T> S) {
   }
   speed = block_max<kBlock>(speed);
   if (threadIdx.x == 0)
-    S.partials[blockIdx.x] = static_cast<double>(speed);
+    S.partials[lb] = static_cast<double>(speed);
   if (bad)
     S.ctl->nonfinite = 1;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_integrate_x(SceneView<T> S) {
+  integrate_x_body(S, blockIdx.x);
 }
 
 // ---- iisph.setup pass 1 (iisph.hpp:197-249): helper c
@@ -391,8 +401,8 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const doub
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double *partials,
-                                                           unsigned nparts) {
+__device__ __forceinline__ void speed_finalize_body(Ctl *c, const double *partials,
+                                                    unsigned nparts) {
   double m = 0.0;
   for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
     m = partials[k] > m ? partials[k] : m;
@@ -401,6 +411,52 @@ __global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double 
     atomic_max_speed<T>(c, static_cast<T>(m));
     publish_step_end(c);
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_speed_finalize(Ctl *c, const double *partials,
+                                                           unsigned nparts) {
+  speed_finalize_body<T>(c, partials, nparts);
+}
+
+// ---- batched forms over the blocks of several scenes (map: block map, entry
+// of scene k in bits 20.., its local block below), each block with its
+// scene's view: the bodies above, so every scene's result is the same as
+// stepped alone. from_xa: a is first taken from the pressure acceleration
+// the last IISPH pass wrote (the copy the fused path makes before a
+// viscosity stage)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_integrate_v_b(const BatchEntry<T> *tab,
+                                                          const uint32_t *map, int from_xa) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  const SceneView<T> &S = E.S;
+  const int i = static_cast<int>((u & kBatchLocalMask) * kBlock + threadIdx.x);
+  if (i >= S.nf)
+    return;
+  vec4_t<T> a;
+  if (from_xa) {
+    a = S.xa[S.xa_a(static_cast<size_t>(i))];
+    S.a[i] = a;
+  } else {
+    a = S.a[i];
+  }
+  integrate_v_body(S, i, a);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_integrate_x_b(const BatchEntry<T> *tab,
+                                                          const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  integrate_x_body(E.S, u & kBatchLocalMask);
+}
+
+// one block per scene
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_speed_finalize_tab(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  speed_finalize_body<T>(E.S.ctl, E.S.partials, E.nblocks);
 }
 
 // ---- neighbour export (Neighborhood::CopyNeighbors, neighborhood.cpp:156-167)
@@ -472,6 +528,23 @@ template <class T>
 void SphOps<T>::integrate_v(hipStream_t st, const SceneView<T> &S) {
   PRTCL_FLUID_LAUNCH(k_integrate_v<T>);
 }
+template <class T>
+void SphOps<T>::integrate_v_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                  unsigned nblocks, bool from_xa) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_integrate_v_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map,
+                       from_xa ? 1 : 0);
+}
+
+template <class T>
+void SphOps<T>::integrate_x_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                  unsigned nblocks, unsigned nscenes) {
+  if (nblocks == 0)
+    return;
+  hipLaunchKernelGGL(k_integrate_x_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+  hipLaunchKernelGGL(k_speed_finalize_tab<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
+}
+
 template <class T>
 void SphOps<T>::integrate_x(hipStream_t st, const SceneView<T> &S) {
   if (S.nf <= 0)

@@ -16,6 +16,12 @@ struct SphOps {
   static void viscosity(hipStream_t st, const SceneView<T> &S);
   static void integrate_v(hipStream_t st, const SceneView<T> &S);
   static void integrate_x(hipStream_t st, const SceneView<T> &S);
+  // the two integrations over every scene of a batch (map: block map of the
+  // scenes' blocks); from_xa: a = the last pressure acceleration first
+  static void integrate_v_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                unsigned nblocks, bool from_xa);
+  static void integrate_x_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                unsigned nblocks, unsigned nscenes);
   static void iisph_setup(hipStream_t st, const SceneView<T> &S);
   static void iisph_accel(hipStream_t st, const SceneView<T> &S, bool solve);
   static void iisph_pressure(hipStream_t st, const SceneView<T> &S, bool solve);

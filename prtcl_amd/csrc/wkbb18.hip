@@ -591,10 +591,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup(SceneView<T> S, NbrList L, 
 // (fused steps): every pair term is the expression of the two kernels, in the
 // same order, with (x_j, m_j / rho_j) formed from x_j, m_j and rho_j as the
 // diagonal pass writes it, so the results equal those of the two kernels
+// (lb: the block's logical index, nb: the scene's block count)
 template <class T, bool kGC>
-__global__ __launch_bounds__(kBlock) void k_diag_setup_list(SceneView<T> S, NbrList L,
-                                                            double *part) {
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void diag_setup_list_body(const SceneView<T> &S, const NbrList &L,
+                                                     double *part, unsigned lb, unsigned nb) {
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < S.nf) {
@@ -702,15 +702,21 @@ __global__ __launch_bounds__(kBlock) void k_diag_setup_list(SceneView<T> S, NbrL
     acc[1] = static_cast<double>(dot3r<T>(r, r));
     acc[2] = static_cast<double>(dot3r<T>(r, y));
   }
-  store_partials<3>(part, gridDim.x, acc, lb);
+  store_partials<3>(part, nb, acc, lb);
+}
+
+template <class T, bool kGC>
+__global__ __launch_bounds__(kBlock) void k_diag_setup_list(SceneView<T> S, NbrList L,
+                                                            double *part) {
+  diag_setup_list_body<T, kGC>(S, L, part, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // q = A p, partial p.q
 template <class T, bool kExact, bool kGC>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, double *part) {
+__device__ __forceinline__ void cg_spmv_body(const SceneView<T> &S, const NbrList &L,
+                                             double *part, unsigned lb, unsigned nb) {
   if (S.ctl->cg_done)
     return;
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[1] = {0.0};
   if (i < S.nf) {
@@ -723,13 +729,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, d
     S.cq[i] = qv;
     acc[0] = static_cast<double>(dot3r<T>(S.cxp[2 * static_cast<size_t>(i) + 1], qv));
   }
-  store_partials<1>(part, gridDim.x, acc, lb);
+  store_partials<1>(part, nb, acc, lb);
+}
+
+template <class T, bool kExact, bool kGC>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(SceneView<T> S, NbrList L, double *part) {
+  cg_spmv_body<T, kExact, kGC>(S, L, part, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // x += alpha p, r += alpha q, y = P r; partials r.r, r.y
 template <class T>
-__device__ __forceinline__ void cg_update_body(const SceneView<T> &S, T alpha, double *part) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void cg_update_body(const SceneView<T> &S, T alpha, double *part,
+                                               unsigned lb, unsigned nb) {
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[2] = {0.0, 0.0};
   if (i < S.nf) {
     const vec4_t<T> p = S.cxp[2 * static_cast<size_t>(i) + 1], q = S.cq[i];
@@ -747,20 +759,20 @@ __device__ __forceinline__ void cg_update_body(const SceneView<T> &S, T alpha, d
     acc[0] = static_cast<double>(dot3r<T>(r, r));
     acc[1] = static_cast<double>(dot3r<T>(r, y));
   }
-  store_partials<2>(part, gridDim.x, acc);
+  store_partials<2>(part, nb, acc, lb);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(SceneView<T> S, double *part) {
   if (S.ctl->cg_done)
     return;
-  cg_update_body(S, static_cast<T>(S.ctl->cg_alpha), part);
+  cg_update_body(S, static_cast<T>(S.ctl->cg_alpha), part, blockIdx.x, gridDim.x);
 }
 
 // p = -y + beta p
 template <class T>
-__device__ __forceinline__ void cg_direction_body(const SceneView<T> &S, T beta) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void cg_direction_body(const SceneView<T> &S, T beta, unsigned lb) {
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> y = S.cy[i];
@@ -775,7 +787,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(SceneView<T> S) {
   if (S.ctl->cg_done)
     return;
-  cg_direction_body(S, static_cast<T>(S.ctl->cg_beta));
+  cg_direction_body(S, static_cast<T>(S.ctl->cg_beta), blockIdx.x);
 }
 
 // The iteration without its two one-block kernels (cg_alpha_in_block,
@@ -789,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_alpha(SceneView<T> S, cons
   double alpha;
   if (!cg_alpha_in_block(S.ctl, pq_part, gridDim.x, &alpha))
     return;
-  cg_update_body(S, static_cast<T>(alpha), part);
+  cg_update_body(S, static_cast<T>(alpha), part, blockIdx.x, gridDim.x);
 }
 
 template <class T>
@@ -799,14 +811,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction_beta(SceneView<T> S, co
   double beta;
   if (!cg_beta_in_block(S.ctl, part, gridDim.x, &beta))
     return;
-  cg_direction_body(S, static_cast<T>(beta));
+  cg_direction_body(S, static_cast<T>(beta), blockIdx.x);
 }
 
 // apply (wkbb18.hpp:401-413): vd = (x - v) H(t - t_birth - dt_fade), a = vd / dt
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_apply(SceneView<T> S) {
+__device__ __forceinline__ void cg_apply_body(const SceneView<T> &S, unsigned lb) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> x = S.cx[i], v = S.v[i];
@@ -814,6 +826,109 @@ __global__ __launch_bounds__(kBlock) void k_cg_apply(SceneView<T> S) {
   const vec4_t<T> vd = make4<vec4_t<T>>((x.x - v.x) * H, (x.y - v.y) * H, (x.z - v.z) * H, T(0));
   S.vd[i] = vd;
   S.a[i] = make4<vec4_t<T>>(vd.x / S.dt, vd.y / S.dt, vd.z / S.dt, T(0));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_apply(SceneView<T> S) {
+  cg_apply_body(S, blockIdx.x);
+}
+
+// ---- batched WKBB18 (prtcl_hip_step over several scenes): the kernels above
+// over the blocks of every scene of a table (map: block map, entry of scene k
+// in bits 20.., its local block below), each block with its scene's view,
+// list, CG partials and block count, so every scene's sums and decisions are
+// those of its single-scene step. A scene with cg_fold derives alpha and
+// beta in its vector kernels, otherwise its one-block kernels do.
+template <class T>
+__device__ __forceinline__ NbrList entry_list(const BatchEntry<T> &E) {
+  return NbrList{E.nidx, E.ninfo, E.nblk};
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_diag_setup_b(const BatchEntry<T> *tab,
+                                                         const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  diag_setup_list_body<T, false>(E.S, entry_list(E), E.cg_part, u & kBatchLocalMask, E.nblocks);
+}
+
+// one block per scene
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_setup_fin_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  cg_setup_fin_body(E.S.ctl, E.cg_part, E.nblocks, E.cg_tol, E.cg_max_k, E.cg_seq);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_b(const BatchEntry<T> *tab,
+                                                      const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  cg_spmv_body<T, false, false>(E.S, entry_list(E), E.cg_part, u & kBatchLocalMask, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_alpha_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  if (!E.cg_fold)
+    cg_alpha_body(E.S.ctl, E.cg_part, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_b(const BatchEntry<T> *tab,
+                                                        const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  const unsigned lb = u & kBatchLocalMask, nb = E.nblocks;
+  Ctl *c = E.S.ctl;
+  if (E.cg_fold) {  // p.q partials at [0, nb), r.r and r.y at [nb, 3 nb)
+    if (cg_done_for_block(c))
+      return;
+    double alpha;
+    if (!cg_alpha_in_block(c, E.cg_part, nb, &alpha, lb))
+      return;
+    cg_update_body(E.S, static_cast<T>(alpha), E.cg_part + nb, lb, nb);
+  } else {
+    if (c->cg_done)
+      return;
+    cg_update_body(E.S, static_cast<T>(c->cg_alpha), E.cg_part, lb, nb);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_beta_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  if (!E.cg_fold)
+    cg_beta_body(E.S.ctl, E.cg_part, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_direction_b(const BatchEntry<T> *tab,
+                                                           const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  const unsigned lb = u & kBatchLocalMask, nb = E.nblocks;
+  Ctl *c = E.S.ctl;
+  if (E.cg_fold) {
+    if (cg_done_for_block(c))
+      return;
+    double beta;
+    if (!cg_beta_in_block(c, E.cg_part + nb, nb, &beta, lb))
+      return;
+    cg_direction_body(E.S, static_cast<T>(beta), lb);
+  } else {
+    if (c->cg_done)
+      return;
+    cg_direction_body(E.S, static_cast<T>(c->cg_beta), lb);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_apply_b(const BatchEntry<T> *tab,
+                                                       const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  cg_apply_body(E.S, u & kBatchLocalMask);
 }
 
 }  // namespace
@@ -928,6 +1043,38 @@ template <class T>
 void Wkbb18Ops<T>::apply(hipStream_t st, const SceneView<T> &S) {
   if (S.nf > 0)
     hipLaunchKernelGGL(k_cg_apply<T>, dim3(grid_blocks(S.nf)), dim3(kBlock), 0, st, S);
+}
+
+template <class T>
+void Wkbb18Ops<T>::diagonal_setup_batch(hipStream_t st, const BatchEntry<T> *tab,
+                                        const uint32_t *map, unsigned nblocks, unsigned nscenes) {
+  if (nscenes == 0)
+    return;
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_diag_setup_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+  hipLaunchKernelGGL(k_cg_setup_fin_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
+}
+
+template <class T>
+void Wkbb18Ops<T>::iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                   unsigned nblocks, unsigned nscenes, bool any_split) {
+  if (nblocks == 0)
+    return;
+  const dim3 gr(nblocks), bl(kBlock);
+  hipLaunchKernelGGL(k_cg_spmv_b<T>, gr, bl, 0, st, tab, map);
+  if (any_split)
+    hipLaunchKernelGGL(k_cg_alpha_b<T>, dim3(nscenes), bl, 0, st, tab);
+  hipLaunchKernelGGL(k_cg_update_b<T>, gr, bl, 0, st, tab, map);
+  if (any_split)
+    hipLaunchKernelGGL(k_cg_beta_b<T>, dim3(nscenes), bl, 0, st, tab);
+  hipLaunchKernelGGL(k_cg_direction_b<T>, gr, bl, 0, st, tab, map);
+}
+
+template <class T>
+void Wkbb18Ops<T>::apply_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                               unsigned nblocks) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_cg_apply_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
 }
 
 template struct Wkbb18Ops<float>;

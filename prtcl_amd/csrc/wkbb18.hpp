@@ -28,6 +28,17 @@ struct Wkbb18Ops {
                                 bool split);
   // vd = (x - v) H, a = vd / dt
   static void apply(hipStream_t st, const SceneView<T> &S);
+  // the same over every scene of a batch (WKBB18 without gradient
+  // correction, fused steps): map is the block map of the scenes' blocks,
+  // each table entry carries the scene's CG partials, tolerance, cap,
+  // sequence number and fold choice. any_split: some scene takes its alpha
+  // and beta from the one-block kernels
+  static void diagonal_setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                   unsigned nblocks, unsigned nscenes);
+  static void iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                              unsigned nblocks, unsigned nscenes, bool any_split);
+  static void apply_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                          unsigned nblocks);
   // correction.compute_gradient_correction: L_i = pinv(sum_f (m_j / rho_i)
   // gradW_ij (x_j - x_i)^T), over the grid (L null) or the step's list
   static void correction(hipStream_t st, const SceneView<T> &S, const NbrList *L);

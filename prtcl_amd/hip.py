@@ -475,8 +475,8 @@ class HipScene:
         """Force the schedule of the fused passes: bit 0 setup, bit 1
         pressure acceleration, bit 2 pressure in LDS; bit 3 eager IISPH
         iterations; bit 4 separate one-block alpha/beta kernels in the CG
-        solves; bit 5 separate WKBB18 diagonal and CG set-up passes
-        (-1: autotune)."""
+        solves; bit 5 separate WKBB18 diagonal and CG set-up passes; bit 6
+        IISPH iteration 1 in its own passes (-1: autotune)."""
         self.ctx.check(self.L.prtcl_hip_set_schedule(self.ctx.h, self.id, mask), "set_schedule")
 
     def schedule(self) -> int:

@@ -191,7 +191,12 @@ inline long min(long a, int b) { return a < b ? a : b; }
 template <class T, class U>
 inline T atomicAdd(T *p, U v) { T o = *p; *p = static_cast<T>(o + v); return o; }
 template <class T, class U>
+inline T atomicExch(T *p, U v) { T o = *p; *p = static_cast<T>(v); return o; }
+template <class T, class U>
 inline T atomicMax(T *p, U v) { T o = *p; if (static_cast<T>(v) > o) *p = static_cast<T>(v); return o; }
+// the GPU threads of the emulation share one address space and run on one
+// host thread at a time: a device-scope fence has nothing to order
+inline void __threadfence() {}
 template <class T, class U>
 inline T atomicMin(T *p, U v) { T o = *p; if (static_cast<T>(v) < o) *p = static_cast<T>(v); return o; }
 

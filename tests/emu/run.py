@@ -24,8 +24,9 @@ What the emulation checks (tests/emu/emu_rt.cpp):
   launches queued, is caught.
 
 Usage:
-    python tests/emu/run.py [--defer] [--workers N] [pytest args ...]
+    python tests/emu/run.py [--defer] [--workers N] [test paths] [pytest args ...]
 e.g. python tests/emu/run.py --defer --workers 6 --ignore=tests/test_gpu_configs.py
+     python tests/emu/run.py --defer tests/test_gpu_batch.py::test_batch_equals_alone
 Reports go to stderr and to tests/emu/_tree/emu_reports.log; ASan reports to
 tests/emu/_tree/asan.*. The emulation runs a kernel at roughly 10^5 - 10^6
 thread-instructions per second per core: the small-scene tests take minutes,
@@ -116,8 +117,11 @@ def main():
                            os.path.join(TREE, "asan"))
     env["EMU_LOG"] = os.path.join(TREE, "emu_reports.log")
     env["EMU_DEFER"] = "1" if args.defer else "0"
-    cmd = [sys.executable, "-u", "-m", "pytest", "tests", "-m", "gpu", "-v",
+    # test paths among the arguments replace the whole tests folder
+    paths = [a for a in rest if a.startswith("tests")]
+    cmd = [sys.executable, "-u", "-m", "pytest", *(paths or ["tests"]), "-m", "gpu", "-v",
            "-p", "no:cacheprovider", "-p", "no:faulthandler"]
+    rest = [a for a in rest if a not in paths]
     if args.workers > 1:
         cmd += ["-n", str(args.workers)]
     rc = subprocess.call(cmd + rest, cwd=TREE, env=env)

@@ -52,32 +52,87 @@ def test_batch_equals_alone(hip_ctx):
             assert np.array_equal(a[f], b[f]), (j, f, float(np.max(np.abs(a[f] - b[f]))))
 
 
+def run_info(ctx, scs, k, together, kinds=None, **kw):
+    """Like run(), with per-scene keyword sets and the WKBB18 counts too."""
+    from prtcl_amd import hip
+    kinds = kinds or [dict() for _ in scs]
+    hs = [hip.HipScene(ctx, sc, **kw, **kd) for sc, kd in zip(scs, kinds)]
+    for h in hs:
+        h.prepare()
+    info = [[] for _ in hs]
+    for _ in range(k):
+        if together:
+            ctx.step(hs, 1)
+        else:
+            for h in hs:
+                h.step(1)
+        for j, h in enumerate(hs):
+            d = h.step_info()
+            info[j].append((d["iisph_iterations"], d["wkbb18_iterations"]))
+    out = [{f: h.download("fluid", f) for f in FIELDS} for h in hs]
+    for h in hs:
+        h.destroy()
+    return out, info
+
+
+def viscous_mix(seed=7):
+    # five config-4-like scenes, three of them with WKBB18 implicit viscosity
+    mix = small_mix(5, seed=seed)
+    for j, mu in ((1, 5.0), (3, 20.0), (4, 60.0)):
+        mix[j].implicit_viscosity = 1
+        mix[j].fluid_viscosity = mix[j].boundary_viscosity = mu
+    return mix
+
+
+@pytest.mark.parametrize("real_type", [0, 1], ids=["fp32", "fp64"])
+def test_batch_with_viscous_scenes(hip_ctx, real_type):
+    """WKBB18 scenes are stepped inside the batch: after the batched IISPH
+    solve, their diagonal and CG set-up, the PCG iterations (each scene's FP64
+    CG control and its own convergence), apply and the integrations run over
+    all of them at once. Every scene matches its result alone bit for bit,
+    with the same IISPH and WKBB18 iteration counts (the last step crosses a
+    host-visible permutation)."""
+    mix = viscous_mix()
+    alone, it_a = run_info(hip_ctx, mix, 9, together=False, real_type=real_type)
+    batch, it_b = run_info(hip_ctx, mix, 9, together=True, real_type=real_type)
+    assert it_a == it_b
+    assert all(w > 0 for j in (1, 3, 4) for _, w in it_b[j])
+    for j, (a, b) in enumerate(zip(alone, batch)):
+        for f in FIELDS:
+            assert np.all(np.isfinite(b[f])), (j, f)
+            assert np.array_equal(a[f], b[f]), (j, f, float(np.max(np.abs(a[f] - b[f]))))
+
+
+def test_batch_viscous_split_cg_equals_alone(hip_ctx):
+    """The batched PCG with the alpha / beta reductions in one-block kernels
+    per scene (schedule bit 4) against each scene alone with them folded into
+    the vector kernels: the same fields."""
+    mix = viscous_mix(seed=8)
+    alone, it_a = run_info(hip_ctx, mix, 3, together=False)
+    hip_ctx.set_batch_schedule(8 | 16)
+    try:
+        batch, it_b = run_info(hip_ctx, mix, 3, together=True)
+    finally:
+        hip_ctx.set_batch_schedule(-1)
+    assert it_a == it_b
+    for j, (a, b) in enumerate(zip(alone, batch)):
+        for f in FIELDS:
+            assert np.array_equal(a[f], b[f]), (j, f)
+
+
 def test_batch_with_unbatchable_scenes(hip_ctx):
     """A list that mixes batchable scenes with scenes the batch does not take
-    (implicit viscosity, FP64, per-procedure kernels): every scene still
-    matches its result alone."""
-    from prtcl_amd import hip
-    mix = small_mix(4, seed=5)
-    mix[1].implicit_viscosity = True
-    mix[1].fluid_viscosity = mix[1].boundary_viscosity = 10.0
-    kinds = [dict(), dict(), dict(real_type=1), dict(step_mode=1)]
-
-    def go(together):
-        hs = [hip.HipScene(hip_ctx, sc, **kw) for sc, kw in zip(mix, kinds)]
-        for h in hs:
-            h.prepare()
-        for _ in range(3):
-            if together:
-                hip_ctx.step(hs, 1)
-            else:
-                for h in hs:
-                    h.step(1)
-        out = [{f: h.download("fluid", f) for f in FIELDS} for h in hs]
-        for h in hs:
-            h.destroy()
-        return out
-
-    alone, batch = go(False), go(True)
+    (explicit viscosity, WKBB18 with gradient correction, per-procedure
+    kernels) and an FP64 scene: every scene still matches its result alone."""
+    mix = small_mix(5, seed=5)
+    mix[1].explicit_viscosity = True
+    mix[1].fluid_viscosity = mix[1].boundary_viscosity = 0.01
+    mix[2].implicit_viscosity = 2
+    mix[2].fluid_viscosity = mix[2].boundary_viscosity = 10.0
+    kinds = [dict(), dict(), dict(), dict(real_type=1), dict(step_mode=1)]
+    alone, it_a = run_info(hip_ctx, mix, 3, together=False, kinds=kinds)
+    batch, it_b = run_info(hip_ctx, mix, 3, together=True, kinds=kinds)
+    assert it_a == it_b
     for j, (a, b) in enumerate(zip(alone, batch)):
         for f in FIELDS:
             assert np.array_equal(a[f], b[f]), (j, f)
