@@ -546,9 +546,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
 // explicit viscosity (viscosity.hpp:171-240) over the list, then
 // v* = v + dt (g + a_visc) H   (gravity.hpp:133, symplectic_euler.hpp:172-200)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_viscosity_list(SceneView<T> S, NbrList L,
-                                                           vec4_t<T> *vout) {
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void k_viscosity_list_body(const SceneView<T> &S, const NbrList &L,
+                                                      vec4_t<T> *vout, unsigned lb) {
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -593,11 +592,18 @@ __global__ __launch_bounds__(kBlock) void k_viscosity_list(SceneView<T> S, NbrLi
                              vi.z + S.dt * (g2 + a2) * H, vi.w);
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_viscosity_list(SceneView<T> S, NbrList L,
+                                                           vec4_t<T> *vout) {
+  k_viscosity_list_body(S, L, vout, xcd_block(blockIdx.x, gridDim.x));
+}
+
 // v* = v + dt a_axial(x) H (gravity.hpp:189-215, symplectic_euler.hpp:172-200)
 // for steps with axial gravity and no explicit viscosity
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gravity_v(SceneView<T> S, vec4_t<T> *vout) {
-  const int i = static_cast<int>(blockIdx.x * kBlock + threadIdx.x);
+__device__ __forceinline__ void k_gravity_v_body(const SceneView<T> &S, vec4_t<T> *vout,
+                                                 unsigned lb) {
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> xi = S.xm[i], vi = S.v[i];
@@ -608,10 +614,15 @@ __global__ __launch_bounds__(kBlock) void k_gravity_v(SceneView<T> S, vec4_t<T> 
                              vi.z + S.dt * g2 * H, vi.w);
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gravity_v(SceneView<T> S, vec4_t<T> *vout) {
+  k_gravity_v_body(S, vout, blockIdx.x);
+}
+
 // source term s from the integrated velocities (explicit-viscosity path)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList L) {
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void k_source_list_body(const SceneView<T> &S, const NbrList &L,
+                                                   unsigned lb) {
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double term = 0.0;
   if (i < S.nf) {
@@ -639,6 +650,11 @@ __global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList 
     if (threadIdx.x == 0)
       S.partials[lb] = bs;
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_source_list(SceneView<T> S, NbrList L) {
+  k_source_list_body(S, L, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Walk of the list entries of particle i (all threads of the block call it).
@@ -905,6 +921,40 @@ __global__ __launch_bounds__(kBlock) void k_setup_b(const BatchEntry<T> *tab,
                                 u & kBatchLocalMask);
 }
 
+// scenes with explicit viscosity or axial gravity: the setup without the
+// source term, then v* (explicit viscosity and gravity, or the axial gravity
+// alone) into vtmp, then the source term and iteration 1 from v* (the three
+// launches of FusedOps::setup for such a scene)
+template <class T, bool kStage>
+__global__ __launch_bounds__(kBlock) void k_setup_nosrc_b(const BatchEntry<T> *tab,
+                                                          const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_setup_body<T, false, kStage>(E.S, NbrList{E.nidx, E.ninfo, E.nblk}, E.vtmp,
+                                 u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_vstar_b(const BatchEntry<T> *tab,
+                                                    const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  if (E.explicit_visc)
+    k_viscosity_list_body(E.S, NbrList{E.nidx, E.ninfo, E.nblk}, E.vtmp, u & kBatchLocalMask);
+  else
+    k_gravity_v_body(E.S, E.vtmp, u & kBatchLocalMask);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_source_b(const BatchEntry<T> *tab,
+                                                     const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  SceneView<T> S2 = E.S;
+  S2.v = E.vtmp;  // the integrated velocities
+  k_source_list_body(S2, NbrList{E.nidx, E.ninfo, E.nblk}, u & kBatchLocalMask);
+}
+
 template <class T, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_accel_b(const BatchEntry<T> *tab,
                                                     const uint32_t *pmap) {
@@ -926,6 +976,14 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_pressure_finalize_b(const BatchEntry<T> *tab) {
   const BatchEntry<T> &E = tab[blockIdx.x];
   k_pressure_finalize_body(E.S.ctl, E.S.partials, E.nblocks);
+}
+
+// iteration 1's convergence step for the scenes whose solve it could end
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_first_finalize_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  if (!E.first_preset)
+    k_pressure_finalize_body(E.S.ctl, E.S.partials, E.nblocks);
 }
 
 // guarded: each scene's integration only once its solve (E.seq) is done;
@@ -1023,7 +1081,7 @@ template <class T>
 void FusedOps<T>::finish_first_iteration_batch(hipStream_t st, const BatchEntry<T> *tab,
                                                unsigned nscenes) {
   if (nscenes > 0)
-    hipLaunchKernelGGL(k_pressure_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
+    hipLaunchKernelGGL(k_first_finalize_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
 }
 
 template <class T>
@@ -1051,6 +1109,20 @@ void FusedOps<T>::setup_batch(hipStream_t st, const BatchEntry<T> *tab, const ui
     hipLaunchKernelGGL((k_setup_b<T, true>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
   else
     hipLaunchKernelGGL((k_setup_b<T, false>), dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+}
+
+template <class T>
+void FusedOps<T>::setup_split_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                    unsigned nblocks, bool lds) {
+  if (nblocks == 0)
+    return;
+  const dim3 gr(nblocks), bl(kBlock);
+  if (lds)
+    hipLaunchKernelGGL((k_setup_nosrc_b<T, true>), gr, bl, 0, st, tab, map);
+  else
+    hipLaunchKernelGGL((k_setup_nosrc_b<T, false>), gr, bl, 0, st, tab, map);
+  hipLaunchKernelGGL(k_vstar_b<T>, gr, bl, 0, st, tab, map);
+  hipLaunchKernelGGL(k_source_b<T>, gr, bl, 0, st, tab, map);
 }
 
 template <class T>

@@ -106,7 +106,8 @@ struct FusedOps {
   static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
   static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
   // the convergence step of IISPH iteration 1, whose pressures and aprde
-  // partials the setup (with S.first_iter) has written
+  // partials the setup (with S.first_iter) has written (batched: for the
+  // scenes without first_preset)
   static void finish_first_iteration(hipStream_t st, const SceneView<T> &S);
   static void finish_first_iteration_batch(hipStream_t st, const BatchEntry<T> *tab,
                                            unsigned nscenes);
@@ -119,6 +120,10 @@ struct FusedOps {
   // the same passes over every scene of a batch (pmap: block map)
   static void setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
                           unsigned npblocks, bool lds);
+  // the three setup launches of scenes with explicit viscosity (entry flag
+  // explicit_visc) or axial gravity, over the blocks of `map`
+  static void setup_split_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                unsigned nblocks, bool lds);
   static void accel_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
                           unsigned npblocks, bool lds);
   static void pressure_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,

@@ -280,6 +280,48 @@ __global__ __launch_bounds__(kBlock) void k_stable_rank(GridDims g, const uint32
   k_stable_rank_body(g, cell, start, perm, n, perm_out, blockIdx.x);
 }
 
+// k_stable_rank and the reorder of the carried state in one pass: the thread
+// of unstable slot s finds the stable slot p of its particle o = perm[s] as
+// k_stable_rank does and moves x, v, id and cell of o to p (the same
+// permutation as gathering through k_stable_rank's result, without writing it)
+template <class V>
+__device__ __forceinline__ void k_place_state_body(GridDims g, const uint32_t *__restrict__ cell,
+                                                   const uint32_t *__restrict__ start,
+                                                   const uint32_t *__restrict__ perm, int n,
+                                                   V *__restrict__ xd, const V *__restrict__ xs,
+                                                   V *__restrict__ vd, const V *__restrict__ vs,
+                                                   uint32_t *__restrict__ id_d,
+                                                   const uint32_t *__restrict__ id_s,
+                                                   uint32_t *__restrict__ c_d, unsigned bid) {
+  const int s = bid * kBlock + threadIdx.x;
+  if (s >= n)
+    return;
+  const uint32_t o = perm[s];
+  const uint32_t c = cell[o];
+  const uint32_t lin = (c & 0x7FFu) + static_cast<uint32_t>(g.nx) *
+                                          (((c >> 11) & 0x7FFu) +
+                                           static_cast<uint32_t>(g.ny) * (c >> 22));
+  const uint32_t b = start[lin], e = start[lin + 1];
+  uint32_t r = 0;
+  for (uint32_t k = b; k < e; ++k)
+    r += perm[k] < o ? 1u : 0u;
+  const uint32_t p = b + r;
+  xd[p] = xs[o];
+  vd[p] = vs[o];
+  id_d[p] = id_s[o];
+  c_d[p] = c;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_place_state(GridDims g, const uint32_t *cell,
+                                                        const uint32_t *start,
+                                                        const uint32_t *perm, int n, V *xd,
+                                                        const V *xs, V *vd, const V *vs,
+                                                        uint32_t *id_d, const uint32_t *id_s,
+                                                        uint32_t *c_d) {
+  k_place_state_body<V>(g, cell, start, perm, n, xd, xs, vd, vs, id_d, id_s, c_d, blockIdx.x);
+}
+
 template <int W>
 struct Words {
   uint32_t w[W];
@@ -385,6 +427,18 @@ __global__ __launch_bounds__(kBlock) void k_stable_rank_b(const BatchEntry<T> *t
                      u & kBatchLocalMask);
 }
 
+// cell_dst must not be the perm buffer: perm is read by every thread
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_place_state_b(const BatchEntry<T> *tab,
+                                                          const uint32_t *pmap) {
+  const uint32_t u = pmap[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  k_place_state_body<vec4_t<T>>(E.S.grid, E.S.fcell, E.start, E.perm, E.S.nf, E.x_dst, E.S.xm,
+                                E.v_dst, E.S.v, E.id_dst,
+                                reinterpret_cast<const uint32_t *>(E.S.id), E.cell_dst,
+                                u & kBatchLocalMask);
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gather_state_b(const BatchEntry<T> *tab,
                                                            const uint32_t *pmap) {
@@ -427,6 +481,24 @@ const uint32_t *GridOps<T>::sort_group(hipStream_t st, const GridDims &g, const 
                        start, perm, n, rank);
   }
   return rank;
+}
+
+template <class T>
+void GridOps<T>::sort_state(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
+                            uint32_t *cell, uint32_t *count, uint32_t *start, uint32_t *rank,
+                            uint32_t *perm, uint32_t *scan_tmp, vec4_t<T> *xd, vec4_t<T> *vd,
+                            const vec4_t<T> *vs, uint32_t *id_d, const uint32_t *id_s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_keys<T>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g, x, n,
+                       cell, count, rank);
+  scan_counts(st, count, start, g.ncells, scan_tmp);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g, cell, rank,
+                       start, n, perm);
+    // the atomic ranks are consumed: the cells go to `rank`
+    hipLaunchKernelGGL(k_place_state<vec4_t<T>>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, g,
+                       cell, start, perm, n, xd, x, vd, vs, id_d, id_s, rank);
+  }
 }
 
 template <class T>
@@ -495,8 +567,7 @@ void GridOps<T>::sort_batch(hipStream_t st, const BatchEntry<T> *tab, const uint
     hipLaunchKernelGGL(k_scan_down_b<T>, dim3(ntiles), dim3(kScanBS), 0, st, tab, tmap);
   if (npblocks > 0) {
     hipLaunchKernelGGL(k_scatter_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
-    hipLaunchKernelGGL(k_stable_rank_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
-    hipLaunchKernelGGL(k_gather_state_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
+    hipLaunchKernelGGL(k_place_state_b<T>, dim3(npblocks), dim3(kBlock), 0, st, tab, pmap);
   }
 }
 

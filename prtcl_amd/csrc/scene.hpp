@@ -229,9 +229,16 @@ struct BatchEntry {
   double tol;
   int min_it, max_it, nprde;
   unsigned seq;
+  // iteration 1 is taken by the setup and cannot end the solve: the solve
+  // starts at 1 iteration without its convergence step
+  int first_preset;
   // the step ends with the WKBB18 path (implicit viscosity), not with the
   // fused integration
   int viscous;
+  // the setup takes three launches (explicit viscosity or axial gravity);
+  // explicit_visc: v* from the explicit viscosity pass (else axial gravity)
+  int split_setup;
+  int explicit_visc;
   // the scene's WKBB18 solve in this step: FP64 CG partials, tolerance, cap
   // and sequence number; cg_fold: alpha and beta derived in the vector
   // kernels (at most kFusedCgBlocks blocks)
@@ -284,11 +291,18 @@ struct GridOps {
   static const uint32_t *sort_group(hipStream_t st, const GridDims &g, const vec4_t<T> *x,
                                     int n, uint32_t *cell, uint32_t *count, uint32_t *start,
                                     uint32_t *rank, uint32_t *perm, uint32_t *scan_tmp);
+  // sort_group and the reorder of the carried state (x -> xd, v -> vd, id ->
+  // id_d, cell -> rank) in one fewer pass; no permutation array results
+  static void sort_state(hipStream_t st, const GridDims &g, const vec4_t<T> *x, int n,
+                         uint32_t *cell, uint32_t *count, uint32_t *start, uint32_t *rank,
+                         uint32_t *perm, uint32_t *scan_tmp, vec4_t<T> *xd, vec4_t<T> *vd,
+                         const vec4_t<T> *vs, uint32_t *id_d, const uint32_t *id_s);
   static void gather4(hipStream_t st, void *dst, const void *src, const uint32_t *perm, int n);
   static void gather_elems(hipStream_t st, void *dst, const void *src,
                            const uint32_t *perm, int n, int elem_bytes);
-  // the whole fluid sort (keys .. gather) of every scene of a batch; the
-  // permutation of each scene ends in its `rank` buffer
+  // the whole fluid sort (keys .. reorder of the carried state into the
+  // entries' destinations) of every scene of a batch; cell_dst must not be
+  // the perm buffer
   static void sort_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *pmap,
                          unsigned npblocks, const uint32_t *tmap, unsigned ntiles,
                          unsigned nscenes);

@@ -212,13 +212,18 @@ __global__ void k_iota(int32_t *id, int n) {
 
 // control-block operations (kernels, not memsets, so they can live in graphs)
 enum CtlOp { kCtlInit = 0, kCtlBeginSolve = 1, kCtlSetNprde = 2, kCtlSetAprde = 3,
-             kCtlSetMaxSpeed = 4, kCtlSetNprdeValue = 5 };
+             kCtlSetMaxSpeed = 4, kCtlSetNprdeValue = 5, kCtlBeginSolveAfterFirst = 6 };
 
-__device__ void begin_solve(Ctl *c, double tol, int min_it, int max_it, unsigned seq) {
+// iters0 = 1: iteration 1 is taken (by the setup pass) and, with min_it >= 2
+// and max_it >= 2, its convergence step could not stop the solve, so the
+// count starts at 1 without that step (its aprde is never read: the solve
+// reports the last iteration's)
+__device__ void begin_solve(Ctl *c, double tol, int min_it, int max_it, unsigned seq,
+                            int iters0 = 0) {
   c->aprde = 0.0;
   c->rel = 0.0;
   c->tol = tol;
-  c->iters = 0;
+  c->iters = iters0;
   c->done = (c->nprde >= 1) ? 0 : 1;
   c->converged = 0;
   c->min_it = min_it;
@@ -235,9 +240,10 @@ __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2, int i3,
       c->host_mirror = mirror;
       break;
     case kCtlBeginSolve:  // i3 >= 0: nprde first, in the same launch
+    case kCtlBeginSolveAfterFirst:
       if (i3 >= 0)
         c->nprde = i3;
-      begin_solve(c, dv, i0, i1, static_cast<unsigned>(i2));
+      begin_solve(c, dv, i0, i1, static_cast<unsigned>(i2), op == kCtlBeginSolveAfterFirst ? 1 : 0);
       break;
     case kCtlSetNprde:
     case kCtlSetNprdeValue:
@@ -253,6 +259,12 @@ __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2, int i3,
   }
 }
 
+// the convergence step of IISPH iteration 1 cannot end a solve of at least 2
+// iterations with a cap of at least 2 (example.lua:418-419)
+inline bool first_iteration_cannot_stop(int min_it, int max_it) {
+  return min_it >= 2 && max_it >= 2;
+}
+
 // nprde and the start of the IISPH solve of every scene of a batch (one
 // thread per scene)
 template <class T>
@@ -262,7 +274,7 @@ __global__ void k_begin_solve_b(const BatchEntry<T> *tab, unsigned nscenes) {
     return;
   const BatchEntry<T> &E = tab[k];
   E.S.ctl->nprde = E.nprde;
-  begin_solve(E.S.ctl, E.tol, E.min_it, E.max_it, E.seq);
+  begin_solve(E.S.ctl, E.tol, E.min_it, E.max_it, E.seq, E.first_preset);
 }
 
 struct Arena {
@@ -701,6 +713,24 @@ class SceneImpl final : public SceneBase {
   // carries over between steps (x, m, v, t_birth, id)
   void build_fluid_grid(bool all) {
     hipStream_t st = res_->stream;
+    if (!all) {
+      // the step's sort: only the carried state moves, placed by the
+      // stable-rank pass itself (x -> xq, v -> tmpV4, id -> tmpI, cell ->
+      // frank, then swap)
+      GridOps<T>::sort_state(st, gd_, P_.xm, nf_, P_.fcell, G_.count, G_.fstart, P_.frank,
+                             P_.fperm, G_.scan_tmp, P_.xq, P_.tmpV4, P_.v,
+                             reinterpret_cast<uint32_t *>(P_.tmpI),
+                             reinterpret_cast<const uint32_t *>(P_.id));
+      if (nf_ > 0) {
+        std::swap(P_.xm, P_.xq);
+        std::swap(P_.v, P_.tmpV4);
+        std::swap(P_.id, P_.tmpI);
+        std::swap(P_.fcell, P_.frank);
+      }
+      fgrid_dirty_ = false;
+      grid_built_ = true;
+      return;
+    }
     const uint32_t *pm = GridOps<T>::sort_group(st, gd_, P_.xm, nf_, P_.fcell, G_.count,
                                                 G_.fstart, P_.frank, P_.fperm, G_.scan_tmp);
     if (nf_ > 0) {
@@ -1156,15 +1186,21 @@ class SceneImpl final : public SceneBase {
     require_grid();
     hipStream_t st = res_->stream;
     const unsigned seq = ++seq_;
-    ctl_op(kCtlBeginSolve, tol, min_it, max_it, static_cast<int>(seq), nprde);
+    // iteration 1 taken by the setup needs its convergence step only where
+    // it could end the solve
+    const bool first = first_done && nf_ > 0 && max_it > 0;
+    const bool preset = first && first_iteration_cannot_stop(min_it, max_it);
+    ctl_op(preset ? kCtlBeginSolveAfterFirst : kCtlBeginSolve, tol, min_it, max_it,
+           static_cast<int>(seq), nprde);
     const SceneView<T> S = view();
     const NbrList L = nbr_list();
     Profiler &pf = res_->prof;
     tprof_.on = tune;
     // iterations enqueued so far (the iteration taken by the setup counts)
     int enq = 0;
-    if (first_done && nf_ > 0 && max_it > 0) {
-      FusedOps<T>::finish_first_iteration(st, S);
+    if (first) {
+      if (!preset)
+        FusedOps<T>::finish_first_iteration(st, S);
       enq = 1;
     }
     // one IISPH iteration (pressure acceleration, pressure, finalize) replayed
@@ -1436,14 +1472,16 @@ class SceneImpl final : public SceneBase {
   }
 
   // ------------------------------------------------------- batched stepping
-  // a scene the batched step can take: fused passes, no explicit viscosity,
-  // implicit viscosity by WKBB18 or none
+  // a scene the batched step can take: fused passes, implicit viscosity by
+  // WKBB18 (with or without gradient correction) or none (explicit viscosity
+  // and axial gravity included)
   bool batchable() const {
-    return desc.step_mode == 0 && !desc.explicit_viscosity &&
-           (desc.implicit_viscosity == 0 || desc.implicit_viscosity == 1) &&
-           !desc.axial_gravity && nf_ > 0 && desc.iisph_max_iterations > 0 &&
+    return desc.step_mode == 0 && desc.implicit_viscosity >= 0 &&
+           desc.implicit_viscosity <= 2 && nf_ > 0 && desc.iisph_max_iterations > 0 &&
            grid_blocks(nf_) <= kBatchLocalMask;
   }
+  // the fused setup takes three launches (FusedOps::setup)
+  bool split_setup() const { return desc.explicit_viscosity || desc.axial_gravity; }
 
   // the grid exists, so a batched step allocates nothing for this scene
   bool stepped_ready() const { return arena_ != nullptr && grid_mem_ != nullptr; }
@@ -1474,40 +1512,59 @@ class SceneImpl final : public SceneBase {
     // block maps: logical block b of the batch (scene order, then local
     // block) is dispatched as hardware block xcd(b), as in the single-scene
     // kernels, so each XCD takes a contiguous run of cell-ordered particles
-    std::vector<uint32_t> plist, tlist;
+    // (plist1 / plist3: the blocks of scenes with the one-launch and with
+    // the three-launch setup)
+    std::vector<uint32_t> plist, tlist, plist1, plist3;
     for (unsigned k = 0; k < ns; ++k) {
       const unsigned nb = grid_blocks(v[k]->nf_);
-      for (unsigned b = 0; b < nb; ++b)
+      for (unsigned b = 0; b < nb; ++b) {
         plist.push_back((k << kBatchLocalBits) | b);
+        (v[k]->split_setup() ? plist3 : plist1).push_back((k << kBatchLocalBits) | b);
+      }
       const unsigned nt = scan_tiles(v[k]->gd_.ncells);
       for (unsigned b = 0; b < nt; ++b)
         tlist.push_back((k << kBatchLocalBits) | b);
     }
-    // the scenes whose step ends with WKBB18 (their own table and block map)
+    // the scenes whose step ends with WKBB18 (their own table and block
+    // maps: all of them, and those without and with the gradient correction)
     std::vector<unsigned> vsc;
-    std::vector<uint32_t> vlist;
+    std::vector<uint32_t> vlist, vlist_n, vlist_g;
     for (unsigned k = 0; k < ns; ++k) {
       if (!v[k]->desc.implicit_viscosity)
         continue;
       const unsigned kv = static_cast<unsigned>(vsc.size());
       vsc.push_back(k);
       const unsigned nb = grid_blocks(v[k]->nf_);
-      for (unsigned b = 0; b < nb; ++b)
+      const bool gc = v[k]->desc.implicit_viscosity == 2;
+      for (unsigned b = 0; b < nb; ++b) {
         vlist.push_back((kv << kBatchLocalBits) | b);
+        (gc ? vlist_g : vlist_n).push_back((kv << kBatchLocalBits) | b);
+      }
     }
     const unsigned np = static_cast<unsigned>(plist.size());
     const unsigned nt = static_cast<unsigned>(tlist.size());
     const unsigned nvs = static_cast<unsigned>(vsc.size());
     const unsigned nvb = static_cast<unsigned>(vlist.size());
+    // the split maps only when some WKBB18 scene has the gradient correction
+    const unsigned nvg = static_cast<unsigned>(vlist_g.size());
+    const unsigned nvn = nvg > 0 ? static_cast<unsigned>(vlist_n.size()) : 0u;
+    // the setup's own block maps only when some scene has the three-launch
+    // setup (else pmap serves)
+    const unsigned np3 = static_cast<unsigned>(plist3.size());
+    const unsigned np1 = np3 > 0 ? static_cast<unsigned>(plist1.size()) : 0u;
     // pinned / device table layout: [sort | setup | solve] entries, the
-    // WKBB18 entries, pmap, tmap, vmap
+    // WKBB18 entries, pmap, tmap, vmap, the setup maps
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t tab_bytes = align(sizeof(BatchEntry<T>) * ns);
     const size_t off_visc = 3 * tab_bytes;
     const size_t off_pmap = off_visc + align(sizeof(BatchEntry<T>) * nvs);
     const size_t off_tmap = off_pmap + align(sizeof(uint32_t) * np);
     const size_t off_vmap = off_tmap + align(sizeof(uint32_t) * nt);
-    const size_t total = off_vmap + align(sizeof(uint32_t) * nvb);
+    const size_t off_s1map = off_vmap + align(sizeof(uint32_t) * nvb);
+    const size_t off_s3map = off_s1map + align(sizeof(uint32_t) * np1);
+    const size_t off_vnmap = off_s3map + align(sizeof(uint32_t) * np3);
+    const size_t off_vgmap = off_vnmap + align(sizeof(uint32_t) * nvn);
+    const size_t total = off_vgmap + align(sizeof(uint32_t) * nvg);
     if (fixed && total > fixed->bytes)
       throw TablesTooLarge{total};
     if (!fixed)
@@ -1530,10 +1587,23 @@ class SceneImpl final : public SceneBase {
       tmap[b] = tlist[b];
     for (unsigned b = 0; b < nvb; ++b)
       vmap[b] = vlist[xcd_of(b, nvb)];
+    uint32_t *s1map = reinterpret_cast<uint32_t *>(hb + off_s1map);
+    uint32_t *s3map = reinterpret_cast<uint32_t *>(hb + off_s3map);
+    for (unsigned b = 0; b < np1; ++b)
+      s1map[b] = plist1[xcd_of(b, np1)];
+    for (unsigned b = 0; b < np3; ++b)
+      s3map[b] = plist3[xcd_of(b, np3)];
+    uint32_t *vnmap = reinterpret_cast<uint32_t *>(hb + off_vnmap);
+    uint32_t *vgmap = reinterpret_cast<uint32_t *>(hb + off_vgmap);
+    for (unsigned b = 0; b < nvn; ++b)
+      vnmap[b] = vlist_n[xcd_of(b, nvn)];
+    for (unsigned b = 0; b < nvg; ++b)
+      vgmap[b] = vlist_g[xcd_of(b, nvg)];
     // the pointer swaps of build_fluid_grid() and of the setup pass are done
     // on the host while the three tables are filled
     std::vector<char> permute(ns, 0);
     std::vector<unsigned> seqs(ns);
+    bool any_first_step = false;  // a scene needs iteration 1's convergence step
     // the setup takes IISPH iteration 1 (batchable scenes have max_it > 0)
     const bool fold_first = !res->batch_tuner[sizeof(T) == 8 ? 1 : 0].current().first_split;
     for (unsigned k = 0; k < ns; ++k) {
@@ -1551,14 +1621,14 @@ class SceneImpl final : public SceneBase {
       E.x_dst = s.P_.xq;
       E.v_dst = s.P_.tmpV4;
       E.id_dst = reinterpret_cast<uint32_t *>(s.P_.tmpI);
-      E.cell_dst = s.P_.fperm;
+      E.cell_dst = s.P_.frank;  // the atomic ranks are consumed by then
       E.nblocks = grid_blocks(s.nf_);
       E.ntiles = scan_tiles(s.gd_.ncells);
       t_sort[k] = E;
       std::swap(s.P_.xm, s.P_.xq);
       std::swap(s.P_.v, s.P_.tmpV4);
       std::swap(s.P_.id, s.P_.tmpI);
-      std::swap(s.P_.fcell, s.P_.fperm);
+      std::swap(s.P_.fcell, s.P_.frank);
       s.fgrid_dirty_ = false;
       s.grid_built_ = true;
       s.steps_since_permute_ += 1;
@@ -1569,6 +1639,8 @@ class SceneImpl final : public SceneBase {
       E.S = s.view();
       E.S.first_iter = fold_first ? 1 : 0;
       E.vtmp = s.P_.tmpV4;
+      E.split_setup = s.split_setup() ? 1 : 0;
+      E.explicit_visc = s.desc.explicit_viscosity ? 1 : 0;
       t_setup[k] = E;
       std::swap(s.P_.v, s.P_.tmpV4);
       E.S = s.view();
@@ -1579,6 +1651,9 @@ class SceneImpl final : public SceneBase {
       E.max_it = s.desc.iisph_max_iterations;
       E.nprde = s.nf_;
       E.seq = seqs[k];
+      E.first_preset =
+          fold_first && first_iteration_cannot_stop(E.min_it, E.max_it) ? 1 : 0;
+      any_first_step = any_first_step || !E.first_preset;
       E.viscous = s.desc.implicit_viscosity ? 1 : 0;
       t_solve[k] = E;
     }
@@ -1612,6 +1687,22 @@ class SceneImpl final : public SceneBase {
     const auto *d_tmap = reinterpret_cast<const uint32_t *>(db + off_tmap);
     const auto *d_visc = reinterpret_cast<const BatchEntry<T> *>(db + off_visc);
     const auto *d_vmap = reinterpret_cast<const uint32_t *>(db + off_vmap);
+    const auto *d_s1map = reinterpret_cast<const uint32_t *>(db + off_s1map);
+    const auto *d_s3map = reinterpret_cast<const uint32_t *>(db + off_s3map);
+    BatchMaps vm;
+    vm.all = reinterpret_cast<const uint32_t *>(db + off_vmap);
+    vm.n_all = nvb;
+    if (nvg > 0) {
+      vm.plain = reinterpret_cast<const uint32_t *>(db + off_vnmap);
+      vm.n_plain = nvn;
+      vm.gc = reinterpret_cast<const uint32_t *>(db + off_vgmap);
+      vm.n_gc = nvg;
+    } else {
+      vm.plain = vm.all;
+      vm.n_plain = nvb;
+      vm.gc = nullptr;
+      vm.n_gc = 0;
+    }
 
     int r = pf.begin(st, kStageSort);
     GridOps<T>::sort_batch(st, d_sort, d_pmap, np, d_tmap, nt, ns);
@@ -1628,13 +1719,19 @@ class SceneImpl final : public SceneBase {
     tp.on = tune;
     r = pf.begin(st, kStageSetup);
     int q = tp.begin(st, kStageSetup);
-    FusedOps<T>::setup_batch(st, d_setup, d_pmap, np, sc.setup_lds);
+    if (np3 == 0) {
+      FusedOps<T>::setup_batch(st, d_setup, d_pmap, np, sc.setup_lds);
+    } else {
+      FusedOps<T>::setup_batch(st, d_setup, d_s1map, np1, sc.setup_lds);
+      FusedOps<T>::setup_split_batch(st, d_setup, d_s3map, np3, sc.setup_lds);
+    }
     tp.end(st, q);
     pf.end(st, r);
     hipLaunchKernelGGL(k_begin_solve_b<T>, dim3(grid_blocks(ns, 64)), dim3(64), 0, st, d_solve,
                        ns);
-    // the convergence step of iteration 1, which the setup has taken
-    if (fold_first)
+    // the convergence step of iteration 1, which the setup has taken, for
+    // the scenes whose solve it could end
+    if (fold_first && any_first_step)
       FusedOps<T>::finish_first_iteration_batch(st, d_solve, ns);
 
     // IISPH iterations of all scenes; a scene whose solve is done exits its
@@ -1751,8 +1848,13 @@ class SceneImpl final : public SceneBase {
       r = pf.begin(st, kStageGravityIntegrateV);
       SphOps<T>::integrate_v_batch(st, d_visc, d_vmap, nvb, true);
       pf.end(st, r);
+      if (nvg > 0) {  // the step's gradient correction (x and rho are final)
+        r = pf.begin(st, kStageCorrection);
+        Wkbb18Ops<T>::correction_batch(st, d_visc, vm);
+        pf.end(st, r);
+      }
       r = pf.begin(st, kStageWkbb18DiagSetup);
-      Wkbb18Ops<T>::diagonal_setup_batch(st, d_visc, d_vmap, nvb, nvs);
+      Wkbb18Ops<T>::diagonal_setup_batch(st, d_visc, vm, nvs);
       pf.end(st, r);
       int cap = 1, pred = 1;
       for (unsigned kv = 0; kv < nvs; ++kv) {
@@ -1764,7 +1866,7 @@ class SceneImpl final : public SceneBase {
       auto cg_enqueue = [&](int n) {
         for (int j = 0; j < n; ++j) {
           const int q1 = pf.begin(st, kStagePcgSpmv, cenq + j);
-          Wkbb18Ops<T>::iteration_batch(st, d_visc, d_vmap, nvb, nvs, any_split);
+          Wkbb18Ops<T>::iteration_batch(st, d_visc, vm, nvs, any_split);
           pf.end(st, q1);
         }
         cenq += n;
@@ -1795,7 +1897,7 @@ class SceneImpl final : public SceneBase {
           break;
       }
       r = pf.begin(st, kStagePcgVector);
-      Wkbb18Ops<T>::apply_batch(st, d_visc, d_vmap, nvb);
+      Wkbb18Ops<T>::apply_batch(st, d_visc, vm);
       pf.end(st, r);
       r = pf.begin(st, kStageGravityIntegrateV);
       SphOps<T>::integrate_v_batch(st, d_visc, d_vmap, nvb, false);
@@ -2248,8 +2350,8 @@ int step_scenes_noalloc(const std::vector<SceneBase *> &scenes, int k, CtxResour
     else
       throw std::invalid_argument(
           "a listed scene cannot be stepped without allocating: it needs the fused IISPH "
-          "step without viscosity or axial gravity, and its grid built (by a step, or by "
-          "neighborhood.update)");
+          "step (WKBB18, gradient-corrected WKBB18 or no implicit viscosity; not pt16), "
+          "and its grid built (by a step, or by neighborhood.update)");
   }
   if (!bf.empty() && !bd.empty())
     throw std::invalid_argument("FP32 and FP64 scenes in one call share the one table buffer");

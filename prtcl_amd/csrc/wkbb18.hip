@@ -166,9 +166,9 @@ __global__ __launch_bounds__(kBlock) void k_corr_identity(SceneView<T> S) {
 // L_i = pinv(sum_f (m_j / rho_i) gradW_ij (x_j - x_i)^T), exact walk in the
 // reference's FP32 operation order, or over the step's neighbour list
 template <class T, bool kList>
-__global__ __launch_bounds__(kBlock) void k_corr(SceneView<T> S, NbrList L) {
+__device__ __forceinline__ void corr_body(const SceneView<T> &S, const NbrList &L,
+                                          unsigned lb) {
 #pragma clang fp contract(off)
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -200,6 +200,11 @@ __global__ __launch_bounds__(kBlock) void k_corr(SceneView<T> S, NbrList L) {
                });
   }
   pinv3(M, S.gc + 9 * static_cast<size_t>(i));
+}
+
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_corr(SceneView<T> S, NbrList L) {
+  corr_body<T, kList>(S, L, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // L_i g with Eigen's fixed-size product order per row: l0 g0 + (l1 g1 + l2 g2)
@@ -844,12 +849,21 @@ __device__ __forceinline__ NbrList entry_list(const BatchEntry<T> &E) {
   return NbrList{E.nidx, E.ninfo, E.nblk};
 }
 
-template <class T>
+template <class T, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_diag_setup_b(const BatchEntry<T> *tab,
                                                          const uint32_t *map) {
   const uint32_t u = map[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  diag_setup_list_body<T, false>(E.S, entry_list(E), E.cg_part, u & kBatchLocalMask, E.nblocks);
+  diag_setup_list_body<T, kGC>(E.S, entry_list(E), E.cg_part, u & kBatchLocalMask, E.nblocks);
+}
+
+// the gradient correction of wkbb18_gc scenes over the list
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_corr_b(const BatchEntry<T> *tab,
+                                                   const uint32_t *map) {
+  const uint32_t u = map[blockIdx.x];
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
+  corr_body<T, true>(E.S, entry_list(E), u & kBatchLocalMask);
 }
 
 // one block per scene
@@ -859,12 +873,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_setup_fin_b(const BatchEntry<T> *
   cg_setup_fin_body(E.S.ctl, E.cg_part, E.nblocks, E.cg_tol, E.cg_max_k, E.cg_seq);
 }
 
-template <class T>
+template <class T, bool kGC>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_b(const BatchEntry<T> *tab,
                                                       const uint32_t *map) {
   const uint32_t u = map[blockIdx.x];
   const BatchEntry<T> &E = tab[u >> kBatchLocalBits];
-  cg_spmv_body<T, false, false>(E.S, entry_list(E), E.cg_part, u & kBatchLocalMask, E.nblocks);
+  cg_spmv_body<T, false, kGC>(E.S, entry_list(E), E.cg_part, u & kBatchLocalMask, E.nblocks);
 }
 
 template <class T>
@@ -1046,35 +1060,47 @@ void Wkbb18Ops<T>::apply(hipStream_t st, const SceneView<T> &S) {
 }
 
 template <class T>
+void Wkbb18Ops<T>::correction_batch(hipStream_t st, const BatchEntry<T> *tab,
+                                    const BatchMaps &m) {
+  if (m.n_gc > 0)
+    hipLaunchKernelGGL(k_corr_b<T>, dim3(m.n_gc), dim3(kBlock), 0, st, tab, m.gc);
+}
+
+template <class T>
 void Wkbb18Ops<T>::diagonal_setup_batch(hipStream_t st, const BatchEntry<T> *tab,
-                                        const uint32_t *map, unsigned nblocks, unsigned nscenes) {
+                                        const BatchMaps &m, unsigned nscenes) {
   if (nscenes == 0)
     return;
-  if (nblocks > 0)
-    hipLaunchKernelGGL(k_diag_setup_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+  if (m.n_plain > 0)
+    hipLaunchKernelGGL((k_diag_setup_b<T, false>), dim3(m.n_plain), dim3(kBlock), 0, st, tab,
+                       m.plain);
+  if (m.n_gc > 0)
+    hipLaunchKernelGGL((k_diag_setup_b<T, true>), dim3(m.n_gc), dim3(kBlock), 0, st, tab, m.gc);
   hipLaunchKernelGGL(k_cg_setup_fin_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab);
 }
 
 template <class T>
-void Wkbb18Ops<T>::iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
-                                   unsigned nblocks, unsigned nscenes, bool any_split) {
-  if (nblocks == 0)
+void Wkbb18Ops<T>::iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const BatchMaps &m,
+                                   unsigned nscenes, bool any_split) {
+  if (m.n_all == 0)
     return;
-  const dim3 gr(nblocks), bl(kBlock);
-  hipLaunchKernelGGL(k_cg_spmv_b<T>, gr, bl, 0, st, tab, map);
+  const dim3 gr(m.n_all), bl(kBlock);
+  if (m.n_plain > 0)
+    hipLaunchKernelGGL((k_cg_spmv_b<T, false>), dim3(m.n_plain), bl, 0, st, tab, m.plain);
+  if (m.n_gc > 0)
+    hipLaunchKernelGGL((k_cg_spmv_b<T, true>), dim3(m.n_gc), bl, 0, st, tab, m.gc);
   if (any_split)
     hipLaunchKernelGGL(k_cg_alpha_b<T>, dim3(nscenes), bl, 0, st, tab);
-  hipLaunchKernelGGL(k_cg_update_b<T>, gr, bl, 0, st, tab, map);
+  hipLaunchKernelGGL(k_cg_update_b<T>, gr, bl, 0, st, tab, m.all);
   if (any_split)
     hipLaunchKernelGGL(k_cg_beta_b<T>, dim3(nscenes), bl, 0, st, tab);
-  hipLaunchKernelGGL(k_cg_direction_b<T>, gr, bl, 0, st, tab, map);
+  hipLaunchKernelGGL(k_cg_direction_b<T>, gr, bl, 0, st, tab, m.all);
 }
 
 template <class T>
-void Wkbb18Ops<T>::apply_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
-                               unsigned nblocks) {
-  if (nblocks > 0)
-    hipLaunchKernelGGL(k_cg_apply_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+void Wkbb18Ops<T>::apply_batch(hipStream_t st, const BatchEntry<T> *tab, const BatchMaps &m) {
+  if (m.n_all > 0)
+    hipLaunchKernelGGL(k_cg_apply_b<T>, dim3(m.n_all), dim3(kBlock), 0, st, tab, m.all);
 }
 
 template struct Wkbb18Ops<float>;

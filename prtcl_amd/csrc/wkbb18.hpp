@@ -6,6 +6,13 @@
 
 namespace prtcl_amd {
 
+// block maps of the WKBB18 scenes of a batch: all of them, those without
+// and those with the gradient correction (wkbb18 / wkbb18_gc)
+struct BatchMaps {
+  const uint32_t *all, *plain, *gc;
+  unsigned n_all, n_plain, n_gc;
+};
+
 template <class T>
 struct Wkbb18Ops {
   // sp_i = (I - dt D_i)^-1 (and, with a list, the boundary matrix B_i and the
@@ -28,17 +35,17 @@ struct Wkbb18Ops {
                                 bool split);
   // vd = (x - v) H, a = vd / dt
   static void apply(hipStream_t st, const SceneView<T> &S);
-  // the same over every scene of a batch (WKBB18 without gradient
-  // correction, fused steps): map is the block map of the scenes' blocks,
+  // the same over every scene of a batch (fused steps, the list forms):
   // each table entry carries the scene's CG partials, tolerance, cap,
-  // sequence number and fold choice. any_split: some scene takes its alpha
-  // and beta from the one-block kernels
-  static void diagonal_setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
-                                   unsigned nblocks, unsigned nscenes);
-  static void iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
-                              unsigned nblocks, unsigned nscenes, bool any_split);
-  static void apply_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
-                          unsigned nblocks);
+  // sequence number and fold choice; the wkbb18_gc scenes' blocks take the
+  // gradient-corrected kernels. any_split: some scene takes its alpha and
+  // beta from the one-block kernels
+  static void correction_batch(hipStream_t st, const BatchEntry<T> *tab, const BatchMaps &m);
+  static void diagonal_setup_batch(hipStream_t st, const BatchEntry<T> *tab, const BatchMaps &m,
+                                   unsigned nscenes);
+  static void iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const BatchMaps &m,
+                              unsigned nscenes, bool any_split);
+  static void apply_batch(hipStream_t st, const BatchEntry<T> *tab, const BatchMaps &m);
   // correction.compute_gradient_correction: L_i = pinv(sum_f (m_j / rho_i)
   // gradW_ij (x_j - x_i)^T), over the grid (L null) or the step's list
   static void correction(hipStream_t st, const SceneView<T> &S, const NbrList *L);

@@ -76,22 +76,33 @@ def run_info(ctx, scs, k, together, kinds=None, **kw):
 
 
 def viscous_mix(seed=7):
-    # five config-4-like scenes, three of them with WKBB18 implicit viscosity
-    mix = small_mix(5, seed=seed)
+    # config-4-like scenes: three with WKBB18 implicit viscosity, one with
+    # WKBB18 and the gradient correction, one with explicit viscosity, two
+    # with axial gravity (one of them also WKBB18), so every setup form and
+    # both step ends are in one batch
+    mix = small_mix(7, seed=seed)
     for j, mu in ((1, 5.0), (3, 20.0), (4, 60.0)):
         mix[j].implicit_viscosity = 1
         mix[j].fluid_viscosity = mix[j].boundary_viscosity = mu
+    mix[2].implicit_viscosity = 2
+    mix[2].fluid_viscosity = mix[2].boundary_viscosity = 10.0
+    mix[5].explicit_viscosity = True
+    mix[5].fluid_viscosity = mix[5].boundary_viscosity = 0.05
+    for j in (4, 6):
+        mix[j].axial_gravity = True
+        mix[j].gravity_center = tuple(float(c) for c in mix[j].fluid_x.mean(axis=0))
     return mix
 
 
 @pytest.mark.parametrize("real_type", [0, 1], ids=["fp32", "fp64"])
 def test_batch_with_viscous_scenes(hip_ctx, real_type):
     """WKBB18 scenes are stepped inside the batch: after the batched IISPH
-    solve, their diagonal and CG set-up, the PCG iterations (each scene's FP64
-    CG control and its own convergence), apply and the integrations run over
-    all of them at once. Every scene matches its result alone bit for bit,
-    with the same IISPH and WKBB18 iteration counts (the last step crosses a
-    host-visible permutation)."""
+    solve, their gradient correction (wkbb18_gc scenes), diagonal and CG
+    set-up, the PCG iterations (each scene's FP64 CG control and its own
+    convergence), apply and the integrations run over all of them at once. Scenes with explicit viscosity or axial gravity take
+    their three setup launches over their own blocks. Every scene matches its
+    result alone bit for bit, with the same IISPH and WKBB18 iteration counts
+    (the last step crosses a host-visible permutation)."""
     mix = viscous_mix()
     alone, it_a = run_info(hip_ctx, mix, 9, together=False, real_type=real_type)
     batch, it_b = run_info(hip_ctx, mix, 9, together=True, real_type=real_type)
@@ -122,11 +133,11 @@ def test_batch_viscous_split_cg_equals_alone(hip_ctx):
 
 def test_batch_with_unbatchable_scenes(hip_ctx):
     """A list that mixes batchable scenes with scenes the batch does not take
-    (explicit viscosity, WKBB18 with gradient correction, per-procedure
-    kernels) and an FP64 scene: every scene still matches its result alone."""
+    (pt16, per-procedure kernels) and an FP64 scene: every scene still
+    matches its result alone."""
     mix = small_mix(5, seed=5)
-    mix[1].explicit_viscosity = True
-    mix[1].fluid_viscosity = mix[1].boundary_viscosity = 0.01
+    mix[1].implicit_viscosity = 3
+    mix[1].fluid_viscosity = mix[1].boundary_viscosity = 10.0
     mix[2].implicit_viscosity = 2
     mix[2].fluid_viscosity = mix[2].boundary_viscosity = 10.0
     kinds = [dict(), dict(), dict(), dict(real_type=1), dict(step_mode=1)]
