@@ -261,9 +261,8 @@ int prtcl_hip_step(prtcl_hip_ctx *ctx, const int32_t *scenes, int32_t n_scenes,
  * GPU, set up with the context) instead of context buffers grown on demand:
  * it calls no hipMalloc / hipFree, captures no graph and times nothing, and
  * its results equal prtcl_hip_step's bit for bit. Every listed scene must be
- * resident, take the batched step (fused IISPH with WKBB18, gradient-
- * corrected WKBB18 or no implicit viscosity; not pt16) and have its grid
- * built (by a step or neighborhood.update); all FP32 or all FP64. Returns
+ * resident, take the batched step (the fused step, step mode 0) and have its
+ * grid built (by a step or neighborhood.update); all FP32 or all FP64. Returns
  * PRTCL_HIP_INVALID_ARGUMENT otherwise, and when
  * the tables exceed 4 MiB (about 2,000 scenes of 10,000 particles). Calls on
  * one GPU share the array: a call copies its tables in only after the last

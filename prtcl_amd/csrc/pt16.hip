@@ -72,9 +72,9 @@ __device__ __forceinline__ void fluid_pairs_list(const SceneView<T> &S, const Nb
 // ---- pt16.setup; with kList (fused steps) one list pass for the fluid sums
 // that also writes the step's pair coefficients m_j W_ij
 template <class T, bool kList>
-__global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, NbrList L, T xi) {
+__device__ __forceinline__ void pt16_setup_body(const SceneView<T> &S, const NbrList &L, T xi,
+                                                unsigned lb) {
 #pragma clang fp contract(off)
-  const unsigned lb = kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -153,13 +153,18 @@ __global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, NbrList L
   S.prho[i] = prho;
 }
 
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_setup(SceneView<T> S, NbrList L, T xi) {
+  pt16_setup_body<T, kList>(S, L, xi, kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
 // ---- vorticity diffusion rhs; with kList the coefficients m_j W_ij of the
 // set-up are streamed and only w_j is gathered (the particle's own term is
 // zero)
 template <class T, bool kList>
-__global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs(SceneView<T> S, NbrList L, T xi) {
+__device__ __forceinline__ void pt16_omega_rhs_body(const SceneView<T> &S, const NbrList &L,
+                                                    T xi, unsigned lb) {
 #pragma clang fp contract(off)
-  const unsigned lb = kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -191,11 +196,16 @@ __global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs(SceneView<T> S, NbrLi
   S.orhs[i] = make4<vec4_t<T>>(a0 * xi, a1 * xi, a2 * xi, T(0));
 }
 
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs(SceneView<T> S, NbrList L, T xi) {
+  pt16_omega_rhs_body<T, kList>(S, L, xi, kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
 // ---- velocity reconstruction rhs
 template <class T, bool kList>
-__global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S, NbrList L) {
+__device__ __forceinline__ void pt16_vrec_rhs_body(const SceneView<T> &S, const NbrList &L,
+                                                   unsigned lb) {
 #pragma clang fp contract(off)
-  const unsigned lb = kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
@@ -228,11 +238,16 @@ __global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S, NbrLis
   S.vrhs[i] = make4<vec4_t<T>>(a[0], a[1], a[2], T(0));
 }
 
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs(SceneView<T> S, NbrList L) {
+  pt16_vrec_rhs_body<T, kList>(S, L, kList ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
 // ---- spin: G += cross_product_matrix(0.5 w) (eigen.hpp:330-339)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_spin(SceneView<T> S) {
+__device__ __forceinline__ void pt16_spin_body(const SceneView<T> &S, unsigned lb) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const vec4_t<T> w = S.om[i];
@@ -243,6 +258,11 @@ __global__ __launch_bounds__(kBlock) void k_pt16_spin(SceneView<T> S) {
 #pragma unroll
   for (int k = 0; k < 9; ++k)
     G[k] += C[k];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_spin(SceneView<T> S) {
+  pt16_spin_body(S, blockIdx.x);
 }
 
 // ---- scalar PCG (one component of one system)
@@ -322,10 +342,10 @@ __device__ __forceinline__ T product_s_list(const SceneView<T> &S, const NbrList
 
 // x = guess, r = A x - b, y = P r, p = -y; partials b.b, r.r, r.y
 template <class T, bool kList>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, NbrList L, int which,
-                                                          int dim, double *part) {
+__device__ __forceinline__ void pt16_cg_setup_body(const SceneView<T> &S, const NbrList &L,
+                                                   int which, int dim, double *part,
+                                                   unsigned lb, unsigned nb) {
 #pragma clang fp contract(off)
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < S.nf) {
@@ -354,17 +374,24 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, NbrLis
     acc[1] = static_cast<double>(r * r);
     acc[2] = static_cast<double>(r * y);
   }
-  store_partials<3>(part, gridDim.x, acc, lb);
+  store_partials<3>(part, nb, acc, lb);
+}
+
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup(SceneView<T> S, NbrList L, int which,
+                                                          int dim, double *part) {
+  pt16_cg_setup_body<T, kList>(S, L, which, dim, part, xcd_block(blockIdx.x, gridDim.x),
+                               gridDim.x);
 }
 
 // q = A p, partial p.q
 template <class T, bool kList>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv(SceneView<T> S, NbrList L, int which,
-                                                         double *part) {
+__device__ __forceinline__ void pt16_cg_spmv_body(const SceneView<T> &S, const NbrList &L,
+                                                  int which, double *part, unsigned lb,
+                                                  unsigned nb) {
 #pragma clang fp contract(off)
   if (S.ctl->cg_done)
     return;
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[1] = {0.0};
   if (i < S.nf) {
@@ -373,14 +400,21 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv(SceneView<T> S, NbrList
     S.scq[i] = q;
     acc[0] = static_cast<double>(S.scp[i] * q);
   }
-  store_partials<1>(part, gridDim.x, acc, lb);
+  store_partials<1>(part, nb, acc, lb);
+}
+
+template <class T, bool kList>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv(SceneView<T> S, NbrList L, int which,
+                                                         double *part) {
+  pt16_cg_spmv_body<T, kList>(S, L, which, part, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // x += alpha p, r += alpha q, y = P r; partials r.r, r.y
 template <class T>
-__device__ __forceinline__ void pt16_update_body(const SceneView<T> &S, T alpha, double *part) {
+__device__ __forceinline__ void pt16_update_body(const SceneView<T> &S, T alpha, double *part,
+                                                 unsigned lb, unsigned nb) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   double acc[2] = {0.0, 0.0};
   if (i < S.nf) {
     const T x = S.scx[i] + alpha * S.scp[i];
@@ -392,21 +426,21 @@ __device__ __forceinline__ void pt16_update_body(const SceneView<T> &S, T alpha,
     acc[0] = static_cast<double>(r * r);
     acc[1] = static_cast<double>(r * y);
   }
-  store_partials<2>(part, gridDim.x, acc);
+  store_partials<2>(part, nb, acc, lb);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pt16_cg_update(SceneView<T> S, double *part) {
   if (S.ctl->cg_done)
     return;
-  pt16_update_body(S, static_cast<T>(S.ctl->cg_alpha), part);
+  pt16_update_body(S, static_cast<T>(S.ctl->cg_alpha), part, blockIdx.x, gridDim.x);
 }
 
 // p = -y + beta p
 template <class T>
-__device__ __forceinline__ void pt16_direction_body(const SceneView<T> &S, T beta) {
+__device__ __forceinline__ void pt16_direction_body(const SceneView<T> &S, T beta, unsigned lb) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   S.scp[i] = -S.scy[i] + beta * S.scp[i];
@@ -416,7 +450,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_pt16_cg_direction(SceneView<T> S) {
   if (S.ctl->cg_done)
     return;
-  pt16_direction_body(S, static_cast<T>(S.ctl->cg_beta));
+  pt16_direction_body(S, static_cast<T>(S.ctl->cg_beta), blockIdx.x);
 }
 
 // the iteration without its two one-block kernels (cg_alpha_in_block,
@@ -430,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_update_alpha(SceneView<T> S,
   double alpha;
   if (!cg_alpha_in_block(S.ctl, pq_part, gridDim.x, &alpha))
     return;
-  pt16_update_body(S, static_cast<T>(alpha), part);
+  pt16_update_body(S, static_cast<T>(alpha), part, blockIdx.x, gridDim.x);
 }
 
 template <class T>
@@ -441,16 +475,17 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_direction_beta(SceneView<T> 
   double beta;
   if (!cg_beta_in_block(S.ctl, part, gridDim.x, &beta))
     return;
-  pt16_direction_body(S, static_cast<T>(beta));
+  pt16_direction_body(S, static_cast<T>(beta), blockIdx.x);
 }
 
 // apply (pt16.hpp:491-502, 700-717): component dim of the vorticity (which 0)
 // or the velocity (which 1) = x unit_step_l(0, (t - t_birth) - dt_fade) +
 // old unit_step_r(0, dt_fade - (t - t_birth))
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pt16_cg_apply(SceneView<T> S, int which, int dim) {
+__device__ __forceinline__ void pt16_cg_apply_body(const SceneView<T> &S, int which, int dim,
+                                                   unsigned lb) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = static_cast<int>(lb * kBlock + threadIdx.x);
   if (i >= S.nf)
     return;
   const T tb = S.v[i].w;
@@ -461,7 +496,209 @@ __global__ __launch_bounds__(kBlock) void k_pt16_cg_apply(SceneView<T> S, int wh
   *f = S.scx[i] * on + *f * keep;
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_apply(SceneView<T> S, int which, int dim) {
+  pt16_cg_apply_body(S, which, dim, blockIdx.x);
+}
+
+// ---- batched forms (prtcl_hip_step over several scenes, list forms): the
+// bodies above over the blocks of every pt16 scene of a table (map: block
+// map, entry of scene k in bits 20.., its local block below), each block with
+// its scene's view, list, CG partials and block count. Solve s of a step
+// (vorticity components 0-2, velocity components 3-5) runs under sequence
+// number cg_seq + s with the tolerance and cap of its system.
+template <class T>
+__device__ __forceinline__ NbrList pt16_list(const BatchEntry<T> &E) {
+  return NbrList{E.nidx, E.ninfo, E.nblk};
+}
+
+#define PT16_BATCH_ENTRY                              \
+  const uint32_t u = map[blockIdx.x];                 \
+  const BatchEntry<T> &E = tab[u >> kBatchLocalBits]; \
+  const unsigned lb = u & kBatchLocalMask
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_setup_b(const BatchEntry<T> *tab,
+                                                         const uint32_t *map) {
+  PT16_BATCH_ENTRY;
+  pt16_setup_body<T, true>(E.S, pt16_list(E), static_cast<T>(E.pt16_xi), lb);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_omega_rhs_b(const BatchEntry<T> *tab,
+                                                             const uint32_t *map) {
+  PT16_BATCH_ENTRY;
+  pt16_omega_rhs_body<T, true>(E.S, pt16_list(E), static_cast<T>(E.pt16_xi), lb);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_spin_b(const BatchEntry<T> *tab,
+                                                        const uint32_t *map) {
+  PT16_BATCH_ENTRY;
+  pt16_spin_body(E.S, lb);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_vrec_rhs_b(const BatchEntry<T> *tab,
+                                                            const uint32_t *map) {
+  PT16_BATCH_ENTRY;
+  pt16_vrec_rhs_body<T, true>(E.S, pt16_list(E), lb);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup_b(const BatchEntry<T> *tab,
+                                                            const uint32_t *map, int which,
+                                                            int dim) {
+  PT16_BATCH_ENTRY;
+  pt16_cg_setup_body<T, true>(E.S, pt16_list(E), which, dim, E.cg_part, lb, E.nblocks);
+}
+
+// one block per scene
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_setup_fin_b(const BatchEntry<T> *tab,
+                                                                int which, int solve) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  cg_setup_fin_body(E.S.ctl, E.cg_part, E.nblocks, E.pt16_tol[which], E.pt16_max_k[which],
+                    E.cg_seq + static_cast<unsigned>(solve));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_spmv_b(const BatchEntry<T> *tab,
+                                                           const uint32_t *map, int which) {
+  PT16_BATCH_ENTRY;
+  pt16_cg_spmv_body<T, true>(E.S, pt16_list(E), which, E.cg_part, lb, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_alpha_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  if (!E.cg_fold)
+    cg_alpha_body(E.S.ctl, E.cg_part, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_beta_b(const BatchEntry<T> *tab) {
+  const BatchEntry<T> &E = tab[blockIdx.x];
+  if (!E.cg_fold)
+    cg_beta_body(E.S.ctl, E.cg_part, E.nblocks);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_update_b(const BatchEntry<T> *tab,
+                                                             const uint32_t *map) {
+  PT16_BATCH_ENTRY;
+  const unsigned nb = E.nblocks;
+  Ctl *c = E.S.ctl;
+  if (E.cg_fold) {  // p.q partials at [0, nb), r.r and r.y at [nb, 3 nb)
+    if (cg_done_for_block(c))
+      return;
+    double alpha;
+    if (!cg_alpha_in_block(c, E.cg_part, nb, &alpha, lb))
+      return;
+    pt16_update_body(E.S, static_cast<T>(alpha), E.cg_part + nb, lb, nb);
+  } else {
+    if (c->cg_done)
+      return;
+    pt16_update_body(E.S, static_cast<T>(c->cg_alpha), E.cg_part, lb, nb);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_direction_b(const BatchEntry<T> *tab,
+                                                                const uint32_t *map) {
+  PT16_BATCH_ENTRY;
+  const unsigned nb = E.nblocks;
+  Ctl *c = E.S.ctl;
+  if (E.cg_fold) {
+    if (cg_done_for_block(c))
+      return;
+    double beta;
+    if (!cg_beta_in_block(c, E.cg_part + nb, nb, &beta, lb))
+      return;
+    pt16_direction_body(E.S, static_cast<T>(beta), lb);
+  } else {
+    if (c->cg_done)
+      return;
+    pt16_direction_body(E.S, static_cast<T>(c->cg_beta), lb);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pt16_cg_apply_b(const BatchEntry<T> *tab,
+                                                            const uint32_t *map, int which,
+                                                            int dim) {
+  PT16_BATCH_ENTRY;
+  pt16_cg_apply_body(E.S, which, dim, lb);
+}
+
+#undef PT16_BATCH_ENTRY
+
 }  // namespace
+
+template <class T>
+void Pt16Ops<T>::setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                             unsigned nblocks) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_pt16_setup_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+}
+
+template <class T>
+void Pt16Ops<T>::omega_rhs_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                 unsigned nblocks) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_pt16_omega_rhs_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+}
+
+template <class T>
+void Pt16Ops<T>::spin_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                            unsigned nblocks) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_pt16_spin_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+}
+
+template <class T>
+void Pt16Ops<T>::vrec_rhs_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                unsigned nblocks) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_pt16_vrec_rhs_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                unsigned nblocks, unsigned nscenes, int which, int dim,
+                                int solve) {
+  if (nscenes == 0)
+    return;
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_pt16_cg_setup_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map,
+                       which, dim);
+  hipLaunchKernelGGL(k_pt16_cg_setup_fin_b<T>, dim3(nscenes), dim3(kBlock), 0, st, tab, which,
+                     solve);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_iteration_batch(hipStream_t st, const BatchEntry<T> *tab,
+                                    const uint32_t *map, unsigned nblocks, unsigned nscenes,
+                                    int which, bool any_split) {
+  if (nblocks == 0)
+    return;
+  const dim3 gr(nblocks), bl(kBlock);
+  hipLaunchKernelGGL(k_pt16_cg_spmv_b<T>, gr, bl, 0, st, tab, map, which);
+  if (any_split)
+    hipLaunchKernelGGL(k_pt16_cg_alpha_b<T>, dim3(nscenes), bl, 0, st, tab);
+  hipLaunchKernelGGL(k_pt16_cg_update_b<T>, gr, bl, 0, st, tab, map);
+  if (any_split)
+    hipLaunchKernelGGL(k_pt16_cg_beta_b<T>, dim3(nscenes), bl, 0, st, tab);
+  hipLaunchKernelGGL(k_pt16_cg_direction_b<T>, gr, bl, 0, st, tab, map);
+}
+
+template <class T>
+void Pt16Ops<T>::cg_apply_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                unsigned nblocks, int which, int dim) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_pt16_cg_apply_b<T>, dim3(nblocks), dim3(kBlock), 0, st, tab, map, which,
+                       dim);
+}
 
 template <class T>
 void Pt16Ops<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList *L, T xi) {

@@ -36,6 +36,26 @@ struct Pt16Ops {
   static void cg_iteration(hipStream_t st, const SceneView<T> &S, const NbrList *L, int which,
                            double *part, bool split);
   static void cg_apply(hipStream_t st, const SceneView<T> &S, int which, int dim);
+  // the same over every pt16 scene of a batch (list forms; map: block map of
+  // the scenes' blocks); each table entry carries the scene's xi, CG
+  // partials, the tolerance and cap of both systems, the sequence number of
+  // the step's first solve (solve s = 0..5 runs under cg_seq + s) and the
+  // fold choice. any_split: some scene takes alpha and beta from the
+  // one-block kernels
+  static void setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                          unsigned nblocks);
+  static void omega_rhs_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                              unsigned nblocks);
+  static void spin_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                         unsigned nblocks);
+  static void vrec_rhs_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                             unsigned nblocks);
+  static void cg_setup_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                             unsigned nblocks, unsigned nscenes, int which, int dim, int solve);
+  static void cg_iteration_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                                 unsigned nblocks, unsigned nscenes, int which, bool any_split);
+  static void cg_apply_batch(hipStream_t st, const BatchEntry<T> *tab, const uint32_t *map,
+                             unsigned nblocks, int which, int dim);
 };
 
 }  // namespace prtcl_amd

@@ -247,6 +247,11 @@ struct BatchEntry {
   int cg_max_k;
   unsigned cg_seq;
   int cg_fold;
+  // pt16: strain-rate viscosity and the tolerance and cap of the vorticity
+  // diffusion (0) and velocity reconstruction (1) systems
+  double pt16_xi;
+  double pt16_tol[2];
+  int pt16_max_k[2];
 };
 // the largest CG launch whose vector kernels derive alpha and beta
 // themselves: every block sums all nb partials of the kernel before it

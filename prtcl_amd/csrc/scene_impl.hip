@@ -1472,12 +1472,11 @@ class SceneImpl final : public SceneBase {
   }
 
   // ------------------------------------------------------- batched stepping
-  // a scene the batched step can take: fused passes, implicit viscosity by
-  // WKBB18 (with or without gradient correction) or none (explicit viscosity
-  // and axial gravity included)
+  // a scene the batched step can take: fused passes (every viscosity and
+  // gravity form of the step)
   bool batchable() const {
     return desc.step_mode == 0 && desc.implicit_viscosity >= 0 &&
-           desc.implicit_viscosity <= 2 && nf_ > 0 && desc.iisph_max_iterations > 0 &&
+           desc.implicit_viscosity <= 3 && nf_ > 0 && desc.iisph_max_iterations > 0 &&
            grid_blocks(nf_) <= kBatchLocalMask;
   }
   // the fused setup takes three launches (FusedOps::setup)
@@ -1527,11 +1526,19 @@ class SceneImpl final : public SceneBase {
     }
     // the scenes whose step ends with WKBB18 (their own table and block
     // maps: all of them, and those without and with the gradient correction)
-    std::vector<unsigned> vsc;
-    std::vector<uint32_t> vlist, vlist_n, vlist_g;
+    std::vector<unsigned> vsc, psc;  // WKBB18 scenes, pt16 scenes
+    std::vector<uint32_t> vlist, vlist_n, vlist_g, ptlist;
     for (unsigned k = 0; k < ns; ++k) {
       if (!v[k]->desc.implicit_viscosity)
         continue;
+      if (v[k]->desc.implicit_viscosity == 3) {
+        const unsigned kp = static_cast<unsigned>(psc.size());
+        psc.push_back(k);
+        const unsigned nb = grid_blocks(v[k]->nf_);
+        for (unsigned b = 0; b < nb; ++b)
+          ptlist.push_back((kp << kBatchLocalBits) | b);
+        continue;
+      }
       const unsigned kv = static_cast<unsigned>(vsc.size());
       vsc.push_back(k);
       const unsigned nb = grid_blocks(v[k]->nf_);
@@ -1548,6 +1555,8 @@ class SceneImpl final : public SceneBase {
     // the split maps only when some WKBB18 scene has the gradient correction
     const unsigned nvg = static_cast<unsigned>(vlist_g.size());
     const unsigned nvn = nvg > 0 ? static_cast<unsigned>(vlist_n.size()) : 0u;
+    const unsigned nps = static_cast<unsigned>(psc.size());
+    const unsigned npb = static_cast<unsigned>(ptlist.size());
     // the setup's own block maps only when some scene has the three-launch
     // setup (else pmap serves)
     const unsigned np3 = static_cast<unsigned>(plist3.size());
@@ -1564,7 +1573,9 @@ class SceneImpl final : public SceneBase {
     const size_t off_s3map = off_s1map + align(sizeof(uint32_t) * np1);
     const size_t off_vnmap = off_s3map + align(sizeof(uint32_t) * np3);
     const size_t off_vgmap = off_vnmap + align(sizeof(uint32_t) * nvn);
-    const size_t total = off_vgmap + align(sizeof(uint32_t) * nvg);
+    const size_t off_pt = off_vgmap + align(sizeof(uint32_t) * nvg);
+    const size_t off_ptmap = off_pt + align(sizeof(BatchEntry<T>) * nps);
+    const size_t total = off_ptmap + align(sizeof(uint32_t) * npb);
     if (fixed && total > fixed->bytes)
       throw TablesTooLarge{total};
     if (!fixed)
@@ -1599,6 +1610,10 @@ class SceneImpl final : public SceneBase {
       vnmap[b] = vlist_n[xcd_of(b, nvn)];
     for (unsigned b = 0; b < nvg; ++b)
       vgmap[b] = vlist_g[xcd_of(b, nvg)];
+    auto *t_pt = reinterpret_cast<BatchEntry<T> *>(hb + off_pt);
+    uint32_t *ptmap = reinterpret_cast<uint32_t *>(hb + off_ptmap);
+    for (unsigned b = 0; b < npb; ++b)
+      ptmap[b] = ptlist[xcd_of(b, npb)];
     // the pointer swaps of build_fluid_grid() and of the setup pass are done
     // on the host while the three tables are filled
     std::vector<char> permute(ns, 0);
@@ -1674,6 +1689,27 @@ class SceneImpl final : public SceneBase {
       any_split = any_split || !E.cg_fold;
       t_visc[kv] = E;
     }
+    // pt16 after the IISPH solve: six scalar CG solves (vorticity and
+    // velocity components) under six consecutive sequence numbers, as
+    // run_pt16 takes them
+    bool pt_any_split = false;
+    std::vector<unsigned> pt_seq0(nps);
+    for (unsigned kp = 0; kp < nps; ++kp) {
+      SceneImpl &s = *v[psc[kp]];
+      BatchEntry<T> E = t_solve[psc[kp]];
+      E.cg_part = s.P_.cg_part;
+      E.pt16_xi = s.pt16_xi_;
+      E.pt16_tol[0] = s.desc.pt16_vorticity_maximum_error;
+      E.pt16_max_k[0] = std::max(0, static_cast<int>(s.desc.pt16_vorticity_maximum_iterations));
+      E.pt16_tol[1] = s.desc.pt16_velocity_maximum_error;
+      E.pt16_max_k[1] = std::max(0, static_cast<int>(s.desc.pt16_velocity_maximum_iterations));
+      pt_seq0[kp] = s.seq_ + 1;
+      s.seq_ += 6;
+      E.cg_seq = pt_seq0[kp];
+      E.cg_fold = !cg_split && E.nblocks <= kFusedCgBlocks ? 1 : 0;
+      pt_any_split = pt_any_split || !E.cg_fold;
+      t_pt[kp] = E;
+    }
     // a fixed device buffer may be shared with other streams: the copy into
     // it waits for the kernels of its last user
     if (fixed)
@@ -1689,6 +1725,8 @@ class SceneImpl final : public SceneBase {
     const auto *d_vmap = reinterpret_cast<const uint32_t *>(db + off_vmap);
     const auto *d_s1map = reinterpret_cast<const uint32_t *>(db + off_s1map);
     const auto *d_s3map = reinterpret_cast<const uint32_t *>(db + off_s3map);
+    const auto *d_pt = reinterpret_cast<const BatchEntry<T> *>(db + off_pt);
+    const auto *d_ptmap = reinterpret_cast<const uint32_t *>(db + off_ptmap);
     BatchMaps vm;
     vm.all = reinterpret_cast<const uint32_t *>(db + off_vmap);
     vm.n_all = nvb;
@@ -1843,6 +1881,50 @@ class SceneImpl final : public SceneBase {
     // diagonal and the CG set-up in one list walk, the PCG of every scene
     // until each reports done, apply, integrate v and x (the fused step's
     // order for implicit viscosity)
+    // The CG iterations of several scenes' solves (scene sc[k] under seq[k],
+    // at most cap iterations): the predicted count at once, then one queued
+    // ahead while the host polls each scene's progress word
+    auto cg_loop_batch = [&](const std::vector<unsigned> &sc, const std::vector<unsigned> &seq,
+                             int pred, int cap, const std::function<void(int)> &iteration) {
+      std::vector<Progress> out(sc.size());
+      int cenq = 0;
+      auto cg_enqueue = [&](int n) {
+        for (int j = 0; j < n; ++j)
+          iteration(cenq + j);
+        cenq += n;
+      };
+      cg_enqueue(std::min(pred, cap));
+      std::vector<char> fin(sc.size(), 0);
+      for (;;) {
+        const int ahead = cenq < cap ? 1 : 0;
+        if (ahead)
+          cg_enqueue(1);
+        bool all = true;
+        for (size_t k = 0; k < sc.size(); ++k) {
+          if (fin[k])
+            continue;
+          out[k] = v[sc[k]]->wait_progress(seq[k], cenq - ahead, 2);
+          if (out[k].done) {
+            fin[k] = 1;
+          } else if (!ahead) {
+            out[k] = v[sc[k]]->wait_progress(seq[k], cenq, 2);
+            fin[k] = 1;
+          } else {
+            all = false;
+          }
+        }
+        if (all)
+          break;
+      }
+      if (pf.on) {
+        PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+        int it = 0;
+        for (const Progress &p : out)
+          it = std::max(it, p.iters);
+        pf.resolve(it);
+      }
+      return out;
+    };
     std::vector<Progress> cgp(nvs);
     if (nvs > 0) {
       r = pf.begin(st, kStageGravityIntegrateV);
@@ -1862,40 +1944,11 @@ class SceneImpl final : public SceneBase {
         cap = std::max(cap, std::max(0, static_cast<int>(s.desc.wkbb18_maximum_iterations)) + 1);
         pred = std::max(pred, std::min(std::max(1, s.predicted_cg_), 16));
       }
-      int cenq = 0;
-      auto cg_enqueue = [&](int n) {
-        for (int j = 0; j < n; ++j) {
-          const int q1 = pf.begin(st, kStagePcgSpmv, cenq + j);
-          Wkbb18Ops<T>::iteration_batch(st, d_visc, vm, nvs, any_split);
-          pf.end(st, q1);
-        }
-        cenq += n;
-      };
-      // the predicted iterations at once, then one queued ahead while the
-      // host polls each scene's progress word
-      cg_enqueue(std::min(pred, cap));
-      std::vector<char> cg_fin(nvs, 0);
-      for (;;) {
-        const int ahead = cenq < cap ? 1 : 0;
-        if (ahead)
-          cg_enqueue(1);
-        bool all = true;
-        for (unsigned kv = 0; kv < nvs; ++kv) {
-          if (cg_fin[kv])
-            continue;
-          cgp[kv] = v[vsc[kv]]->wait_progress(cg_seqs[kv], cenq - ahead, 2);
-          if (cgp[kv].done) {
-            cg_fin[kv] = 1;
-          } else if (!ahead) {
-            cgp[kv] = v[vsc[kv]]->wait_progress(cg_seqs[kv], cenq, 2);
-            cg_fin[kv] = 1;
-          } else {
-            all = false;
-          }
-        }
-        if (all)
-          break;
-      }
+      cgp = cg_loop_batch(vsc, cg_seqs, pred, cap, [&](int tag) {
+        const int q1 = pf.begin(st, kStagePcgSpmv, tag);
+        Wkbb18Ops<T>::iteration_batch(st, d_visc, vm, nvs, any_split);
+        pf.end(st, q1);
+      });
       r = pf.begin(st, kStagePcgVector);
       Wkbb18Ops<T>::apply_batch(st, d_visc, vm);
       pf.end(st, r);
@@ -1905,13 +1958,59 @@ class SceneImpl final : public SceneBase {
       r = pf.begin(st, kStageIntegrateX);
       SphOps<T>::integrate_x_batch(st, d_visc, d_vmap, nvb, nvs);
       pf.end(st, r);
-      if (pf.on) {
-        PRTCL_HIP_CHECK(hipStreamSynchronize(st));
-        int it = 0;
-        for (const Progress &p : cgp)
-          it = std::max(it, p.iters);
-        pf.resolve(it);
+    }
+    // the pt16 scenes (the order of the single-scene step): a = the pressure
+    // acceleration, integrate v, set-up, vorticity diffusion (three solves),
+    // spin, velocity reconstruction (three solves, which write the
+    // velocity), integrate x
+    std::vector<Progress> ptp(nps);
+    std::vector<int> pt_last(nps * 2, 0);
+    if (nps > 0) {
+      r = pf.begin(st, kStageGravityIntegrateV);
+      SphOps<T>::integrate_v_batch(st, d_pt, d_ptmap, npb, true);
+      pf.end(st, r);
+      r = pf.begin(st, kStagePt16Setup);
+      Pt16Ops<T>::setup_batch(st, d_pt, d_ptmap, npb);
+      Pt16Ops<T>::omega_rhs_batch(st, d_pt, d_ptmap, npb);
+      pf.end(st, r);
+      for (int which = 0; which < 2; ++which) {
+        if (which == 1) {
+          Pt16Ops<T>::spin_batch(st, d_pt, d_ptmap, npb);
+          r = pf.begin(st, kStagePt16Setup);
+          Pt16Ops<T>::vrec_rhs_batch(st, d_pt, d_ptmap, npb);
+          pf.end(st, r);
+        }
+        int cap = 1, pred = 1;
+        for (unsigned kp = 0; kp < nps; ++kp) {
+          const SceneImpl &s = *v[psc[kp]];
+          const int mk = which == 0 ? s.desc.pt16_vorticity_maximum_iterations
+                                    : s.desc.pt16_velocity_maximum_iterations;
+          cap = std::max(cap, std::max(0, mk) + 1);
+          pred = std::max(pred, std::min(std::max(1, s.predicted_pt16_[which]), 16));
+        }
+        for (int dim = 0; dim < 3; ++dim) {
+          const int solve = 3 * which + dim;
+          std::vector<unsigned> seqs_s(nps);
+          for (unsigned kp = 0; kp < nps; ++kp)
+            seqs_s[kp] = pt_seq0[kp] + static_cast<unsigned>(solve);
+          r = pf.begin(st, kStagePcgVector);
+          Pt16Ops<T>::cg_setup_batch(st, d_pt, d_ptmap, npb, nps, which, dim, solve);
+          pf.end(st, r);
+          ptp = cg_loop_batch(psc, seqs_s, pred, cap, [&](int tag) {
+            const int q1 = pf.begin(st, kStagePcgSpmv, tag);
+            Pt16Ops<T>::cg_iteration_batch(st, d_pt, d_ptmap, npb, nps, which, pt_any_split);
+            pf.end(st, q1);
+          });
+          r = pf.begin(st, kStagePcgVector);
+          Pt16Ops<T>::cg_apply_batch(st, d_pt, d_ptmap, npb, which, dim);
+          pf.end(st, r);
+          for (unsigned kp = 0; kp < nps; ++kp)
+            pt_last[2 * kp + which] = ptp[kp].iters;
+        }
       }
+      r = pf.begin(st, kStageIntegrateX);
+      SphOps<T>::integrate_x_batch(st, d_pt, d_ptmap, npb, nps);
+      pf.end(st, r);
     }
     if (fixed)  // the last kernel reading the tables is enqueued
       PRTCL_HIP_CHECK(hipEventRecord(fixed->released, st));
@@ -1921,6 +2020,16 @@ class SceneImpl final : public SceneBase {
       SceneImpl &s = *v[vsc[kv]];
       s.predicted_cg_ = std::max(1, cgp[kv].iters);
       s.last_wkbb_ = cgp[kv].iters;
+    }
+    // as run_pt16 leaves them: the predictions from each system's last
+    // component, the velocity system's settings and last count
+    for (unsigned kp = 0; kp < nps; ++kp) {
+      SceneImpl &s = *v[psc[kp]];
+      s.predicted_pt16_[0] = std::max(1, pt_last[2 * kp]);
+      s.predicted_pt16_[1] = std::max(1, pt_last[2 * kp + 1]);
+      s.pt16_iters_ = pt_last[2 * kp + 1];
+      s.pt16_tol_ = s.desc.pt16_velocity_maximum_error;
+      s.pt16_max_k_ = s.desc.pt16_velocity_maximum_iterations;
     }
     int status = PRTCL_HIP_OK;
     for (unsigned k = 0; k < ns; ++k) {
@@ -2350,7 +2459,7 @@ int step_scenes_noalloc(const std::vector<SceneBase *> &scenes, int k, CtxResour
     else
       throw std::invalid_argument(
           "a listed scene cannot be stepped without allocating: it needs the fused IISPH "
-          "step (WKBB18, gradient-corrected WKBB18 or no implicit viscosity; not pt16), "
+          "step (step mode 0), "
           "and its grid built (by a step, or by neighborhood.update)");
   }
   if (!bf.empty() && !bd.empty())

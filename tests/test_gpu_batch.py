@@ -77,10 +77,12 @@ def run_info(ctx, scs, k, together, kinds=None, **kw):
 
 def viscous_mix(seed=7):
     # config-4-like scenes: three with WKBB18 implicit viscosity, one with
-    # WKBB18 and the gradient correction, one with explicit viscosity, two
-    # with axial gravity (one of them also WKBB18), so every setup form and
-    # both step ends are in one batch
-    mix = small_mix(7, seed=seed)
+    # WKBB18 and the gradient correction, one with pt16, one with explicit
+    # viscosity, two with axial gravity (one of them also WKBB18), so every
+    # setup form and every step end are in one batch
+    mix = small_mix(8, seed=seed)
+    mix[7].implicit_viscosity = 3
+    mix[7].fluid_viscosity = mix[7].boundary_viscosity = 10.0
     for j, mu in ((1, 5.0), (3, 20.0), (4, 60.0)):
         mix[j].implicit_viscosity = 1
         mix[j].fluid_viscosity = mix[j].boundary_viscosity = mu
@@ -96,10 +98,12 @@ def viscous_mix(seed=7):
 
 @pytest.mark.parametrize("real_type", [0, 1], ids=["fp32", "fp64"])
 def test_batch_with_viscous_scenes(hip_ctx, real_type):
-    """WKBB18 scenes are stepped inside the batch: after the batched IISPH
-    solve, their gradient correction (wkbb18_gc scenes), diagonal and CG
-    set-up, the PCG iterations (each scene's FP64 CG control and its own
-    convergence), apply and the integrations run over all of them at once. Scenes with explicit viscosity or axial gravity take
+    """Viscous scenes are stepped inside the batch: after the batched IISPH
+    solve, the WKBB18 scenes' gradient correction (wkbb18_gc scenes),
+    diagonal and CG set-up, PCG iterations (each scene's FP64 CG control and
+    its own convergence), apply and integrations run over all of them at
+    once, and the pt16 scenes' set-up, six scalar PCG solves and
+    integrations over theirs. Scenes with explicit viscosity or axial gravity take
     their three setup launches over their own blocks. Every scene matches its
     result alone bit for bit, with the same IISPH and WKBB18 iteration counts
     (the last step crosses a host-visible permutation)."""
@@ -132,9 +136,10 @@ def test_batch_viscous_split_cg_equals_alone(hip_ctx):
 
 
 def test_batch_with_unbatchable_scenes(hip_ctx):
-    """A list that mixes batchable scenes with scenes the batch does not take
-    (pt16, per-procedure kernels) and an FP64 scene: every scene still
-    matches its result alone."""
+    """A list that mixes batched scenes (pt16 and gradient-corrected WKBB18
+    among them) with a scene the batch does not take (per-procedure kernels)
+    and an FP64 scene (a batch of its own): every scene still matches its
+    result alone."""
     mix = small_mix(5, seed=5)
     mix[1].implicit_viscosity = 3
     mix[1].fluid_viscosity = mix[1].boundary_viscosity = 10.0

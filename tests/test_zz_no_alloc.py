@@ -40,9 +40,12 @@ def test_noalloc_step_equals_step():
     ctx = hip.Context(0)
     try:
         mix = [scenes.random_scene(13, i, 2000, 12000, 4) for i in range(3)]
-        # one scene with WKBB18 implicit viscosity (its PCG runs batched too)
+        # one scene with WKBB18 implicit viscosity and one with pt16 (their
+        # solves run batched too)
         mix[1].implicit_viscosity = 1
         mix[1].fluid_viscosity = mix[1].boundary_viscosity = 10.0
+        mix[2].implicit_viscosity = 3
+        mix[2].fluid_viscosity = mix[2].boundary_viscosity = 10.0
         pairs = [_pair(ctx, sc) for sc in mix]
         ref = [p[0] for p in pairs]
         new = [p[1] for p in pairs]
@@ -82,8 +85,8 @@ def test_noalloc_step_refuses_what_would_allocate():
         sc = scenes.tiny(6, steps=2)
         fresh = hip.HipScene(ctx, sc)  # no grid yet: a step would allocate it
         assert _noalloc(ctx, [fresh], 1) == hip.INVALID_ARGUMENT
-        # pt16 viscosity is stepped scene by scene, not batched
-        visc = hip.HipScene(ctx, scenes.tiny(6, steps=2, implicit_viscosity=3))
+        # the per-procedure step is stepped scene by scene, not batched
+        visc = hip.HipScene(ctx, scenes.tiny(6, steps=2), step_mode=1)
         visc.prepare()
         assert _noalloc(ctx, [visc], 1) == hip.INVALID_ARGUMENT
         ok = hip.HipScene(ctx, sc)
