@@ -147,10 +147,18 @@ def test_steps_config0(hip_ctx, oracle_lib, nsteps, step_mode, real_type):
         g.step(1)
         its_o.append(info["iisph_iterations"])
         its_g.append(g.step_info()["iisph_iterations"])
-    diff = np.abs(np.array(its_o) - np.array(its_g))
-    assert diff.max() <= 1, (its_o, its_g)
-    _compare_state(o, g, {"position": 1e-5, "velocity": 2e-3, "density": 2e-3,
-                          "pressure": 2e-2}, f"after {nsteps} steps")
+    # the same IISPH iteration count in every step (the kernels and the oracle
+    # are deterministic; measured equal in all three modes), and bars about
+    # ten times the measured errors (position 4-7e-7, velocity 3-5e-6,
+    # density 6-9e-6, pressure 2-3e-4), the density bar a tenth of the
+    # IISPH stopping tolerance, so a solve that stopped early would show
+    assert its_o == its_g, (its_o, its_g)
+    bars = {"position": 5e-6, "velocity": 5e-5, "density": 1e-4, "pressure": 3e-3}
+    errs = _compare_state(o, g, {f: np.inf for f in bars}, f"after {nsteps} steps")
+    print(f"step_mode={step_mode} real_type={real_type} errors after {nsteps} steps: {errs}; "
+          f"iterations oracle {its_o} gpu {its_g}", flush=True)
+    for f, bar in bars.items():
+        assert errs[f] <= bar, (f, errs[f], bar)
 
 
 def test_dense_block_list_overflow(hip_ctx, oracle_lib):
