@@ -65,7 +65,7 @@ def build(san: str = "address") -> str:
     os.makedirs(OBJ, exist_ok=True)
     if os.path.exists(TREE):
         shutil.rmtree(TREE)
-    ign = shutil.ignore_patterns(".git", "_work", "gpurun_out", "__pycache__", "_build", "*.so",
+    ign = shutil.ignore_patterns(".git", "_work", "__pycache__", "_build", "*.so",
                                  "emu")
     for d in ("prtcl_amd", "tests", "oracle", "include"):
         shutil.copytree(os.path.join(ROOT, d), os.path.join(TREE, d), ignore=ign)
