@@ -149,6 +149,11 @@ void launch(const char *name, void (*k)(P...), dim3 g, dim3 b, size_t, hipStream
 #define __shfl_xor(v, m, ...) ::emu::shfl_xor_(v, m, __LINE__, __FILE__, ##__VA_ARGS__)
 #define __ballot(p) ::emu::wave_op(::emu::kOpBallot, (p) ? 1u : 0u, 0, 64, __LINE__, __FILE__)
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
+// a wavefront barrier: every live lane of the wave reaches it before any goes
+// on (a ballot rendezvous); fences order nothing on one host thread
+#define __builtin_amdgcn_wave_barrier() \
+  ((void)::emu::wave_op(::emu::kOpBallot, 0u, 0, 64, __LINE__, __FILE__))
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_rsqf(x) (1.0f / std::sqrt(static_cast<float>(x)))
 #define __builtin_amdgcn_rcpf(x) (1.0f / static_cast<float>(x))
 
