@@ -92,6 +92,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", type=int, default=-1,
                     help="force the schedule mask of the main scene (-1: autotune)")
+    ap.add_argument("--batch-schedule", type=int, default=-1,
+                    help="force the schedule mask of batched steps (-1: autotune)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--batch-scenes", type=int, default=64,
@@ -157,6 +159,8 @@ def main():
     g.destroy()
 
     if args.workload == "dambreak_1m":
+        if args.batch_schedule >= 0:
+            ctx.set_batch_schedule(args.batch_schedule)
         if args.batch_scenes > 0:
             out["batched"] = bench_batch(ctx, args.batch_scenes, args.batch_steps, args.warmup)
         if args.batch_viscous_scenes > 0:
@@ -594,6 +598,7 @@ def bench_batch(ctx, count, steps, warmup, scs=None, name=None):
         "per_scene_p99_step_ms": round(1e3 * float(np.percentile(lat_p, 99)), 4),
         "max_iisph_iterations_mean": round(float(np.mean(its_b)), 2),
     }
+    out["schedule_mask"] = ctx.batch_schedule()
     if nvisc:
         out["viscous_scenes"] = nvisc
         out["max_wkbb18_iterations_mean"] = round(float(np.mean(wks_b)), 2)

@@ -1853,10 +1853,13 @@ class SceneImpl final : public SceneBase {
     }
     PRTCL_HIP_CHECK(hipGetLastError());
     if (tune) {
-      int it = 0;
-      for (const Progress &p : prog)
-        it = std::max(it, p.iters);
-      tp.resolve(it);
+      // the schedules are compared on the iterations every scene of the
+      // batch runs (below the smallest min_it): later launches skip the
+      // blocks of scenes already done, so their cost depends on the step
+      int all_run = 1 << 30;
+      for (SceneImpl *sc : v)
+        all_run = std::min(all_run, static_cast<int>(sc->desc.iisph_min_iterations));
+      tp.resolve(std::max(fold_first ? 2 : 1, all_run));
       tn.record(0, tp.ms[kStageSetup], static_cast<double>(tp.launches[kStageSetup]));
       tn.record(1, tp.ms[kStagePressureAccel],
                 static_cast<double>(tp.launches[kStagePressureAccel]));
