@@ -98,7 +98,9 @@ def parse():
                     help="target length of the CPU baseline sample")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the batched workload (0 skips it)")
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=100,
+                    help="timed steps of the batched workloads (the 64-scene batch runs "
+                         "100 steps, as its configuration; the served mix at most 30)")
     ap.add_argument("--batch-viscous-scenes", type=int, default=32,
                     help="scenes of the batched mix with WKBB18 scenes (0 skips it)")
     ap.add_argument("--no-viscous", action="store_true",
@@ -167,7 +169,7 @@ def main():
             # the served stream's scene mix: 30 % with WKBB18, stepped batched
             vs = scenes.config4_scenes(args.batch_viscous_scenes)
             out["batched_viscous"] = bench_batch(
-                ctx, len(vs), min(args.batch_steps, 10), min(args.warmup, 3), scs=vs,
+                ctx, len(vs), min(args.batch_steps, 30), min(args.warmup, 3), scs=vs,
                 name=f"served_mix{len(vs)}")
         if not args.no_viscous:
             # config 2: the 250k viscous column (WKBB18 PCG every step)
