@@ -1524,8 +1524,9 @@ class SceneImpl final : public SceneBase {
       for (unsigned b = 0; b < nt; ++b)
         tlist.push_back((k << kBatchLocalBits) | b);
     }
-    // the scenes whose step ends with WKBB18 (their own table and block
-    // maps: all of them, and those without and with the gradient correction)
+    // the scenes whose step ends with WKBB18 or pt16, with tables and block
+    // maps of their own: the WKBB18 scenes all together and, apart, those
+    // without and with the gradient correction; the pt16 scenes
     std::vector<unsigned> vsc, psc;  // WKBB18 scenes, pt16 scenes
     std::vector<uint32_t> vlist, vlist_n, vlist_g, ptlist;
     for (unsigned k = 0; k < ns; ++k) {
@@ -1562,7 +1563,8 @@ class SceneImpl final : public SceneBase {
     const unsigned np3 = static_cast<unsigned>(plist3.size());
     const unsigned np1 = np3 > 0 ? static_cast<unsigned>(plist1.size()) : 0u;
     // pinned / device table layout: [sort | setup | solve] entries, the
-    // WKBB18 entries, pmap, tmap, vmap, the setup maps
+    // WKBB18 entries, pmap, tmap, vmap, the setup maps, the WKBB18 split
+    // maps, the pt16 entries and map
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t tab_bytes = align(sizeof(BatchEntry<T>) * ns);
     const size_t off_visc = 3 * tab_bytes;
