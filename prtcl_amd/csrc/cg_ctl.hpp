@@ -35,10 +35,7 @@ __device__ __forceinline__ void store_partials(double *partials, unsigned nblock
 // the sum of n partials in block order (block_sum: valid in thread 0's
 // wavefront only)
 __device__ __forceinline__ double sum_partials(const double *p, unsigned n) {
-  double acc = 0.0;
-  for (unsigned k = threadIdx.x; k < n; k += kBlock)
-    acc += p[k];
-  return block_sum<kBlock>(acc);
+  return block_sum<kBlock>(strided_sum<kBlock>(p, n));
 }
 
 // the same sum (thread 0's value) in every thread of the block

@@ -171,4 +171,48 @@ __device__ __forceinline__ T block_max(T v) {
   return r;
 }
 
+// A thread's share of the block partials p[0, n): the terms k = threadIdx.x,
+// threadIdx.x + BS, ... in that order, summed (max: their maximum, at least
+// m0). The loads of a batch of 16 terms are all issued before the first add,
+// so a thread waits for one memory round trip per batch and not per term
+// (the partials come from other XCDs, a miss in this XCD's L2). The order of
+// the adds is that of the plain strided loop, so the sums are bit-identical.
+template <int BS, class T>
+__device__ __forceinline__ T strided_sum(const T *p, unsigned n) {
+  constexpr unsigned kBatch = 16;
+  T acc = T(0);
+  for (unsigned k0 = threadIdx.x; k0 < n; k0 += kBatch * BS) {
+    T t[kBatch];
+#pragma unroll
+    for (unsigned u = 0; u < kBatch; ++u) {
+      const unsigned k = k0 + u * BS;
+      t[u] = p[k < n ? k : n - 1];
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kBatch; ++u)
+      if (k0 + u * BS < n)
+        acc += t[u];
+  }
+  return acc;
+}
+
+template <int BS>
+__device__ __forceinline__ double strided_max(const double *p, unsigned n, double m0) {
+  constexpr unsigned kBatch = 16;
+  double m = m0;
+  for (unsigned k0 = threadIdx.x; k0 < n; k0 += kBatch * BS) {
+    double t[kBatch];
+#pragma unroll
+    for (unsigned u = 0; u < kBatch; ++u) {
+      const unsigned k = k0 + u * BS;
+      t[u] = p[k < n ? k : n - 1];
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kBatch; ++u)
+      if (k0 + u * BS < n)
+        m = t[u] > m ? t[u] : m;
+  }
+  return m;
+}
+
 }  // namespace prtcl_amd

@@ -812,10 +812,7 @@ __device__ __forceinline__ void k_pressure_finalize_body(Ctl *c, const double *p
                                                         unsigned nparts) {
   if (c->done)
     return;
-  double acc = 0.0;
-  for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
-    acc += partials[k];
-  acc = block_sum<kBlock>(acc);
+  const double acc = block_sum<kBlock>(strided_sum<kBlock>(partials, nparts));
   if (threadIdx.x == 0) {
     c->aprde = acc;
     const int it = c->iters + 1;
@@ -893,10 +890,7 @@ __device__ __forceinline__ void k_speed_finalize_body(Ctl *c, const double *part
                                                       unsigned nparts, unsigned seq) {
   if (seq != 0u && !integrate_now(c, seq))
     return;
-  double m = 0.0;
-  for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
-    m = partials[k] > m ? partials[k] : m;
-  m = block_max<kBlock>(m);
+  const double m = block_max<kBlock>(strided_max<kBlock>(partials, nparts, 0.0));
   if (threadIdx.x == 0) {
     max_speed_atomic<T>(c, static_cast<T>(m));
     if (seq != 0u)

@@ -158,9 +158,7 @@ __global__ __launch_bounds__(kScanBS) void k_scan_down(uint32_t *count, uint32_t
                                                        const uint32_t *tile_sums,
                                                        uint32_t *start) {
   const unsigned bid = blockIdx.x;
-  uint32_t p = 0;
-  for (unsigned t = threadIdx.x; t < bid; t += kScanBS)
-    p += tile_sums[t];
+  uint32_t p = strided_sum<kScanBS>(tile_sums, bid);
   uint32_t all;
   block_excl_scan(p, &all);  // the block total, in every thread
   p = all;

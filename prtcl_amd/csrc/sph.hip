@@ -377,10 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const doub
                                                               unsigned nparts) {
   if (kSolve && c->done)
     return;
-  double acc = 0.0;
-  for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
-    acc += partials[k];
-  acc = block_sum<kBlock>(acc);
+  const double acc = block_sum<kBlock>(strided_sum<kBlock>(partials, nparts));
   if (threadIdx.x == 0) {
     if (kSolve) {
       c->aprde = acc;
@@ -403,10 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_finalize(Ctl *c, const doub
 template <class T>
 __device__ __forceinline__ void speed_finalize_body(Ctl *c, const double *partials,
                                                     unsigned nparts) {
-  double m = 0.0;
-  for (unsigned k = threadIdx.x; k < nparts; k += kBlock)
-    m = partials[k] > m ? partials[k] : m;
-  m = block_max<kBlock>(m);
+  const double m = block_max<kBlock>(strided_max<kBlock>(partials, nparts, 0.0));
   if (threadIdx.x == 0) {
     atomic_max_speed<T>(c, static_cast<T>(m));
     publish_step_end(c);
