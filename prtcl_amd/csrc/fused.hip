@@ -539,8 +539,11 @@ __device__ __forceinline__ void k_setup_body(const SceneView<T> &S, const NbrLis
 
 template <class T, bool kWithSource, bool kStage>
 __global__ __launch_bounds__(kBlock) void k_setup(SceneView<T> S, NbrList L,
-                                                  vec4_t<T> *vout) {
+                                                  vec4_t<T> *vout, SolveBegin B) {
   k_setup_body<T, kWithSource, kStage>(S, L, vout, xcd_block(blockIdx.x, gridDim.x));
+  // no thread of the setup reads the control block
+  if (B.seq != 0u && blockIdx.x == 0 && threadIdx.x == 0)
+    begin_solve(S.ctl, B);
 }
 
 // explicit viscosity (viscosity.hpp:171-240) over the list, then
@@ -1005,7 +1008,8 @@ __global__ __launch_bounds__(kBlock) void k_speed_finalize_b(const BatchEntry<T>
 
 template <class T>
 void FusedOps<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
-                        bool explicit_viscosity, vec4_t<T> *vtmp, bool lds) {
+                        bool explicit_viscosity, vec4_t<T> *vtmp, bool lds,
+                        const SolveBegin &B) {
   if (S.nf <= 0)
     return;
   const dim3 gr(grid_blocks(S.nf)), bl(kBlock);
@@ -1014,15 +1018,15 @@ void FusedOps<T>::setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
   // integrated velocities first
   if (!explicit_viscosity && !S.axial) {
     if (lds)
-      hipLaunchKernelGGL((k_setup<T, true, true>), gr, bl, 0, st, S, L, vtmp);
+      hipLaunchKernelGGL((k_setup<T, true, true>), gr, bl, 0, st, S, L, vtmp, B);
     else
-      hipLaunchKernelGGL((k_setup<T, true, false>), gr, bl, 0, st, S, L, vtmp);
+      hipLaunchKernelGGL((k_setup<T, true, false>), gr, bl, 0, st, S, L, vtmp, B);
     return;
   }
   if (lds)
-    hipLaunchKernelGGL((k_setup<T, false, true>), gr, bl, 0, st, S, L, vtmp);
+    hipLaunchKernelGGL((k_setup<T, false, true>), gr, bl, 0, st, S, L, vtmp, B);
   else
-    hipLaunchKernelGGL((k_setup<T, false, false>), gr, bl, 0, st, S, L, vtmp);
+    hipLaunchKernelGGL((k_setup<T, false, false>), gr, bl, 0, st, S, L, vtmp, B);
   if (explicit_viscosity)
     hipLaunchKernelGGL(k_viscosity_list<T>, gr, bl, 0, st, S, L, vtmp);
   else

@@ -100,8 +100,10 @@ struct ScheduleTuner {
 template <class T>
 struct FusedOps {
   // density, neighbour list, IISPH c / AA / s, integrated velocity into vtmp
+  // B.seq != 0: the first kernel also begins solve B (one launch fewer)
   static void setup(hipStream_t st, const SceneView<T> &S, const NbrList &L,
-                    bool explicit_viscosity, vec4_t<T> *vtmp, bool lds);
+                    bool explicit_viscosity, vec4_t<T> *vtmp, bool lds,
+                    const SolveBegin &B = SolveBegin{});
   // the pressure acceleration goes to the second half of S.xa only
   static void accel(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);
   static void pressure(hipStream_t st, const SceneView<T> &S, const NbrList &L, bool lds);

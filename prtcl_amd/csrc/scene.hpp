@@ -108,6 +108,39 @@ __device__ __forceinline__ void publish_progress(Ctl *c) {
   }
 }
 
+// A new solve's control state. iters0 = 1: iteration 1 is taken (by the
+// setup pass) and, with min_it >= 2 and max_it >= 2, its convergence step
+// could not stop the solve, so the count starts at 1 without that step (its
+// aprde is never read: the solve reports the last iteration's)
+__device__ __forceinline__ void begin_solve(Ctl *c, double tol, int min_it, int max_it,
+                                            unsigned seq, int iters0 = 0) {
+  c->aprde = 0.0;
+  c->rel = 0.0;
+  c->tol = tol;
+  c->iters = iters0;
+  c->done = (c->nprde >= 1) ? 0 : 1;
+  c->converged = 0;
+  c->min_it = min_it;
+  c->max_it = max_it;
+  c->ticket = 0u;
+  c->seq = seq;
+  publish_progress(c);
+}
+
+// the begin of a solve carried by a launch that runs before the solve's
+// first iteration (seq 0: none); nprde >= 0 is set first
+struct SolveBegin {
+  double tol = 0.0;
+  int min_it = 0, max_it = 0, nprde = -1, iters0 = 0;
+  unsigned seq = 0u;
+};
+
+__device__ __forceinline__ void begin_solve(Ctl *c, const SolveBegin &B) {
+  if (B.nprde >= 0)
+    c->nprde = B.nprde;
+  begin_solve(c, B.tol, B.min_it, B.max_it, B.seq, B.iters0);
+}
+
 struct GridDims {
   int nx, ny, nz;     // cells per axis (dense box)
   int ox, oy, oz;     // integer cell index of the box origin

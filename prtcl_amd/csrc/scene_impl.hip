@@ -214,25 +214,6 @@ __global__ void k_iota(int32_t *id, int n) {
 enum CtlOp { kCtlInit = 0, kCtlBeginSolve = 1, kCtlSetNprde = 2, kCtlSetAprde = 3,
              kCtlSetMaxSpeed = 4, kCtlSetNprdeValue = 5, kCtlBeginSolveAfterFirst = 6 };
 
-// iters0 = 1: iteration 1 is taken (by the setup pass) and, with min_it >= 2
-// and max_it >= 2, its convergence step could not stop the solve, so the
-// count starts at 1 without that step (its aprde is never read: the solve
-// reports the last iteration's)
-__device__ void begin_solve(Ctl *c, double tol, int min_it, int max_it, unsigned seq,
-                            int iters0 = 0) {
-  c->aprde = 0.0;
-  c->rel = 0.0;
-  c->tol = tol;
-  c->iters = iters0;
-  c->done = (c->nprde >= 1) ? 0 : 1;
-  c->converged = 0;
-  c->min_it = min_it;
-  c->max_it = max_it;
-  c->ticket = 0u;
-  c->seq = seq;
-  publish_progress(c);
-}
-
 __global__ void k_ctl(Ctl *c, int op, double dv, int i0, int i1, int i2, int i3,
                       unsigned long long *mirror) {
   switch (op) {
@@ -1180,18 +1161,22 @@ class SceneImpl final : public SceneBase {
   // predicted iterations (and again behind any further ones) in its guarded
   // form, so it runs as soon as the device decides convergence instead of
   // after the host has seen the decision; on return it is enqueued.
+  // begun != 0: the fused setup launch has begun solve `begun` with these
+  // arguments, so no control launch is needed
   int solve_impl(double tol, int min_it, int max_it, int *iters, double *rel, bool fused,
                  const FusedSchedule &sc = FusedSchedule{}, bool tune = false,
-                 int nprde = -1, bool first_done = false, bool integrate = false) {
+                 int nprde = -1, bool first_done = false, bool integrate = false,
+                 unsigned begun = 0u) {
     require_grid();
     hipStream_t st = res_->stream;
-    const unsigned seq = ++seq_;
+    const unsigned seq = begun != 0u ? begun : ++seq_;
     // iteration 1 taken by the setup needs its convergence step only where
     // it could end the solve
     const bool first = first_done && nf_ > 0 && max_it > 0;
     const bool preset = first && first_iteration_cannot_stop(min_it, max_it);
-    ctl_op(preset ? kCtlBeginSolveAfterFirst : kCtlBeginSolve, tol, min_it, max_it,
-           static_cast<int>(seq), nprde);
+    if (begun == 0u)
+      ctl_op(preset ? kCtlBeginSolveAfterFirst : kCtlBeginSolve, tol, min_it, max_it,
+             static_cast<int>(seq), nprde);
     const SceneView<T> S = view();
     const NbrList L = nbr_list();
     Profiler &pf = res_->prof;
@@ -1328,6 +1313,7 @@ class SceneImpl final : public SceneBase {
       // the fused setup takes IISPH iteration 1 (no neighbours needed at p = 0)
       const bool first_iter =
           fused && !sc.first_split && nf_ > 0 && desc.iisph_max_iterations > 0;
+      SolveBegin sb;
       if (fused) {
         // one neighbour pass for density + list + IISPH setup (three with
         // explicit viscosity); the integrated velocity lands in tmpV4
@@ -1336,8 +1322,18 @@ class SceneImpl final : public SceneBase {
         const int q = tprof_.begin(st, kStageSetup);
         SceneView<T> Sv = view();
         Sv.first_iter = first_iter ? 1 : 0;
+        // the setup also begins the step's IISPH solve (solve_impl's control
+        // launch, with the same arguments)
+        if (nf_ > 0) {
+          sb.tol = desc.iisph_tolerance;
+          sb.min_it = desc.iisph_min_iterations;
+          sb.max_it = desc.iisph_max_iterations;
+          sb.nprde = nf_;
+          sb.iters0 = first_iter && first_iteration_cannot_stop(sb.min_it, sb.max_it) ? 1 : 0;
+          sb.seq = ++seq_;
+        }
         FusedOps<T>::setup(st, Sv, nbr_list(), desc.explicit_viscosity != 0, P_.tmpV4,
-                           sc.setup_lds);
+                           sc.setup_lds, sb);
         tprof_.end(st, q);
         pf.end(st, r);
         std::swap(P_.v, P_.tmpV4);
@@ -1372,7 +1368,7 @@ class SceneImpl final : public SceneBase {
                                   nf_ > 0 && desc.iisph_max_iterations > 0;
       int rc = solve_impl(desc.iisph_tolerance, desc.iisph_min_iterations,
                           desc.iisph_max_iterations, &it, &rel, fused, sc, tune, nf_,
-                          first_iter, spec_integrate);
+                          first_iter, spec_integrate, sb.seq);
       if (fused)
         tuner_.finish_step();
       if (rc != PRTCL_HIP_OK)
