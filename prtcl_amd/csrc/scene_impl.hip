@@ -210,6 +210,16 @@ __global__ void k_iota(int32_t *id, int n) {
     id[s] = s;
 }
 
+// a permute: the ids move to keep (the permutation the host may ask for)
+// and the slots become the new ids
+__global__ void k_take_ids(int32_t *id, int32_t *keep, int n) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < n) {
+    keep[s] = id[s];
+    id[s] = s;
+  }
+}
+
 // control-block operations (kernels, not memsets, so they can live in graphs)
 enum CtlOp { kCtlInit = 0, kCtlBeginSolve = 1, kCtlSetNprde = 2, kCtlSetAprde = 3,
              kCtlSetMaxSpeed = 4, kCtlSetNprdeValue = 5, kCtlBeginSolveAfterFirst = 6 };
@@ -764,18 +774,30 @@ class SceneImpl final : public SceneBase {
   // permutation (new host index s held old host index id[s]) is copied to
   // page-locked memory, where last_permutation() finds it. Used by the
   // procedure and by the permutes inside steps; allocates nothing.
+  // the ids stay in HBM (permd) until the host asks for the permutation
+  // (last_permutation) or the scene is evicted, so a permute inside a step
+  // costs one device pass and no transfer over PCIe
   void host_permute(hipStream_t st) {
-    if (nf_ > 0) {
-      PRTCL_HIP_CHECK(hipMemcpyAsync(perm_host_, P_.id, 4 * size_t(nf_), hipMemcpyDeviceToHost, st));
-      hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nf_)), dim3(kBlock), 0, st, P_.id, nf_);
-    }
-    if (nb_ > 0) {
-      PRTCL_HIP_CHECK(hipMemcpyAsync(perm_host_ + nf_, P_.bid, 4 * size_t(nb_),
-                                     hipMemcpyDeviceToHost, st));
-      hipLaunchKernelGGL(k_iota, dim3(grid_blocks(nb_)), dim3(kBlock), 0, st, P_.bid, nb_);
-    }
+    if (nf_ > 0)
+      hipLaunchKernelGGL(k_take_ids, dim3(grid_blocks(nf_)), dim3(kBlock), 0, st, P_.id,
+                         P_.permd, nf_);
+    if (nb_ > 0)
+      hipLaunchKernelGGL(k_take_ids, dim3(grid_blocks(nb_)), dim3(kBlock), 0, st, P_.bid,
+                         P_.permd + nf_, nb_);
     permuted_ = true;
-    perm_in_flight_ = true;
+    perm_on_device_ = true;
+  }
+
+  // the last permutation from HBM into its page-locked home
+  void fetch_permutation() {
+    if (!perm_on_device_)
+      return;
+    hipStream_t st = res_->stream;
+    const size_t n = size_t(nf_) + size_t(nb_);
+    if (n > 0)
+      PRTCL_HIP_CHECK(hipMemcpyAsync(perm_host_, P_.permd, 4 * n, hipMemcpyDeviceToHost, st));
+    PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+    perm_on_device_ = false;
   }
 
   void last_permutation(int group, uint32_t *perm) override {
@@ -785,10 +807,7 @@ class SceneImpl final : public SceneBase {
         perm[i] = static_cast<uint32_t>(i);
       return;
     }
-    if (perm_in_flight_) {
-      PRTCL_HIP_CHECK(hipStreamSynchronize(res_->stream));
-      perm_in_flight_ = false;
-    }
+    fetch_permutation();  // resident: evict() fetches it first
     const int32_t *src = perm_host_ + (group == PRTCL_HIP_FLUID ? 0 : nf_);
     for (int i = 0; i < n; ++i)
       perm[i] = static_cast<uint32_t>(src[i]);
@@ -2069,6 +2088,7 @@ class SceneImpl final : public SceneBase {
       return;
     hipStream_t st = res_->stream;
     PRTCL_HIP_CHECK(hipStreamSynchronize(st));
+    fetch_permutation();  // while evicted, last_permutation reads the host copy
     graphs_.clear();  // they point into the arena
     host_arena_.resize(arena_bytes_);
     PRTCL_HIP_CHECK(hipMemcpy(host_arena_.data(), arena_, arena_bytes_, hipMemcpyDeviceToHost));
@@ -2194,6 +2214,7 @@ class SceneImpl final : public SceneBase {
     BlockRanges *nblk;
     V4 *bxV, *btmpV4;
     int32_t *bid, *btmpI;
+    int32_t *permd;  // the ids at the last permute: fluid, then boundary
     uint32_t *bcell, *brank, *bperm;
     double *partials, *cg_part;
     Ctl *ctl;
@@ -2233,6 +2254,7 @@ class SceneImpl final : public SceneBase {
     A.take(P.btmpV4, b);
     A.take(P.bid, b);
     A.take(P.btmpI, b);
+    A.take(P.permd, std::max<size_t>(1, static_cast<size_t>(f) + static_cast<size_t>(b)));
     A.take(P.bcell, b);
     A.take(P.brank, b);
     A.take(P.bperm, b);
@@ -2264,7 +2286,7 @@ class SceneImpl final : public SceneBase {
                   &P_.gc, &P_.tvg, &P_.prho, &P_.psrho, &P_.pone, &P_.odiag, &P_.vdiag,
                   &P_.scx, &P_.scr, &P_.scp, &P_.scq, &P_.scy, &P_.pcoef})
       f(reinterpret_cast<void **>(q));
-    for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
+    for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI, &P_.permd})
       f(reinterpret_cast<void **>(q));
     for (uint32_t **q : {&P_.fcell, &P_.frank, &P_.fperm, &P_.ninfo, &P_.bcell, &P_.brank,
                          &P_.bperm})
@@ -2308,7 +2330,7 @@ class SceneImpl final : public SceneBase {
                   &P_.gc, &P_.tvg, &P_.prho, &P_.psrho, &P_.pone, &P_.odiag, &P_.vdiag,
                   &P_.scx, &P_.scr, &P_.scp, &P_.scq, &P_.scy, &P_.pcoef})
       Arena::rebase(*q, base);
-    for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI})
+    for (int32_t **q : {&P_.id, &P_.tmpI, &P_.bid, &P_.btmpI, &P_.permd})
       Arena::rebase(*q, base);
     Arena::rebase(P_.nidx, base);
     Arena::rebase(P_.nblk, base);
@@ -2377,7 +2399,7 @@ class SceneImpl final : public SceneBase {
   double last_rel_ = 0;
   int last_wkbb_ = 0;
   int32_t *perm_host_ = nullptr;  // last host-visible permutation (page-locked)
-  bool permuted_ = false, perm_in_flight_ = false;
+  bool permuted_ = false, perm_on_device_ = false;
 };
 
 }  // namespace
